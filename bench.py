@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark of the PT-SA square-packing hot path on one MI355X.
+
+Workload (the plain GPU PT-SA configuration): n = 50 unit squares, 256 replicas
+on one geometric temperature ladder 1e-5..5e-2, translation and rotation step
+sigma 0.05 adapted to 30 % acceptance, Philox seed 7, centres uniform in
+[0.5, L0-0.5]^2 with L0 = ceil(sqrt(50)) + 1 = 9, angles uniform, replica
+exchange every 10 sweeps, box shrink 1e-4 on zero overlap, final FP64 polish
+and certified-side export.
+
+One step = 1000 sweeps of every replica (100 exchange intervals of 10 sweeps,
+run in the plain form: one launch sequence, host synchronisation after every
+interval).  One sweep = n proposals per replica, so one step is
+256 * 50 * 1000 = 1.28e7 proposals.  --steps K times exactly K steps after
+--warmup W untimed steps.
+
+Prints one JSON line: proposals/s, the sweep kernel's roofline entry (FP32
+VALU: algorithmic FLOP per proposal 2*(6C + 28P) + 45 with C and P counted on
+the device), and the CPU baseline (the oracle's FP64 C++ PT-SA on a bounded
+sample of the same workload, timed on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "squares-in-squares_amd"))
+
+N, REPLICAS, L0 = 50, 256, float(math.ceil(math.sqrt(50)) + 1)
+SWEEPS_PER_STEP, SPE = 1000, 10
+PEAK_FP32_TFLOPS = 157.3
+
+
+def c2_params(sis, sweeps_total):
+    return sis.default_params(tmin=1e-5, tmax=5e-2, move_sigma=0.05, rot_sigma=0.05, target_accept=0.3,
+                              shrink=1e-4, replicas=REPLICAS, ladder=0, sweeps=sweeps_total,
+                              sweeps_per_exchange=SPE, polish_iters=200, rng_seed=7, init_mode=1)
+
+
+def cpu_baseline(target_s: float = 12.0):
+    """The oracle's C++ FP64 PT-SA (reference semantics) on the same workload
+    shape, bounded to about target_s seconds of host time."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+
+    o = oracle_lib.load()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    probe = 20
+    r = o.ptsa(n=N, L0=L0, replicas=REPLICAS, tmin=1e-5, tmax=5e-2, sweeps=probe, swap_every=SPE, shrink=1e-4,
+               seed=7, threads=threads)
+    per_sweep = max(r["seconds"], 1e-6) / probe
+    sweeps = int(max(probe, min(20000, target_s / per_sweep)) // SPE * SPE)
+    r = o.ptsa(n=N, L0=L0, replicas=REPLICAS, tmin=1e-5, tmax=5e-2, sweeps=sweeps, swap_every=SPE, shrink=1e-4,
+               seed=7, threads=threads)
+    return {"value": r["proposals"] / r["seconds"], "unit": "proposals/s", "cores": threads, "kind": "port",
+            "sample": f"n=50, 256 replicas, {sweeps} sweeps ({r['proposals']} proposals) of the FP64 C++ "
+                      f"PT-SA in {r['seconds']:.2f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(instances shard across GPUs with no inter-GPU traffic; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import sisamd as sis
+
+    cus, dev = sis.device_info()
+    total_steps = a.warmup + a.steps
+    p = c2_params(sis, total_steps * SWEEPS_PER_STEP)
+    sess = sis.BatchSession([(N, L0, None)], p)
+    intervals_per_step = SWEEPS_PER_STEP // SPE
+    for _ in range(a.warmup):
+        sess.run(intervals_per_step, mode=0)
+    t0 = time.perf_counter()
+    sweep_ms = 0.0
+    launches = 0
+    for _ in range(a.steps):
+        sm, nl, _ = sess.run(intervals_per_step, mode=0)
+        sweep_ms += sm
+        launches += nl
+    wall = time.perf_counter() - t0
+    res = sess.finalize()[0]
+    sess.close()
+
+    timed_props = a.steps * SWEEPS_PER_STEP * N * REPLICAS
+    value = timed_props / wall
+    # algorithmic FLOP per proposal from the device counters (whole run)
+    c_per = res.candidates / res.proposals
+    p_per = res.near_pairs / res.proposals  # old + new position
+    flop_per_prop = 12.0 * c_per + 28.0 * p_per + 45.0
+    achieved = timed_props * flop_per_prop / (sweep_ms * 1e-3) / 1e12 if sweep_ms > 0 else None
+    line = {
+        "metric": "MC proposals/s; p50/p99 time-to-certified-L, served batched vs plain GPU",
+        "value": value,
+        "unit": "proposals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded uniform random initial packings)",
+        "config": {
+            "workload": "plain PT-SA on GPU: n=50, 256 replicas, geometric ladder 1e-5..5e-2, sigma 0.05/0.05 "
+                        "adapted to 30% acceptance, exchange every 10 sweeps, Philox seed 7, FP64 polish at end",
+            "n": N, "replicas": REPLICAS, "L0": L0, "sweeps_per_step": SWEEPS_PER_STEP,
+            "sweeps_timed": a.steps * SWEEPS_PER_STEP, "schedule": "plain (host sync per interval)",
+            "device": dev, "compute_units": cus,
+        },
+        "roofline": {
+            "bound": "valu", "kernel": "k_sweep",
+            "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
+            "traffic": None,
+            "flop_per_proposal": flop_per_prop, "C_per_proposal": c_per, "P_per_proposal": p_per,
+            "sweep_kernel_ms_avg": sweep_ms / max(1, launches), "sweep_launches": launches,
+        },
+        "quality": {
+            "feasible": res.feasible, "target_L": res.target_L, "L_cert": res.L_cert, "shrinks": res.shrinks,
+            "e32": res.e32, "e64_before_polish": res.e64_before, "e32_vs_e64_gap": res.e_gap,
+            "accept_rate": res.accepted / res.proposals,
+            "exchange_rate": res.exchanges_accepted / max(1, res.exchanges_tried),
+        },
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

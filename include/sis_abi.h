@@ -1,0 +1,182 @@
+/*
+ * squares-in-squares on MI355X: C ABI of the PT-SA square-packing engine.
+ *
+ * The reference (koboshchan/squares-in-squares) exposes this path only as a
+ * process: `packing <seed.txt> <epochs> <steps> <chains> <rng_seed> <out.json>`
+ * (cuda/packing.cu:768-868), driven through subprocess by cuda/cpp_loop.py:62-63,
+ * cuda/full_run.py:50-51, cuda/scheduler.py:86-87, cuda/batch.py:31,
+ * cuda/demo.py:84-86 and cuda/autotune.py:18.  Each entry point below replaces
+ * one piece of that process contract; the `packing` binary built from this
+ * library keeps the argv/env/JSON contract itself.
+ *
+ * Conventions: plain pointers and sizes; inputs are caller-owned and only read
+ * during the call; outputs go to caller-allocated buffers.  Every function
+ * returns an int status: SIS_OK (0), SIS_EINVAL (2, the CLI's exit code for bad
+ * input, cuda/packing.cu:769-775), SIS_EDEVICE (3, a HIP error), SIS_PENDING (1,
+ * sis_poll only).  sis_last_error() gives the message of the calling thread's
+ * last failure.  No C++ exception crosses this boundary.
+ */
+#ifndef SIS_ABI_H
+#define SIS_ABI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIS_OK 0
+#define SIS_PENDING 1
+#define SIS_EINVAL 2
+#define SIS_EDEVICE 3
+
+/* One packing instance: n unit squares in a box of side L.
+ * Replaces the seed file `n L` + n lines `x y angle` (cuda/packing.cu:772-775). */
+typedef struct sis_instance {
+    int n;              /* 1..1000 */
+    double L;           /* (0, 64] */
+    const double* q;    /* n*3 (x, y, angle in radians) or NULL for uniform random init */
+} sis_instance;
+
+/* Search parameters.  Replaces argv epochs/steps/chains/rng_seed and the
+ * PACK_* environment knobs (cuda/packing.cu:769-786). */
+typedef struct sis_params {
+    double tmin, tmax;          /* geometric temperature ladder (PACK_TMIN/PACK_TMAX) */
+    double move_sigma;          /* translation step sigma (PACK_MOVE) */
+    double rot_sigma;           /* rotation step sigma, radians (PACK_ROTATION) */
+    double target_accept;       /* acceptance the step sizes adapt to */
+    double shrink;              /* L decrement on a zero-overlap configuration (PACK_SHRINK) */
+    double cell;                /* cell width of the colour checkerboard, >= 0.7075 */
+    double init_jitter;         /* uniform jitter applied to seed configurations */
+    int64_t sweeps;             /* sweeps per replica; one sweep = n proposals */
+    int replicas;               /* replicas (chains) per instance, 1..4096 */
+    int ladder;                 /* temperatures per ladder; 0 = one ladder over all replicas */
+    int sweeps_per_exchange;    /* sweeps between replica-exchange steps */
+    int replicas_per_group;     /* replicas per workgroup tile; 0 = autotuned */
+    int polish_iters;           /* FP64 conjugate-gradient polish iterations at the end */
+    int init_mode;              /* 0: seed configuration in every replica; 1: uniform random */
+    int flags;                  /* reserved, 0 */
+    uint64_t rng_seed;          /* Philox key */
+} sis_params;
+
+/* Result of one instance.  Replaces raw{n}.json (cuda/packing.cu:859-865). */
+typedef struct sis_result {
+    double* q_out;              /* caller-owned n*3 (x, y, angle): translated, dilated export */
+    double target_L;            /* box side at which the exported configuration had zero overlap */
+    double L_cert;              /* bounding side of q_out (FP64 certified-side test) */
+    double lambda;              /* centre dilation applied by the certified-side test */
+    double e32;                 /* FP32 energy of the source configuration */
+    double e64_before;          /* FP64 energy of the same configuration before polish */
+    double e64_after;           /* FP64 energy after polish */
+    double e_gap;               /* |e32 - e64_before| */
+    uint64_t proposals;         /* attempted single-square moves */
+    uint64_t accepted;
+    uint64_t sweeps;            /* sweeps per replica completed */
+    uint64_t exchanges_tried, exchanges_accepted;
+    uint64_t candidates;        /* stencil neighbours examined (C) */
+    uint64_t near_pairs;        /* neighbours with d^2 < 2, old + new position (P) */
+    int shrinks;                /* number of box shrinks */
+    int feasible;               /* 1 if a zero-overlap configuration was found */
+    int replicas;               /* replicas actually used */
+    int status;                 /* SIS_OK or an error code */
+    double seconds;             /* host wall time of the solve */
+    double device_seconds;      /* time of the sweep + exchange kernels (HIP events) */
+} sis_result;
+
+/* Server configuration for the continuous-batching request loop. */
+typedef struct sis_server_cfg {
+    double hbm_budget_bytes;    /* admission cap on resident replica state */
+    int max_instances;          /* resident instances at once */
+    int intervals_per_round;    /* exchange intervals per device round */
+    int certifier_threads;      /* host threads that certify finished packings */
+    int certify;                /* 1: certify every result (dilation ladder + exact certifier) */
+} sis_server_cfg;
+
+typedef struct sis_server sis_server;
+
+/* Library set-up on a device (no reference counterpart: the reference binds
+ * device 0 implicitly, cuda/packing.cu:787). */
+int sis_init(int device);
+const char* sis_last_error(void);
+/* Compute units and marketing name of the device in use (the reference prints
+ * prop.multiProcessorCount and prop.name into raw.json, cuda/packing.cu:862-863). */
+int sis_device_info(int* compute_units, char* name, int name_len);
+
+/* Defaults of cuda/packing.cu:777-786 mapped to this engine. */
+void sis_default_params(sis_params* p);
+
+/* Plain form: one instance per launch sequence with a host synchronisation
+ * every exchange interval, as the reference's epoch loop
+ * (cuda/packing.cu:823-853) driven once per n by cuda/cpp_loop.py:47-80. */
+int sis_solve(const sis_instance* inst, const sis_params* params, sis_result* out);
+
+/* Batched form: `count` instances of any n, all replica tiles in one launch per
+ * interval, the whole schedule captured into a HIP graph.  Replaces the
+ * reference's one-process-per-n pool (cuda/cpp_loop.py:82-106). */
+int sis_solve_batch(const sis_instance* insts, int count, const sis_params* params,
+                    sis_result* out);
+
+/* Step-wise control of resident instances: the reference's epoch loop
+ * (cuda/packing.cu:823-853) opened up so a caller can run, time and finish
+ * it in pieces.  mode 0 = plain (host synchronisation after every interval),
+ * mode 1 = HIP graph (no host synchronisation).  intervals <= 0 runs what is
+ * left of the budget.  sweep_ms is the HIP-event time of the sweep kernel
+ * launches (plain mode; 0 in graph mode), total_ms that of the whole run. */
+typedef struct sis_batch sis_batch;
+sis_batch* sis_batch_create(const sis_instance* insts, int count, const sis_params* params);
+int sis_batch_run(sis_batch* b, int64_t intervals, int mode, double* sweep_ms, int64_t* sweep_launches,
+                  double* total_ms);
+int sis_batch_finalize(sis_batch* b, sis_result* out);
+int sis_batch_destroy(sis_batch* b);
+
+/* Serving: requests of any n admitted while others run (continuous batching).
+ * Replaces cuda/cpp_loop.py:82-106 (ThreadPoolExecutor + VRAM admission). */
+sis_server* sis_server_create(const sis_server_cfg* cfg);
+int sis_submit(sis_server* s, const sis_instance* inst, const sis_params* params,
+               uint64_t* ticket);
+/* SIS_OK when finished (out filled; out->q_out must hold n*3 doubles),
+ * SIS_PENDING while queued or running. */
+int sis_poll(sis_server* s, uint64_t ticket, sis_result* out);
+/* Seconds from submit to a certified result, and the certified L (50 decimals). */
+int sis_ticket_times(sis_server* s, uint64_t ticket, double* queued_s, double* done_s,
+                     double* certified_s, char* L50, int L50_len);
+int sis_server_destroy(sis_server* s);
+
+/* Exact certificate of one packing: decimal strings in, 50-decimal L out.
+ * Replaces the certifier process boundary (cuda/certifier.cpp:933-1077,
+ * called by cuda/cpp_loop.py:17-28); semantics of
+ * cuda/certifier.cpp:738-912 (accept contact, reject any interior overlap).
+ * Returns SIS_OK and writes L (>= 52 bytes) or SIS_EINVAL with the reason. */
+int sis_certify_strings(int n, const char* const* xs, const char* const* ys,
+                        const char* const* angles, char* L50, int L50_len);
+/* Dilation-ladder export of a raw configuration then certification
+ * (cuda/demo.py:25-40): tries centre factors 1.000002, 1.00001, 1.0001.
+ * On success writes the accepted factor and the 50-decimal L. */
+int sis_export_certify(int n, const double* q, double* dilation, char* L50, int L50_len);
+
+/* ---- device parity entry points (hot-path kernels on caller data) ---- */
+
+/* SAT depth of pairs: in[8k..8k+5] = dx, dy, ci, si, cj, sj (dx = x_j - x_i).
+ * The reference predicate is cuda/packing.cu:26-34 (depth), restated with
+ * cached cos/sin as in cuda/packing.cu:177-189.  Writes depth and depth^2. */
+int sis_dev_depth32(int npairs, const float* in, float* depth, float* depth2);
+/* Cell coordinate clamp(floor((x + shift) / w), lo, hi); with w = 2, shift = 0,
+ * lo = 0, hi = 15 this is the reference's binning (cuda/packing.cu:22-25). */
+int sis_dev_cellcoord(int count, const float* x, float shift, float w, int lo, int hi,
+                      int* out);
+/* Run `sweeps` sweeps of the FP32 sweep kernel on R replicas of one instance,
+ * in place: xycs = R*n*4 floats (x, y, cos, sin).  temps/steps per replica.
+ * Returns per-replica accepted moves and FP32 energies after the sweeps.
+ * Semantics: cuda/packing.cu:52-95 (walk). */
+int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
+                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
+                   uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,
+                   uint64_t counts[3]);
+/* FP64 polish + certified side of one configuration (cuda/packing.cu:624-697):
+ * q in/out n*3; writes the exported (translated, dilated) configuration. */
+int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,
+                   double* energy_after, double* lam, double* L_cert, double* q_export);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,827 @@
+/*
+ * CPU oracle for the PT-SA square-packing path (test infrastructure only; see
+ * sis_oracle.h).  Built with g++ -O2 -ffp-contract=off so that every rounding
+ * is the one written here.
+ *
+ * Parity status per group (what pins each one; the tests live under tests/):
+ *  - Philox4x32-10: Random123 known-answer vectors + rocRAND's open header.
+ *  - reference FP32 geometry (o_ref_*): the reference's own selftest corpus
+ *    (mt19937(923)) against the FP64 axis oracle, tolerance 2e-6, and the
+ *    cell-list vs all-pairs identity (1e-4), as cuda/tests/selftest.cu:53.
+ *  - product FP32 formulation (o_*_cs): closed form R = 1/2(1+|cos|+|sin|)
+ *    equals the reference's per-axis radius; axis-aligned reduction to
+ *    1-D interval overlap; brute-force tiny cases.
+ *  - FP64 pair/boundary/gradient: seeded_test corpus (mt19937(832)) vs the
+ *    independent axis oracle (1e-12) and central finite differences (1e-7),
+ *    as cuda/tests/seeded_test.cu:30.
+ *  - certified side: every emitted configuration passes the reference's
+ *    interval certifier (oracle/_ref); lambda >= 1 and L >= sqrt(n) - 1e-7.
+ *  - CG polish: energy never increases (accepts only strictly lower energy).
+ *  - PT-SA engine: forced-exchange invariant, detailed-balance histogram on a
+ *    two-level toy, shrink bookkeeping; trajectories are parity unpinned
+ *    against the reference (cuRAND streams cannot be reproduced).
+ *  - sweep model (o_sweep32_model): a CPU restatement of the product's own
+ *    colour-checkerboard sweep; pinned by FP64 recomputation of every
+ *    accepted move's energy change and by exhaustive-neighbour checks.
+ */
+#include "sis_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10, from Salmon, Moraes, Dror, Shaw, "Parallel random numbers: */
+/* as easy as 1, 2, 3", SC'11: ten rounds of two 32x32->64 multiplies with   */
+/* a Weyl-sequence key schedule.                                             */
+/* ------------------------------------------------------------------------ */
+void o_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+/* (k + 1/2) * 2^-23 for the top 23 bits k: exact, symmetric, in (0, 1). */
+float o_u01f(uint32_t r) { return ((float)(r >> 9) + 0.5f) * (1.0f / 8388608.0f); }
+
+double o_u01d(uint32_t hi, uint32_t lo) {
+    uint64_t k = ((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11);
+    k &= ((1ull << 52) - 1);
+    return ((double)k + 0.5) * (1.0 / 4503599627370496.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference FP32 geometry.                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* cuda/packing.cu:22-25: 16x16 grid of width 2, clamped to the edge cells. */
+int o_ref_cell(float x, float y) {
+    int cx = (int)floorf(x / 2);
+    int cy = (int)floorf(y / 2);
+    cx = cx < 0 ? 0 : (cx > 15 ? 15 : cx);
+    cy = cy < 0 ? 0 : (cy > 15 ? 15 : cy);
+    return cy * 16 + cx;
+}
+
+/* cuda/packing.cu:26-34: penetration depth along the four edge normals, each
+ * with its own projected radius built from four |projections|. */
+float o_ref_depth32(const float p[3], const float q[3]) {
+    const float cp = cosf(p[2]), sp = sinf(p[2]);
+    const float cq = cosf(q[2]), sq = sinf(q[2]);
+    const float nx[4] = {cp, -sp, cq, -sq};
+    const float ny[4] = {sp, cp, sq, cq};
+    float best = 1e10f;
+    for (int k = 0; k < 4; ++k) {
+        float half = .5f * (fabsf(nx[k] * cp + ny[k] * sp) + fabsf(-nx[k] * sp + ny[k] * cp) +
+                            fabsf(nx[k] * cq + ny[k] * sq) + fabsf(-nx[k] * sq + ny[k] * cq));
+        float sep = fabsf((p[0] - q[0]) * nx[k] + (p[1] - q[1]) * ny[k]);
+        best = fminf(best, half - sep);
+    }
+    return fmaxf(0.f, best);
+}
+
+/* cuda/packing.cu:35-39: squared excursion of the rotated square's extent
+ * beyond the box [0, L]^2, per axis, measured from the box centre. */
+float o_ref_wall32(const float p[3], float L) {
+    float ext = .5f * (fabsf(cosf(p[2])) + fabsf(sinf(p[2])));
+    float ox = fmaxf(0.f, fabsf(p[0] - L / 2) + ext - L / 2);
+    float oy = fmaxf(0.f, fabsf(p[1] - L / 2) + ext - L / 2);
+    return ox * ox + oy * oy;
+}
+
+/* cuda/packing.cu:40-47 through the list build at :62-64, and the all-pairs
+ * sum of cuda/tests/selftest.cu:25-29.  Lists are built in ascending index
+ * order with head insertion, which is one of the orders the reference's
+ * atomicExch insertion can produce. */
+void o_ref_local32(int n, const float* q, float L, float* local_out, float* full_out) {
+    std::vector<int> head(256, -1), next(n, -1);
+    for (int i = 0; i < n; ++i) {
+        int c = o_ref_cell(q[3 * i], q[3 * i + 1]);
+        next[i] = head[c];
+        head[c] = i;
+    }
+    for (int i = 0; i < n; ++i) {
+        const float* p = q + 3 * i;
+        float e = o_ref_wall32(p, L);
+        int c = o_ref_cell(p[0], p[1]);
+        int cx = c % 16, cy = c / 16;
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                int xx = cx + ox, yy = cy + oy;
+                if (xx < 0 || xx >= 16 || yy < 0 || yy >= 16) continue;
+                for (int j = head[yy * 16 + xx]; j != -1; j = next[j])
+                    if (j != i) {
+                        float d = o_ref_depth32(p, q + 3 * j);
+                        e += d * d;
+                    }
+            }
+        local_out[i] = e;
+        float full = o_ref_wall32(p, L);
+        for (int j = 0; j < n; ++j)
+            if (j != i) {
+                float d = o_ref_depth32(p, q + 3 * j);
+                full += d * d;
+            }
+        full_out[i] = full;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The product's FP32 formulation.  The device kernels evaluate exactly these */
+/* operations in this order, with explicit fma, so results agree bit for bit. */
+/* ------------------------------------------------------------------------ */
+
+int o_cellcoord(float x, float shift, float w, int lo, int hi) {
+    int k = (int)floorf((x + shift) / w);
+    return k < lo ? lo : (k > hi ? hi : k);
+}
+
+/* Linear SAT depth: the four edge normals of both squares share the projected
+ * radius R = 1/2 (1 + |cos(ti - tj)| + |sin(ti - tj)|); the depth is
+ * max(0, R - max_k |d . n_k|).  Same value as cuda/packing.cu:26-34 in exact
+ * arithmetic; dx, dy = centre of j minus centre of i. */
+static inline float sat_gap32(float dx, float dy, float ci, float si, float cj, float sj) {
+    float cd = fmaf(ci, cj, si * sj);
+    float sd = fmaf(si, cj, -(ci * sj));
+    float R = fmaf(0.5f, fabsf(cd) + fabsf(sd), 0.5f);
+    float a1 = fabsf(fmaf(dx, ci, dy * si));
+    float a2 = fabsf(fmaf(dy, ci, -(dx * si)));
+    float a3 = fabsf(fmaf(dx, cj, dy * sj));
+    float a4 = fabsf(fmaf(dy, cj, -(dx * sj)));
+    float m = fmaxf(fmaxf(a1, a2), fmaxf(a3, a4));
+    return R - m;
+}
+
+float o_depth32_cs(float dx, float dy, float ci, float si, float cj, float sj) {
+    float d2 = fmaf(dx, dx, dy * dy);
+    if (!(d2 < 2.0f)) return 0.0f;
+    float g = sat_gap32(dx, dy, ci, si, cj, sj);
+    return g > 0.0f ? g : 0.0f;
+}
+
+float o_depth2_32_cs(float dx, float dy, float ci, float si, float cj, float sj) {
+    float d2 = fmaf(dx, dx, dy * dy);
+    float g = sat_gap32(dx, dy, ci, si, cj, sj);
+    return (d2 < 2.0f && g > 0.0f) ? g * g : 0.0f;
+}
+
+float o_wall32_cs(float x, float y, float c, float s, float L) {
+    float ext = 0.5f * (fabsf(c) + fabsf(s));
+    float h = 0.5f * L;
+    float ox = fmaxf(0.0f, (fabsf(x - h) + ext) - h);
+    float oy = fmaxf(0.0f, (fabsf(y - h) + ext) - h);
+    return fmaf(ox, ox, oy * oy);
+}
+
+/* Rotate (c, s) by t with Taylor polynomials (|t| <= 0.8: truncation below
+ * 2e-10) and one Newton step back onto the unit circle. */
+void o_rot32(float c, float s, float t, float* c_out, float* s_out) {
+    float t2 = t * t;
+    float sp = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f),
+                             -1.6666667e-1f), 1.0f);
+    float sn = t * sp;
+    float cs = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.4801587e-5f, -1.3888889e-3f), 4.1666667e-2f),
+                             -0.5f), 1.0f);
+    float c2 = fmaf(c, cs, -(s * sn));
+    float s2 = fmaf(s, cs, c * sn);
+    float nn = fmaf(c2, c2, s2 * s2);
+    float k = fmaf(-0.5f, nn, 1.5f);
+    *c_out = c2 * k;
+    *s_out = s2 * k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference FP64 geometry.                                                 */
+/* ------------------------------------------------------------------------ */
+
+static inline double sgn(double v) { return (double)((v > 0) - (v < 0)); }
+static inline double sgn_dead(double v) { return fabs(v) > 1e-6 ? (v >= 0 ? 1.0 : -1.0) : 0.0; }
+
+/* cuda/packing.cu:168-172: angle modulo pi/2 into [0, pi/2). */
+double o_wrap_angle(double a) {
+    const double quarter = 1.57079632679489661923;
+    a = fmod(a, quarter);
+    if (a < 0.0) a += quarter;
+    return a;
+}
+
+/* Squared SAT penetration of p by q plus d/dp of it, with cached cos/sin.
+ * Energy: cuda/packing.cu:177-189; gradient of the active axis:
+ * cuda/seeded.cu:40-66 (hard sign) and cuda/packing.cu:191-214 (sign with a
+ * 1e-6 dead zone, `smooth`). */
+static double pair64_impl(const double p[3], const double q[3], double c, double s, double d,
+                          double t, double grad[3], bool smooth) {
+    const double dx = q[0] - p[0], dy = q[1] - p[1];
+    if (dx * dx + dy * dy >= 2.000001) return 0.0;
+    const double cd = c * d + s * t, sd = s * d - c * t;
+    const double R = .5 + .5 * (fabs(cd) + fabs(sd));
+    const double dR = smooth ? .5 * (-sgn_dead(cd) * sd + sgn_dead(sd) * cd)
+                             : .5 * (-sgn(cd) * sd + sgn(sd) * cd);
+    const double nx[4] = {c, -s, d, -t}, ny[4] = {s, c, t, d};
+    double gap = -1e100, gx = 0, gy = 0, ga = 0;
+    for (int k = 0; k < 4; ++k) {
+        const double v = dx * nx[k] + dy * ny[k];
+        const double z = fabs(v) - R;
+        if (z > gap) {
+            gap = z;
+            const double sv = smooth ? (v >= 0.0 ? 1.0 : -1.0) : sgn(v);
+            gx = -sv * nx[k];
+            gy = -sv * ny[k];
+            ga = -dR + (k < 2 ? sv * (-dx * ny[k] + dy * nx[k]) : 0.0);
+        }
+    }
+    if (gap >= 0.0) return 0.0;
+    if (grad) {
+        grad[0] += 2.0 * gap * gx;
+        grad[1] += 2.0 * gap * gy;
+        grad[2] += 2.0 * gap * ga;
+    }
+    return gap * gap;
+}
+
+/* cuda/seeded.cu:88-90: pair() computes the trig itself. */
+double o_pair64(const double p[3], const double q[3], double grad[3]) {
+    return pair64_impl(p, q, cos(p[2]), sin(p[2]), cos(q[2]), sin(q[2]), grad, false);
+}
+
+double o_pair64_smooth_cs(const double p[3], const double q[3], double c, double s, double d,
+                          double t, double grad[3]) {
+    return pair64_impl(p, q, c, s, d, t, grad, true);
+}
+
+/* cuda/seeded.cu:92-113: four one-sided wall terms and their gradient. */
+double o_boundary64(const double p[3], double L, double grad[3]) {
+    const double c = cos(p[2]), s = sin(p[2]);
+    const double r = .5 * (fabs(c) + fabs(s));
+    const double v[4] = {r - p[0], p[0] + r - L, r - p[1], p[1] + r - L};
+    const double dR = .5 * (-sgn_dead(c) * s + sgn_dead(s) * c);
+    double e = 0;
+    for (int k = 0; k < 4; ++k)
+        if (v[k] > 0) {
+            e += v[k] * v[k];
+            if (grad) {
+                const double f = 2 * v[k];
+                grad[0] += f * (k == 0 ? -1 : k == 1 ? 1 : 0);
+                grad[1] += f * (k == 2 ? -1 : k == 3 ? 1 : 0);
+                grad[2] += f * dR;
+            }
+        }
+    return e;
+}
+
+/* cuda/tests/seeded_test.cu:5-11: independent axis oracle, squared depth. */
+double o_sat_oracle64(const double p[3], const double q[3]) {
+    const double c = cos(p[2]), s = sin(p[2]), d = cos(q[2]), t = sin(q[2]);
+    const double ax[4][2] = {{c, s}, {-s, c}, {d, t}, {-t, d}};
+    double gap = -1e100;
+    for (auto& a : ax) {
+        double r = .5 * (fabs(a[0] * c + a[1] * s) + fabs(-a[0] * s + a[1] * c) +
+                         fabs(a[0] * d + a[1] * t) + fabs(-a[0] * t + a[1] * d));
+        gap = std::max(gap, fabs((p[0] - q[0]) * a[0] + (p[1] - q[1]) * a[1]) - r);
+    }
+    double depth = std::max(0.0, -gap);
+    return depth * depth;
+}
+
+/* cuda/tests/selftest.cu:7-16: FP64 axis oracle on FP32 inputs, linear depth. */
+double o_sat_oracle_depth_f(const float p[3], const float q[3]) {
+    const double c = cos((double)p[2]), s = sin((double)p[2]);
+    const double d = cos((double)q[2]), t = sin((double)q[2]);
+    const double ax[4][2] = {{c, s}, {-s, c}, {d, t}, {-t, d}};
+    double v = 1e9;
+    for (auto& a : ax) {
+        double r = .5 * (fabs(a[0] * c + a[1] * s) + fabs(-a[0] * s + a[1] * c) +
+                         fabs(a[0] * d + a[1] * t) + fabs(-a[0] * t + a[1] * d));
+        v = std::min(v, r - fabs(((double)p[0] - q[0]) * a[0] + ((double)p[1] - q[1]) * a[1]));
+    }
+    return std::max(0.0, v);
+}
+
+/* cuda/packing.cu:240-252: walls (one side per axis) plus all pairs i<j. */
+double o_total64_cs(int n, const double* x, const double* y, const double* c, const double* s,
+                    double L) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double r = 0.5 * (fabs(c[i]) + fabs(s[i]));
+        if (x[i] < r) { double u = r - x[i]; e += u * u; }
+        else if (x[i] > L - r) { double u = x[i] + r - L; e += u * u; }
+        if (y[i] < r) { double u = r - y[i]; e += u * u; }
+        else if (y[i] > L - r) { double u = y[i] + r - L; e += u * u; }
+        const double pi[3] = {x[i], y[i], 0.0};
+        for (int j = i + 1; j < n; ++j) {
+            const double pj[3] = {x[j], y[j], 0.0};
+            e += pair64_impl(pi, pj, c[i], s[i], c[j], s[j], nullptr, false);
+        }
+    }
+    return e;
+}
+
+double o_total64(int n, const double* q, double L) {
+    std::vector<double> x(n), y(n), c(n), s(n);
+    for (int i = 0; i < n; ++i) {
+        x[i] = q[3 * i]; y[i] = q[3 * i + 1];
+        c[i] = cos(q[3 * i + 2]); s[i] = sin(q[3 * i + 2]);
+    }
+    return o_total64_cs(n, x.data(), y.data(), c.data(), s.data(), L);
+}
+
+/* cuda/packing.cu:254-297: smallest uniform centre dilation lambda that puts
+ * every near pair (d^2 < 2.05) at least tau = 2.6e-8 beyond contact on its
+ * best axis; reject lambda > 1.008; then the bounding side of the dilated
+ * configuration, and (when it improves current_best) the translated output. */
+double o_certified_side(int n, const double* q, double current_best, double* out,
+                        double* lam_out) {
+    const double tau = 2.6e-8;
+    std::vector<double> c(n), s(n);
+    for (int i = 0; i < n; ++i) { c[i] = cos(q[3 * i + 2]); s[i] = sin(q[3 * i + 2]); }
+    double lam = 1.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = q[3 * i] - q[3 * j], dy = q[3 * i + 1] - q[3 * j + 1];
+            if (dx * dx + dy * dy >= 2.05) continue;
+            const double R = 0.5 + 0.5 * (fabs(c[i] * c[j] + s[i] * s[j]) +
+                                          fabs(-s[i] * c[j] + c[i] * s[j]));
+            const double v = std::max(std::max(fabs(dx * c[i] + dy * s[i]), fabs(-dx * s[i] + dy * c[i])),
+                                      std::max(fabs(dx * c[j] + dy * s[j]), fabs(-dx * s[j] + dy * c[j])));
+            const double need = v > 1e-12 ? (R + tau) / v : 1e9;
+            if (need > lam) lam = need;
+        }
+    if (lam_out) *lam_out = lam;
+    if (lam > 1.008) return 1e9;
+    double lox = 1e9, hix = -1e9, loy = 1e9, hiy = -1e9;
+    for (int i = 0; i < n; ++i) {
+        const double r = 0.5 * (fabs(c[i]) + fabs(s[i]));
+        const double x = q[3 * i] * lam, y = q[3 * i + 1] * lam;
+        lox = std::min(lox, x - r); hix = std::max(hix, x + r);
+        loy = std::min(loy, y - r); hiy = std::max(hiy, y + r);
+    }
+    const double side = std::max(hix - lox, hiy - loy);
+    if (side < current_best && out)
+        for (int i = 0; i < n; ++i) {
+            out[3 * i] = q[3 * i] * lam - lox;
+            out[3 * i + 1] = q[3 * i + 1] * lam - loy;
+            out[3 * i + 2] = o_wrap_angle(q[3 * i + 2]);
+        }
+    return side;
+}
+
+/* cuda/packing.cu:624-687: Polak-Ribiere+ conjugate gradient on the overlap +
+ * wall energy; restart every 25 iterations; five backtracking trials of
+ * factor 0.45 from step (0.04 initially, x1.25 up to 0.15 on success, x0.5
+ * down to 1e-7 on failure); only strictly lower energies are accepted. */
+int o_polish64(int n, double* q, double L, int iters, double* energy_inout) {
+    std::vector<double> g(3 * n), gold(3 * n, 0.0), dir(3 * n, 0.0), saved(3 * n);
+    double energy = o_total64(n, q, L);
+    double step = 0.04;
+    int it = 0;
+    for (; it < iters; ++it) {
+        if (energy < 1e-24) break;
+        std::vector<double> c(n), s(n);
+        for (int i = 0; i < n; ++i) { c[i] = cos(q[3 * i + 2]); s[i] = sin(q[3 * i + 2]); }
+        double gg = 0, dgg = 0;
+        for (int i = 0; i < n; ++i) {
+            double gi[3] = {0, 0, 0};
+            const double ext = 0.5 * (fabs(c[i]) + fabs(s[i]));
+            const double dR = .5 * (-sgn_dead(c[i]) * s[i] + sgn_dead(s[i]) * c[i]);
+            const double px = q[3 * i], py = q[3 * i + 1];
+            if (px < ext) { double u = ext - px; gi[0] -= 2 * u; gi[2] += 2 * u * dR; }
+            else if (px > L - ext) { double u = px + ext - L; gi[0] += 2 * u; gi[2] += 2 * u * dR; }
+            if (py < ext) { double u = ext - py; gi[1] -= 2 * u; gi[2] += 2 * u * dR; }
+            else if (py > L - ext) { double u = py + ext - L; gi[1] += 2 * u; gi[2] += 2 * u * dR; }
+            for (int j = 0; j < n; ++j)
+                if (j != i) pair64_impl(q + 3 * i, q + 3 * j, c[i], s[i], c[j], s[j], gi, true);
+            for (int k = 0; k < 3; ++k) {
+                g[3 * i + k] = gi[k];
+                gg += gold[3 * i + k] * gold[3 * i + k];
+                dgg += (gi[k] - gold[3 * i + k]) * gi[k];
+            }
+        }
+        const double beta = (it == 0 || gg < 1e-30 || it % 25 == 0) ? 0.0 : std::max(0.0, dgg / gg);
+        double norm2 = 0;
+        for (int k = 0; k < 3 * n; ++k) {
+            dir[k] = -g[k] + beta * dir[k];
+            gold[k] = g[k];
+            saved[k] = q[k];
+            norm2 += dir[k] * dir[k];
+        }
+        if (norm2 < 1e-28) break;
+        double alpha = step;
+        bool ok = false;
+        for (int ls = 0; ls < 5; ++ls) {
+            for (int i = 0; i < n; ++i) {
+                q[3 * i] = saved[3 * i] + alpha * dir[3 * i];
+                q[3 * i + 1] = saved[3 * i + 1] + alpha * dir[3 * i + 1];
+                q[3 * i + 2] = o_wrap_angle(saved[3 * i + 2] + alpha * dir[3 * i + 2]);
+            }
+            const double ne = o_total64(n, q, L);
+            if (ne < energy) {
+                energy = ne;
+                step = std::min(0.15, alpha * 1.25);
+                ok = true;
+                break;
+            }
+            alpha *= 0.45;
+        }
+        if (!ok) {
+            for (int k = 0; k < 3 * n; ++k) { q[k] = saved[k]; dir[k] = 0.0; }
+            step = std::max(1e-7, step * 0.5);
+        }
+    }
+    if (energy_inout) *energy_inout = energy;
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU PT-SA engine (FP64).  Follows the reference's FP64 search semantics:  */
+/* single-square moves with an O(n) energy change (cuda/packing.cu:542-563), */
+/* geometric temperature ladder (:516-517), Metropolis on log u (:555),      */
+/* even/odd exchange within a ladder (:750-765), shrink-and-rescale of the   */
+/* box on a zero-overlap configuration (:833-848).  Sweeps visit the squares */
+/* in index order; steps are uniform of half-width sqrt(3)*sigma, adapted    */
+/* per temperature toward the target acceptance at every exchange point.    */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+struct Replica {
+    std::vector<double> x, y, a, c, s;
+    double E = 0;
+    uint64_t acc = 0, prop = 0, acc_total = 0;
+};
+
+struct SpinBarrier {
+    std::atomic<int> count{0};
+    std::atomic<int> phase{0};
+    int total;
+    explicit SpinBarrier(int t) : total(t) {}
+    void wait() {
+        int ph = phase.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == total) {
+            count.store(0, std::memory_order_relaxed);
+            phase.store(ph + 1, std::memory_order_release);
+        } else {
+            while (phase.load(std::memory_order_acquire) == ph) std::this_thread::yield();
+        }
+    }
+};
+
+double replica_energy(const Replica& r, int n, double L) {
+    return o_total64_cs(n, r.x.data(), r.y.data(), r.c.data(), r.s.data(), L);
+}
+
+double wall64_cs(double x, double y, double c, double s, double L) {
+    const double r = .5 * (fabs(c) + fabs(s));
+    const double v[4] = {r - x, x + r - L, r - y, y + r - L};
+    double e = 0;
+    for (double t : v)
+        if (t > 0) e += t * t;
+    return e;
+}
+
+}  // namespace
+
+int o_ptsa_run(const o_ptsa_cfg* cfg, o_ptsa_out* out) {
+    const int n = cfg->n, R = cfg->replicas;
+    const int ladder = cfg->ladder > 0 ? cfg->ladder : R;
+    if (n < 1 || R < 1 || R % ladder != 0 || cfg->swap_every < 1 || cfg->sweeps < 0) return 2;
+    const uint32_t key[2] = {(uint32_t)cfg->seed, (uint32_t)(cfg->seed >> 32)};
+    double L = cfg->L0;
+    std::vector<double> T(ladder);
+    for (int k = 0; k < ladder; ++k)
+        T[k] = ladder == 1 ? cfg->tmin
+                           : cfg->tmin * pow(cfg->tmax / cfg->tmin, (double)k / (ladder - 1));
+    std::vector<double> sig_xy(ladder, cfg->move_sigma), sig_th(ladder, cfg->rot_sigma);
+    std::vector<int> slot_of(R), rep_of(R);
+    for (int r = 0; r < R; ++r) { slot_of[r] = r % ladder; rep_of[r] = r; }
+
+    std::vector<Replica> reps(R);
+    for (int r = 0; r < R; ++r) {
+        Replica& rp = reps[r];
+        rp.x.resize(n); rp.y.resize(n); rp.a.resize(n); rp.c.resize(n); rp.s.resize(n);
+        for (int i = 0; i < n; ++i) {
+            if (cfg->init) {
+                rp.x[i] = cfg->init[3 * ((size_t)r * n + i)];
+                rp.y[i] = cfg->init[3 * ((size_t)r * n + i) + 1];
+                rp.a[i] = cfg->init[3 * ((size_t)r * n + i) + 2];
+            } else {
+                uint32_t ctr[4] = {(uint32_t)i, (uint32_t)r, 0u, 3u}, o[4];
+                o_philox(ctr, key, o);
+                rp.x[i] = 0.5 + (L - 1.0) * o_u01d(o[0], o[1]);
+                rp.y[i] = 0.5 + (L - 1.0) * o_u01d(o[2], o[3]);
+                uint32_t ctr2[4] = {(uint32_t)i, (uint32_t)r, 1u, 3u};
+                o_philox(ctr2, key, o);
+                rp.a[i] = 1.5707963267948966 * o_u01d(o[0], o[1]);
+            }
+            rp.c[i] = cos(rp.a[i]);
+            rp.s[i] = sin(rp.a[i]);
+        }
+        rp.E = replica_energy(rp, n, L);
+    }
+
+    const int threads = std::max(1, std::min(cfg->threads, R));
+    const int64_t blocks = (cfg->sweeps + cfg->swap_every - 1) / cfg->swap_every;
+    SpinBarrier bar(threads);
+    std::atomic<bool> stop{false};
+    double L_feasible = cfg->L0;
+    bool have_feasible = false;
+    int shrinks = 0;
+    uint64_t ex_try = 0, ex_acc = 0;
+    if (out && out->best_q)
+        for (int i = 0; i < n; ++i) {
+            out->best_q[3 * i] = reps[0].x[i];
+            out->best_q[3 * i + 1] = reps[0].y[i];
+            out->best_q[3 * i + 2] = reps[0].a[i];
+        }
+
+    auto worker = [&](int tid) {
+        for (int64_t b = 0; b < blocks; ++b) {
+            const int64_t s0 = b * cfg->swap_every;
+            const int64_t s1 = std::min<int64_t>(cfg->sweeps, s0 + cfg->swap_every);
+            for (int r = tid; r < R; r += threads) {
+                Replica& rp = reps[r];
+                const int slot = slot_of[r];
+                const double a_xy = std::sqrt(3.0) * sig_xy[slot], a_th = std::sqrt(3.0) * sig_th[slot];
+                const double invT = 1.0 / T[slot];
+                for (int64_t sw = s0; sw < s1; ++sw) {
+                    for (int i = 0; i < n; ++i) {
+                        uint32_t ctr[4] = {(uint32_t)i, (uint32_t)r, (uint32_t)sw,
+                                           (uint32_t)(sw >> 32) << 8 | 4u};
+                        uint32_t o[4];
+                        o_philox(ctr, key, o);
+                        rp.prop++;
+                        const double nx = rp.x[i] + a_xy * (2.0 * o_u01f(o[0]) - 1.0);
+                        const double ny = rp.y[i] + a_xy * (2.0 * o_u01f(o[1]) - 1.0);
+                        if (nx < 0 || nx > L || ny < 0 || ny > L) continue;
+                        const double na = rp.a[i] + a_th * (2.0 * o_u01f(o[2]) - 1.0);
+                        const double nc = cos(na), ns = sin(na);
+                        double dE = wall64_cs(nx, ny, nc, ns, L) -
+                                    wall64_cs(rp.x[i], rp.y[i], rp.c[i], rp.s[i], L);
+                        const double pn[3] = {nx, ny, 0}, po[3] = {rp.x[i], rp.y[i], 0};
+                        for (int j = 0; j < n; ++j) {
+                            if (j == i) continue;
+                            const double pj[3] = {rp.x[j], rp.y[j], 0};
+                            dE += pair64_impl(pn, pj, nc, ns, rp.c[j], rp.s[j], nullptr, false) -
+                                  pair64_impl(po, pj, rp.c[i], rp.s[i], rp.c[j], rp.s[j], nullptr, false);
+                        }
+                        if (dE <= 0.0 || log((double)o_u01f(o[3])) < -dE * invT) {
+                            rp.x[i] = nx; rp.y[i] = ny; rp.a[i] = o_wrap_angle(na);
+                            rp.c[i] = nc; rp.s[i] = ns;
+                            rp.acc++;
+                        }
+                    }
+                }
+                rp.E = replica_energy(rp, n, L);
+            }
+            bar.wait();
+            if (tid == 0) {
+                /* adaptation per temperature slot from the replica that held it */
+                for (int r = 0; r < R; ++r) {
+                    Replica& rp = reps[r];
+                    if (rp.prop) {
+                        double rate = (double)rp.acc / (double)rp.prop;
+                        int k = slot_of[r];
+                        double f = exp((rate - cfg->target_accept) * 0.5);
+                        sig_xy[k] = std::min(0.5, std::max(1e-7, sig_xy[k] * f));
+                        sig_th[k] = std::min(0.45, std::max(1e-7, sig_th[k] * f));
+                    }
+                    rp.acc_total += rp.acc;
+                    rp.acc = 0;
+                }
+                /* even/odd exchange on adjacent temperatures of each ladder */
+                const int parity = (int)(b & 1);
+                for (int base = 0; base < R; base += ladder)
+                    for (int k = parity; k + 1 < ladder; k += 2) {
+                        int ra = rep_of[base + k], rb = rep_of[base + k + 1];
+                        uint32_t ctr[4] = {(uint32_t)(base + k), 0u, (uint32_t)b, 2u}, o[4];
+                        o_philox(ctr, key, o);
+                        ex_try++;
+                        double lhs = log((double)o_u01f(o[0]));
+                        if (lhs < (1.0 / T[k] - 1.0 / T[k + 1]) * (reps[ra].E - reps[rb].E)) {
+                            std::swap(rep_of[base + k], rep_of[base + k + 1]);
+                            slot_of[ra] = k + 1;
+                            slot_of[rb] = k;
+                            ex_acc++;
+                        }
+                    }
+                /* shrink on a zero-overlap replica: keep it, then squeeze */
+                int zero = -1;
+                for (int k = 0; k < R && zero < 0; ++k)
+                    if (reps[rep_of[k]].E == 0.0) zero = rep_of[k];
+                if (zero >= 0) {
+                    if (L < L_feasible || !have_feasible) {
+                        L_feasible = L;
+                        have_feasible = true;
+                        if (out && out->best_q)
+                            for (int i = 0; i < n; ++i) {
+                                out->best_q[3 * i] = reps[zero].x[i];
+                                out->best_q[3 * i + 1] = reps[zero].y[i];
+                                out->best_q[3 * i + 2] = reps[zero].a[i];
+                            }
+                    }
+                    double Ln = std::max(L - cfg->shrink, std::sqrt((double)n));
+                    if (Ln < L) {
+                        double f = Ln / L;
+                        for (auto& rp : reps)
+                            for (int i = 0; i < n; ++i) { rp.x[i] *= f; rp.y[i] *= f; }
+                        L = Ln;
+                        shrinks++;
+                        for (auto& rp : reps) rp.E = replica_energy(rp, n, L);
+                    }
+                }
+            }
+            bar.wait();
+            if (stop.load()) break;
+        }
+    };
+
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    if (out) {
+        uint64_t prop = 0, acc = 0;
+        double emin = 1e300;
+        for (auto& rp : reps) {
+            prop += rp.prop;
+            acc += rp.acc_total + rp.acc;
+            emin = std::min(emin, rp.E);
+        }
+        out->L_final = L;
+        out->L_feasible = L_feasible;
+        out->proposals = prop;
+        out->accepted = acc;
+        out->exchanges_tried = ex_try;
+        out->exchanges_accepted = ex_acc;
+        out->shrinks = shrinks;
+        out->seconds = secs;
+        out->e_min = emin;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU model of the product's FP32 colour-checkerboard sweep.  The product  */
+/* follows the reference's FP32 walk (cuda/packing.cu:52-95: uniform cell    */
+/* list, simultaneous moves only for squares that cannot interact, Metropolis */
+/* on the local energy change) with a 3x3-colour checkerboard of cells of    */
+/* width w >= sqrt(2)/2 (Anderson et al., "Scalable Metropolis Monte Carlo   */
+/* for simulation of hard shapes", Comput. Phys. Commun. 204 (2016) 21-30):  */
+/* moves that leave their cell are rejected, so same-colour cells never      */
+/* interact, and the grid origin is shifted at random every sweep.           */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+constexpr int kRankCap = 4;          /* ranks >= this go to the serial group */
+constexpr int kGroups = 9 * kRankCap;
+
+struct Grid {
+    int m, M;
+    float shx, shy;
+    std::vector<int> kx, ky, cell, start, sorted, rank;
+};
+
+void build_grid(Grid& g, int n, float L, float w, float shx, float shy, const float* x,
+                const float* y) {
+    g.m = (int)floorf((L + w) / w) + 1;
+    g.M = g.m + 4;
+    g.shx = shx; g.shy = shy;
+    g.kx.assign(n, 0); g.ky.assign(n, 0); g.cell.assign(n, 0); g.rank.assign(n, 0);
+    g.start.assign(g.M * g.M + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        g.kx[i] = o_cellcoord(x[i], shx, w, 0, g.m - 1) + 2;
+        g.ky[i] = o_cellcoord(y[i], shy, w, 0, g.m - 1) + 2;
+        g.cell[i] = g.ky[i] * g.M + g.kx[i];
+        g.start[g.cell[i] + 1]++;
+    }
+    for (int c = 0; c < g.M * g.M; ++c) g.start[c + 1] += g.start[c];
+    std::vector<int> fill(g.M * g.M, 0);
+    g.sorted.assign(n, 0);
+    /* canonical order: ascending square index inside each cell */
+    for (int i = 0; i < n; ++i) {
+        g.rank[i] = fill[g.cell[i]]++;
+        g.sorted[g.start[g.cell[i]] + g.rank[i]] = i;
+    }
+}
+
+}  // namespace
+
+int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float* s,
+                    uint64_t counts[3]) {
+    const int n = cfg->n;
+    const float L = cfg->L, w = cfg->w;
+    const uint32_t key[2] = {cfg->key0, cfg->key1};
+    uint32_t o[4];
+    {
+        uint32_t ctr[4] = {0u, cfg->replica, cfg->sweep, 1u};
+        o_philox(ctr, key, o);
+    }
+    const float shx = w * o_u01f(o[0]), shy = w * o_u01f(o[1]);
+    Grid g;
+    build_grid(g, n, L, w, shx, shy, x, y);
+    std::vector<std::vector<int>> groups(kGroups + 1);
+    for (int i = 0; i < n; ++i) {
+        int colour = (g.kx[i] % 3) + 3 * (g.ky[i] % 3);
+        int key_g = g.rank[i] < kRankCap ? g.rank[i] * 9 + colour : kGroups;
+        groups[key_g].push_back(i);
+    }
+    int accepted = 0;
+    uint64_t C = 0, P = 0;
+    for (int gi = 0; gi <= kGroups; ++gi) {
+        /* members of one group cannot interact: any order gives the same result */
+        for (int i : groups[gi]) {
+            uint32_t ctr[4] = {(uint32_t)i, cfg->replica, cfg->sweep, 0u};
+            o_philox(ctr, key, o);
+            const float fx = fmaf(2.0f, o_u01f(o[0]), -1.0f);
+            const float fy = fmaf(2.0f, o_u01f(o[1]), -1.0f);
+            const float ft = fmaf(2.0f, o_u01f(o[2]), -1.0f);
+            const float nx = x[i] + cfg->step_xy * fx;
+            const float ny = y[i] + cfg->step_xy * fy;
+            if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) continue;
+            if (o_cellcoord(nx, g.shx, w, 0, g.m - 1) + 2 != g.kx[i] ||
+                o_cellcoord(ny, g.shy, w, 0, g.m - 1) + 2 != g.ky[i])
+                continue;
+            float nc, ns;
+            o_rot32(c[i], s[i], cfg->step_th * ft, &nc, &ns);
+            float dE = o_wall32_cs(nx, ny, nc, ns, L) - o_wall32_cs(x[i], y[i], c[i], s[i], L);
+            for (int oy = -2; oy <= 2; ++oy) {
+                const int row = (g.ky[i] + oy) * g.M;
+                const int a = g.start[row + g.kx[i] - 2], b = g.start[row + g.kx[i] + 3];
+                for (int k = a; k < b; ++k) {
+                    const int j = g.sorted[k];
+                    if (j == i) continue;
+                    const float dxn = x[j] - nx, dyn = y[j] - ny;
+                    const float dxo = x[j] - x[i], dyo = y[j] - y[i];
+                    const float en = o_depth2_32_cs(dxn, dyn, nc, ns, c[j], s[j]);
+                    const float eo = o_depth2_32_cs(dxo, dyo, c[i], s[i], c[j], s[j]);
+                    dE += en - eo;
+                    C++;
+                    P += (fmaf(dxn, dxn, dyn * dyn) < 2.0f) + (fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+                }
+            }
+            const float u = o_u01f(o[3]);
+            if (dE <= 0.0f || u < exp2f(-dE * cfg->inv_t_log2e)) {
+                x[i] = nx; y[i] = ny; c[i] = nc; s[i] = ns;
+                accepted++;
+            }
+        }
+    }
+    if (counts) {
+        counts[0] += (uint64_t)n;
+        counts[1] += C;
+        counts[2] += P;
+    }
+    return accepted;
+}
+
+float o_energy32_model(int n, float L, float w, const float* x, const float* y, const float* c,
+                       const float* s) {
+    Grid g;
+    build_grid(g, n, L, w, 0.0f, 0.0f, x, y);
+    std::vector<float> e(n);
+    for (int i = 0; i < n; ++i) {
+        float ei = o_wall32_cs(x[i], y[i], c[i], s[i], L);
+        for (int oy = -2; oy <= 2; ++oy) {
+            const int row = (g.ky[i] + oy) * g.M;
+            const int a = g.start[row + g.kx[i] - 2], b = g.start[row + g.kx[i] + 3];
+            for (int k = a; k < b; ++k) {
+                const int j = g.sorted[k];
+                if (j <= i) continue;
+                ei += o_depth2_32_cs(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]);
+            }
+        }
+        e[i] = ei;
+    }
+    for (int stride = 1; stride < n; stride <<= 1)
+        for (int i = 0; i + stride < n; i += 2 * stride) e[i] += e[i + stride];
+    return n > 0 ? e[0] : 0.0f;
+}
+
+/* src/squares/core.py:27-35: half-integer grid with integer pitch. */
+int o_grid_packing(int n, double* q) {
+    if (n < 1) return 2;
+    int width = 1;
+    while (width * width < n) ++width;
+    for (int i = 0; i < n; ++i) {
+        q[3 * i] = (i % width) + 0.5;
+        q[3 * i + 1] = (i / width) + 0.5;
+        q[3 * i + 2] = 0.0;
+    }
+    return 0;
+}

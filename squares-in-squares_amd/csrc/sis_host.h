@@ -1,0 +1,109 @@
+// Host-side types of the runtime (resident batches, plans, results).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sis_abi.h"
+#include "sis_internal.h"
+#include "sis_kernels.h"
+
+namespace sis {
+
+struct DeviceInfo {
+    int id = -1;
+    int cus = 0;
+    size_t lds_per_block = 0;
+    std::string name, arch;
+};
+
+struct InstSpec {
+    int n = 0;
+    double L = 0;
+    std::vector<double> q;      // n*3 or empty
+    sis_params p{};
+};
+
+struct Plan {
+    int G = 1, M = 0, spe = 1, ladder = 1, ntiles = 0;
+    size_t lds = 0;
+};
+
+struct FinalRecord {
+    std::vector<double> q;
+    double target_L = 0, L_cert = 0, lambda = 0, e32 = 0, e64_before = 0, e64_after = 0;
+    InstDesc desc{};
+};
+
+class Batch {
+public:
+    ~Batch();
+    int build(std::vector<InstSpec>&& in);
+    int enqueue_interval();
+    int run(long long intervals, int mode);
+    int run_plain(long long intervals);
+    int run_graph(int per_graph, long long launches);
+    int finalize(std::vector<FinalRecord>& rec);
+    void release();
+
+    std::vector<InstSpec> specs;
+    std::vector<Plan> plans;
+    std::vector<InstDesc> h_inst;
+    std::vector<Tile> h_tiles;
+    size_t n_sq = 0, n_rep = 0, n_arch = 0, max_lds = 0;
+    long long max_intervals = 0, intervals_done = 0;
+    double device_ms = 0, last_ms = 0, sweep_ms = 0;
+    long long sweep_launches = 0;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_intervals = 0;
+
+    InstDesc* d_inst = nullptr;
+    Tile* d_tiles = nullptr;
+    RepState* d_reps = nullptr;
+    SlotState* d_slots = nullptr;
+    int* d_ros = nullptr;
+    float4* d_state = nullptr;
+    float4* d_arch = nullptr;
+    FinalizeJob* d_jobs = nullptr;
+    FinalizeOut* d_fout = nullptr;
+    double* d_qout = nullptr;
+};
+
+int fail(int code, const std::string& msg);
+
+#define SIS_HIPCHECK(x)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return ::sis::fail(SIS_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer released on scope exit (error paths included).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count) { return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1)); }
+    operator T*() const { return p; }
+};
+int ensure_device(int device);
+const DeviceInfo& device_info();
+void default_params(sis_params* p);
+int validate(const sis_instance* in, const sis_params* p);
+void initial_state(const InstSpec& s, std::vector<float4>& out);
+int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
+InstSpec make_spec(const sis_instance* in, const sis_params* p);
+void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);
+const void* sweep_kernel_ptr();
+
+}  // namespace sis

@@ -1,0 +1,78 @@
+// Internal layout shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sis {
+
+constexpr int kMaxN = 1000;
+constexpr int kRankCap = 4;              // occupancy ranks with their own group
+constexpr int kGroups = 9 * kRankCap;    // colour x rank groups; one more = serial group
+constexpr int kSweepThreads = 64;        // sweep workgroup = one wavefront
+constexpr int kLadderThreads = 256;
+constexpr int kPolishThreads = 256;
+constexpr int kPolishKPT = (kMaxN + kPolishThreads - 1) / kPolishThreads;
+constexpr float kMinCell = 0.7075f;      // 2w > sqrt(2) + 6e-4: same colour never interacts
+
+// One resident packing instance.  Written by the host at admission, then
+// owned by the device until the instance finishes.
+struct InstDesc {
+    int n, R, ladder, G;
+    int state_off;              // float4 index of (replica 0, square 0)
+    int rep_off;                // first global replica / temperature-slot index
+    int M;                      // cell-grid side (row stride), sized for the initial L
+    int spe;                    // sweeps per exchange interval
+    long long sweeps_total;
+    long long sweeps_done;
+    double L;                   // current box side
+    double L_arch;              // box side of the archived zero-overlap configuration
+    float Lf;                   // L rounded to float, used by the sweep kernel
+    float scale;                // centre scale applied when the next sweep stages the state
+    float w;                    // cell width
+    float shrink;
+    float target_acc;
+    int arch_off;               // float4 index of the archive
+    uint32_t key0, key1;        // Philox key (request seed)
+    uint32_t interval;          // exchange steps done
+    int arch_valid;
+    int shrinks;
+    int feasible;
+    unsigned long long proposals, accepted, cand, near, ex_tried, ex_acc;
+};
+
+struct Tile {
+    int inst, r0, g, pad;
+};
+
+struct RepState {               // per global replica
+    float E;                    // FP32 energy after the last sweep launch
+    int slot;                   // temperature slot held (0..R-1 within the instance)
+    unsigned acc;               // accepted moves in the last interval
+    unsigned pad;
+};
+
+struct SlotState {              // per global temperature slot
+    float inv_t_log2e;          // log2(e) / T
+    float step_xy;              // half-width of the uniform translation step
+    float step_th;              // half-width of the uniform rotation step
+    float T;
+};
+
+// LDS of one sweep tile (G replicas of n squares, M x M cells).
+__host__ __device__ inline int grid_words(int M) { return (M * M + 3) & ~1; }
+__host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
+    size_t gn = (size_t)G * n;
+    size_t b = 16 * gn;                         // float4 x, y, cos, sin
+    b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
+    b += 2 * gn * 4;                            // sorted, cell coords, rank/key, items
+    b = (b + 15) & ~(size_t)15;
+    b += (size_t)G * 6 * 4;                     // per-replica shift x/y, steps, 1/T, accepts
+    b += 2 * (kGroups + 2) * 4;                 // group counts and starts
+    return b;
+}
+
+__host__ __device__ inline int grid_cells(float L, float w) {
+    return (int)floorf((L + w) / w) + 1;
+}
+
+}  // namespace sis

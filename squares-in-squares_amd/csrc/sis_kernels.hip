@@ -1,0 +1,870 @@
+// gfx950 kernels of the PT-SA square-packing path.
+//
+//  k_sweep     FP32 Metropolis sweeps of replica tiles staged in LDS, colour
+//              checkerboard of cells so that non-interacting squares move at
+//              once (reference semantics: cuda/packing.cu:52-95, `walk`).
+//  k_ladder    per instance: step adaptation, replica exchange by swapping
+//              temperature slots, FP64 zero-overlap test, archive and box
+//              shrink (cuda/packing.cu:141-152, 750-765, 823-853).
+//  k_finalize  FP64 conjugate-gradient polish, certified-side dilation and
+//              export of one configuration per instance (cuda/packing.cu:
+//              254-297, 624-697).
+//  k_depth32 / k_cellcoord   parity probes of the geometry on caller data.
+//
+// All arithmetic whose bits are checked against the CPU oracle is written
+// with explicit fmaf and compiled with contraction off.
+#pragma clang fp contract(off)
+
+#include "sis_internal.h"
+#include "sis_kernels.h"
+
+namespace sis {
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG: Philox4x32-10 (Salmon et al., SC'11).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                        uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+__device__ __forceinline__ float u01(uint32_t r) {
+    return ((float)(r >> 9) + 0.5f) * (1.0f / 8388608.0f);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 geometry.  (dx, dy) = centre j - centre i; (c, s) = cos/sin of angles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cellcoord(float x, float shift, float w, int lo, int hi) {
+    int k = (int)floorf((x + shift) / w);
+    return k < lo ? lo : (k > hi ? hi : k);
+}
+
+__device__ __forceinline__ float sat_gap(float dx, float dy, float ci, float si, float cj,
+                                         float sj) {
+    const float cd = __builtin_fmaf(ci, cj, si * sj);
+    const float sd = __builtin_fmaf(si, cj, -(ci * sj));
+    const float R = __builtin_fmaf(0.5f, fabsf(cd) + fabsf(sd), 0.5f);
+    const float a1 = fabsf(__builtin_fmaf(dx, ci, dy * si));
+    const float a2 = fabsf(__builtin_fmaf(dy, ci, -(dx * si)));
+    const float a3 = fabsf(__builtin_fmaf(dx, cj, dy * sj));
+    const float a4 = fabsf(__builtin_fmaf(dy, cj, -(dx * sj)));
+    const float m = fmaxf(fmaxf(a1, a2), fmaxf(a3, a4));
+    return R - m;
+}
+
+__device__ __forceinline__ float depth2(float dx, float dy, float ci, float si, float cj,
+                                        float sj) {
+    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    const float g = sat_gap(dx, dy, ci, si, cj, sj);
+    return (d2 < 2.0f && g > 0.0f) ? g * g : 0.0f;
+}
+
+__device__ __forceinline__ float wall32(float x, float y, float c, float s, float L) {
+    const float ext = 0.5f * (fabsf(c) + fabsf(s));
+    const float h = 0.5f * L;
+    const float ox = fmaxf(0.0f, (fabsf(x - h) + ext) - h);
+    const float oy = fmaxf(0.0f, (fabsf(y - h) + ext) - h);
+    return __builtin_fmaf(ox, ox, oy * oy);
+}
+
+// Rotate (c, s) by t: Taylor polynomials (|t| <= 0.8) and one Newton step of
+// the norm, so the state carries cos/sin and never calls a transcendental.
+__device__ __forceinline__ void rot32(float c, float s, float t, float& co, float& so) {
+    const float t2 = t * t;
+    const float sp = __builtin_fmaf(
+        t2, __builtin_fmaf(t2, __builtin_fmaf(t2, __builtin_fmaf(t2, 2.7557319e-6f, -1.9841270e-4f),
+                                              8.3333333e-3f), -1.6666667e-1f), 1.0f);
+    const float sn = t * sp;
+    const float cs = __builtin_fmaf(
+        t2, __builtin_fmaf(t2, __builtin_fmaf(t2, __builtin_fmaf(t2, 2.4801587e-5f, -1.3888889e-3f),
+                                              4.1666667e-2f), -0.5f), 1.0f);
+    const float c2 = __builtin_fmaf(c, cs, -(s * sn));
+    const float s2 = __builtin_fmaf(s, cs, c * sn);
+    const float nn = __builtin_fmaf(c2, c2, s2 * s2);
+    const float k = __builtin_fmaf(-0.5f, nn, 1.5f);
+    co = c2 * k;
+    so = s2 * k;
+}
+
+// ---------------------------------------------------------------------------
+// LDS helpers.
+// ---------------------------------------------------------------------------
+// 16-bit counters packed two per 32-bit LDS word (counts never exceed 1000).
+__device__ __forceinline__ unsigned add_u16(uint16_t* base, int idx, unsigned v) {
+    unsigned* w = reinterpret_cast<unsigned*>(base) + (idx >> 1);
+    const unsigned sh = (idx & 1) << 4;
+    return (atomicAdd(w, v << sh) >> sh) & 0xFFFFu;
+}
+
+// Inclusive prefix sum of K u16 values by one wavefront: contiguous chunk per
+// lane, wave scan of the chunk totals, second pass writes.
+__device__ void wave_scan_u16(uint16_t* a, int K) {
+    const int lane = threadIdx.x;
+    const int chunk = (K + kSweepThreads - 1) / kSweepThreads;
+    const int b = lane * chunk, e = min(K, b + chunk);
+    unsigned sum = 0;
+    for (int k = b; k < e; ++k) sum += a[k];
+    unsigned incl = sum;
+#pragma unroll
+    for (int off = 1; off < kSweepThreads; off <<= 1) {
+        unsigned v = __shfl_up(incl, off, kSweepThreads);
+        if (lane >= off) incl += v;
+    }
+    unsigned run = incl - sum;
+    for (int k = b; k < e; ++k) {
+        run += a[k];
+        a[k] = (uint16_t)run;
+    }
+}
+
+struct TileLds {
+    float4* pos;
+    uint16_t* A;      // per replica grid_words(M) cell starts
+    uint16_t* sorted;
+    uint16_t* cellk;  // kx | ky << 8
+    uint16_t* rank;   // rank, then group key
+    uint16_t* items;
+    float* shx;
+    float* shy;
+    float* sxy;
+    float* sth;
+    float* ik;
+    int* acc;
+    int* gcnt;        // kGroups + 2
+    int* gstart;      // kGroups + 2
+};
+
+__device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
+    TileLds t;
+    const int gn = G * n;
+    t.pos = reinterpret_cast<float4*>(smem);
+    t.A = reinterpret_cast<uint16_t*>(t.pos + gn);
+    t.sorted = t.A + (size_t)G * grid_words(M);
+    t.cellk = t.sorted + gn;
+    t.rank = t.cellk + gn;
+    t.items = t.rank + gn;
+    size_t off = (size_t)(reinterpret_cast<unsigned char*>(t.items + gn) - smem);
+    off = (off + 15) & ~(size_t)15;
+    float* f = reinterpret_cast<float*>(smem + off);
+    t.shx = f;
+    t.shy = f + G;
+    t.sxy = f + 2 * G;
+    t.sth = f + 3 * G;
+    t.ik = f + 4 * G;
+    t.acc = reinterpret_cast<int*>(f + 5 * G);
+    t.gcnt = t.acc + G;
+    t.gstart = t.gcnt + (kGroups + 2);
+    return t;
+}
+
+// Cell lists of every replica of the tile in canonical order (ascending
+// square index inside a cell), with the occupancy rank of each square.
+__device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) {
+    const int lane = threadIdx.x;
+    const int gw = grid_words(M);
+    unsigned* A32 = reinterpret_cast<unsigned*>(t.A);
+    for (int k = lane; k < (G * gw) / 2; k += kSweepThreads) A32[k] = 0u;
+    __syncthreads();
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        const int g = q / n;
+        const float4 p = t.pos[q];
+        const int kx = cellcoord(p.x, t.shx[g], w, 0, m - 1) + 2;
+        const int ky = cellcoord(p.y, t.shy[g], w, 0, m - 1) + 2;
+        t.cellk[q] = (uint16_t)(kx | (ky << 8));
+        add_u16(t.A + g * gw, ky * M + kx + 2, 1u);
+    }
+    __syncthreads();
+    for (int g = 0; g < G; ++g) wave_scan_u16(t.A + g * gw, M * M + 2);
+    __syncthreads();
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        const int g = q / n, i = q - g * n;
+        const int ck = t.cellk[q];
+        const int c = (ck >> 8) * M + (ck & 255);
+        const unsigned slot = add_u16(t.A + g * gw, c + 1, 1u);
+        t.sorted[g * n + slot] = (uint16_t)i;
+    }
+    __syncthreads();
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        const int g = q / n, i = q - g * n;
+        const int ck = t.cellk[q];
+        const int c = (ck >> 8) * M + (ck & 255);
+        const uint16_t* Ag = t.A + g * gw;
+        const int a = Ag[c], b = Ag[c + 1];
+        int r = 0;
+        for (int k = a; k < b; ++k) r += t.sorted[g * n + k] < i;
+        t.rank[q] = (uint16_t)r;
+    }
+    __syncthreads();
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        const int g = q / n, i = q - g * n;
+        const int ck = t.cellk[q];
+        const int c = (ck >> 8) * M + (ck & 255);
+        t.sorted[g * n + t.A[g * gw + c] + t.rank[q]] = (uint16_t)i;
+    }
+    __syncthreads();
+}
+
+// One Metropolis proposal for square i of tile replica g.
+__device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float w,
+                                        float L, int r0, uint32_t sweep, uint32_t k0,
+                                        uint32_t k1, unsigned long long& cC,
+                                        unsigned long long& cP) {
+    const int g = q / n, i = q - g * n;
+    const int gw = grid_words(M);
+    const float4 p = t.pos[q];
+    const int ck = t.cellk[q];
+    const int kx = ck & 255, ky = ck >> 8;
+    const uint4 o = philox((uint32_t)i, (uint32_t)(r0 + g), sweep, 0u, k0, k1);
+    const float fx = __builtin_fmaf(2.0f, u01(o.x), -1.0f);
+    const float fy = __builtin_fmaf(2.0f, u01(o.y), -1.0f);
+    const float ft = __builtin_fmaf(2.0f, u01(o.z), -1.0f);
+    const float sxy = t.sxy[g];
+    const float nx = p.x + sxy * fx;
+    const float ny = p.y + sxy * fy;
+    if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) return;
+    if (cellcoord(nx, t.shx[g], w, 0, m - 1) + 2 != kx ||
+        cellcoord(ny, t.shy[g], w, 0, m - 1) + 2 != ky)
+        return;
+    float nc, ns;
+    rot32(p.z, p.w, t.sth[g] * ft, nc, ns);
+    float dE = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
+    const uint16_t* Ag = t.A + g * gw;
+    const uint16_t* sg = t.sorted + g * n;
+    const float4* pg = t.pos + g * n;
+    unsigned cnt = 0, near = 0;
+#pragma unroll 1
+    for (int oy = -2; oy <= 2; ++oy) {
+        const int row = (ky + oy) * M + kx;
+        const int a = Ag[row - 2], b = Ag[row + 3];
+        for (int k = a; k < b; ++k) {
+            const int j = sg[k];
+            if (j == i) continue;
+            const float4 v = pg[j];
+            const float dxn = v.x - nx, dyn = v.y - ny;
+            const float dxo = v.x - p.x, dyo = v.y - p.y;
+            const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
+            const float eo = depth2(dxo, dyo, p.z, p.w, v.z, v.w);
+            dE += en - eo;
+            ++cnt;
+            near += (__builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f) +
+                    (__builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        }
+    }
+    cC += cnt;
+    cP += near;
+    const float u = u01(o.w);
+    if (dE <= 0.0f || u < exp2f(-dE * t.ik[g])) {
+        t.pos[q] = make_float4(nx, ny, nc, ns);
+        atomicAdd(&t.acc[g], 1);
+    }
+}
+
+// Canonical FP32 energy of every tile replica: per-square wall + pairs with
+// larger index inside the 5x5 stencil, then a fixed pairwise tree over the
+// square index, so the bits do not depend on the tile shape.
+__device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float w, float L,
+                            float* out /* G floats, LDS or global */) {
+    const int lane = threadIdx.x;
+    for (int g = lane; g < G; g += kSweepThreads) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
+    __syncthreads();
+    bin_tile(t, n, G, M, m, w);
+    float* e = reinterpret_cast<float*>(t.rank);   // rank + items: 4 bytes per square
+    const int gw = grid_words(M);
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        const int g = q / n, i = q - g * n;
+        const float4 p = t.pos[q];
+        const int ck = t.cellk[q];
+        const int kx = ck & 255, ky = ck >> 8;
+        const uint16_t* Ag = t.A + g * gw;
+        const uint16_t* sg = t.sorted + g * n;
+        const float4* pg = t.pos + g * n;
+        float ei = wall32(p.x, p.y, p.z, p.w, L);
+        for (int oy = -2; oy <= 2; ++oy) {
+            const int row = (ky + oy) * M + kx;
+            const int a = Ag[row - 2], b = Ag[row + 3];
+            for (int k = a; k < b; ++k) {
+                const int j = sg[k];
+                if (j <= i) continue;
+                const float4 v = pg[j];
+                ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
+            }
+        }
+        e[q] = ei;
+    }
+    __syncthreads();
+    for (int s = 1; s < n; s <<= 1) {
+        for (int q = lane; q < G * n; q += kSweepThreads) {
+            const int g = q / n, i = q - g * n;
+            if ((i & (2 * s - 1)) == 0 && i + s < n) e[q] += e[q + s];
+        }
+        __syncthreads();
+    }
+    for (int g = lane; g < G; g += kSweepThreads) out[g] = e[g * n];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep: `spe` sweeps (or what is left of the budget) of every replica of
+// one tile.  One sweep = every square proposes exactly once.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict__ tiles,
+                                                         InstDesc* __restrict__ insts,
+                                                         RepState* __restrict__ reps,
+                                                         const SlotState* __restrict__ slots,
+                                                         float4* __restrict__ state) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const Tile tl = tiles[blockIdx.x];
+    InstDesc* D = insts + tl.inst;
+    const long long left = D->sweeps_total - D->sweeps_done;
+    const int nsw = (int)min((long long)D->spe, left);
+    if (nsw <= 0) return;
+    const int n = D->n, G = tl.g, M = D->M, lane = threadIdx.x;
+    const float L = D->Lf, w = D->w, scale = D->scale;
+    const int m = grid_cells(L, w);
+    const uint32_t k0 = D->key0, k1 = D->key1;
+    const uint32_t sweep0 = (uint32_t)D->sweeps_done;
+    const TileLds t = carve(smem, n, M, G);
+    float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
+
+    for (int q = lane; q < G * n; q += kSweepThreads) {
+        float4 v = gstate[q];
+        v.x = fminf(fmaxf(v.x * scale, 0.0f), L);
+        v.y = fminf(fmaxf(v.y * scale, 0.0f), L);
+        t.pos[q] = v;
+    }
+    for (int g = lane; g < G; g += kSweepThreads) {
+        const RepState rs = reps[D->rep_off + tl.r0 + g];
+        const SlotState ss = slots[D->rep_off + rs.slot];
+        t.sxy[g] = ss.step_xy;
+        t.sth[g] = ss.step_th;
+        t.ik[g] = ss.inv_t_log2e;
+        t.acc[g] = 0;
+    }
+    __syncthreads();
+
+    unsigned long long cC = 0, cP = 0;
+    for (int sw = 0; sw < nsw; ++sw) {
+        const uint32_t sweep = sweep0 + (uint32_t)sw;
+        for (int g = lane; g < G; g += kSweepThreads) {
+            const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep, 1u, k0, k1);
+            t.shx[g] = w * u01(o.x);
+            t.shy[g] = w * u01(o.y);
+        }
+        for (int k = lane; k < kGroups + 2; k += kSweepThreads) t.gcnt[k] = 0;
+        __syncthreads();
+        bin_tile(t, n, G, M, m, w);
+        // group key: occupancy rank x colour; ranks >= kRankCap go serial
+        for (int q = lane; q < G * n; q += kSweepThreads) {
+            const int ck = t.cellk[q];
+            const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
+            const int r = t.rank[q];
+            const int key = r < kRankCap ? r * 9 + colour : kGroups;
+            t.rank[q] = (uint16_t)key;
+            atomicAdd(&t.gcnt[key], 1);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int run = 0;
+            for (int k = 0; k <= kGroups; ++k) {
+                t.gstart[k] = run;
+                run += t.gcnt[k];
+            }
+            t.gstart[kGroups + 1] = run;
+        }
+        __syncthreads();
+        for (int k = lane; k <= kGroups; k += kSweepThreads) t.gcnt[k] = t.gstart[k];
+        __syncthreads();
+        for (int q = lane; q < G * n; q += kSweepThreads) {
+            const int key = t.rank[q];
+            if (key < kGroups) t.items[atomicAdd(&t.gcnt[key], 1)] = (uint16_t)q;
+        }
+        __syncthreads();
+        for (int gi = 0; gi < kGroups; ++gi) {
+            const int a = t.gstart[gi], b = t.gstart[gi + 1];
+            for (int it = a + lane; it < b; it += kSweepThreads)
+                propose(t, t.items[it], n, M, m, w, L, tl.r0, sweep, k0, k1, cC, cP);
+            __syncthreads();
+        }
+        if (t.gstart[kGroups + 1] > t.gstart[kGroups]) {
+            if (lane == 0)
+                for (int q = 0; q < G * n; ++q)
+                    if (t.rank[q] == kGroups)
+                        propose(t, q, n, M, m, w, L, tl.r0, sweep, k0, k1, cC, cP);
+            __syncthreads();
+        }
+    }
+
+    float* Eout = t.sth;  // reuse: steps are no longer needed
+    tile_energy(t, n, G, M, m, w, L, Eout);
+    for (int q = lane; q < G * n; q += kSweepThreads) gstate[q] = t.pos[q];
+    unsigned long long accsum = 0;
+    for (int g = lane; g < G; g += kSweepThreads) {
+        RepState* rs = reps + D->rep_off + tl.r0 + g;
+        rs->E = Eout[g];
+        rs->acc = (unsigned)t.acc[g];
+        accsum += (unsigned)t.acc[g];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        cC += __shfl_xor(cC, off, kSweepThreads);
+        cP += __shfl_xor(cP, off, kSweepThreads);
+        accsum += __shfl_xor(accsum, off, kSweepThreads);
+    }
+    if (lane == 0) {
+        atomicAdd(&D->cand, cC);
+        atomicAdd(&D->near, cP);
+        atomicAdd(&D->accepted, accsum);
+        atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (fixed tree, deterministic).
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ double block_sum_d(double v, double* tmp) {
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) tmp[threadIdx.x] += tmp[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = tmp[0];
+    __syncthreads();
+    return r;
+}
+template <int NT>
+__device__ double block_max_d(double v, double* tmp) {
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) tmp[threadIdx.x] = fmax(tmp[threadIdx.x], tmp[threadIdx.x + s]);
+        __syncthreads();
+    }
+    double r = tmp[0];
+    __syncthreads();
+    return r;
+}
+template <int NT>
+__device__ double block_min_d(double v, double* tmp) {
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) tmp[threadIdx.x] = fmin(tmp[threadIdx.x], tmp[threadIdx.x + s]);
+        __syncthreads();
+    }
+    double r = tmp[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 geometry (cuda/packing.cu:177-222).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pair64(double dx, double dy, double c, double s, double d,
+                                         double t) {
+    if (dx * dx + dy * dy >= 2.000001) return 0.0;
+    const double cd = c * d + s * t, sd = s * d - c * t;
+    const double R = .5 + .5 * (fabs(cd) + fabs(sd));
+    const double v1 = fabs(dx * c + dy * s), v2 = fabs(-dx * s + dy * c);
+    const double v3 = fabs(dx * d + dy * t), v4 = fabs(-dx * t + dy * d);
+    const double gap = fmax(fmax(v1, v2), fmax(v3, v4)) - R;
+    return gap >= 0.0 ? 0.0 : gap * gap;
+}
+
+__device__ __forceinline__ double sgn_dead(double v) {
+    return fabs(v) > 1e-6 ? (v >= 0 ? 1.0 : -1.0) : 0.0;
+}
+
+// Energy and gradient w.r.t. square i of its overlap with j (active axis).
+__device__ __forceinline__ void pair64_grad(double dx, double dy, double c, double s, double d,
+                                            double t, double& gx, double& gy, double& ga) {
+    if (dx * dx + dy * dy >= 2.000001) return;
+    const double cd = c * d + s * t, sd = s * d - c * t;
+    const double R = .5 + .5 * (fabs(cd) + fabs(sd));
+    const double dR = .5 * (-sgn_dead(cd) * sd + sgn_dead(sd) * cd);
+    const double nx[4] = {c, -s, d, -t}, ny[4] = {s, c, t, d};
+    double gap = -1e100, ax = 0, ay = 0, aa = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double v = dx * nx[k] + dy * ny[k];
+        const double z = fabs(v) - R;
+        if (z > gap) {
+            gap = z;
+            const double sv = v >= 0.0 ? 1.0 : -1.0;
+            ax = -sv * nx[k];
+            ay = -sv * ny[k];
+            aa = -dR + (k < 2 ? sv * (-dx * ny[k] + dy * nx[k]) : 0.0);
+        }
+    }
+    if (gap >= 0.0) return;
+    gx += 2.0 * gap * ax;
+    gy += 2.0 * gap * ay;
+    ga += 2.0 * gap * aa;
+}
+
+__device__ __forceinline__ double wall64(double x, double y, double c, double s, double L) {
+    const double r = 0.5 * (fabs(c) + fabs(s));
+    double e = 0.0;
+    if (x < r) { const double u = r - x; e += u * u; }
+    else if (x > L - r) { const double u = x + r - L; e += u * u; }
+    if (y < r) { const double u = r - y; e += u * u; }
+    else if (y > L - r) { const double u = y + r - L; e += u * u; }
+    return e;
+}
+
+__device__ __forceinline__ double wrap_quarter(double a) {
+    const double quarter = 1.57079632679489661923;
+    a = fmod(a, quarter);
+    if (a < 0.0) a += quarter;
+    return a;
+}
+
+// Total FP64 energy of the configuration held in LDS (all pairs i<j).
+template <int NT>
+__device__ double total64(const double* x, const double* y, const double* c, const double* s,
+                          int n, double L, double* tmp) {
+    double e = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        e += wall64(x[i], y[i], c[i], s[i], L);
+        for (int j = i + 1; j < n; ++j) e += pair64(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]);
+    }
+    return block_sum_d<NT>(e, tmp);
+}
+
+// ---------------------------------------------------------------------------
+// k_ladder: one workgroup per instance, after every sweep launch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict__ insts,
+                                                           RepState* __restrict__ reps,
+                                                           SlotState* __restrict__ slots,
+                                                           int* __restrict__ rep_of_slot,
+                                                           float4* __restrict__ state,
+                                                           float4* __restrict__ arch) {
+    __shared__ double x[kMaxN], y[kMaxN], cc[kMaxN], ss[kMaxN];
+    __shared__ double tmp[kLadderThreads];
+    __shared__ int zero_slot;
+    InstDesc* D = insts + blockIdx.x;
+    const long long left = D->sweeps_total - D->sweeps_done;
+    if (left <= 0) return;
+    const int ran = (int)min((long long)D->spe, left);
+    const int tid = threadIdx.x;
+    const int R = D->R, n = D->n, lad = D->ladder, base = D->rep_off;
+
+    // 1. step adaptation per temperature slot, from the replica that held it
+    for (int r = tid; r < R; r += kLadderThreads) {
+        const RepState rs = reps[base + r];
+        const float rate = (float)rs.acc / (float)((long long)n * ran);
+        const float f = expf((rate - D->target_acc) * 0.5f);
+        SlotState* sl = slots + base + rs.slot;
+        sl->step_xy = fminf(fmaxf(sl->step_xy * f, 1.7320508e-7f), 0.8660254f);
+        sl->step_th = fminf(fmaxf(sl->step_th * f, 1.7320508e-7f), 0.7794229f);
+    }
+    __syncthreads();
+
+    // 2. even/odd exchange of adjacent temperatures inside each ladder
+    const int parity = (int)(D->interval & 1u);
+    unsigned long long tried = 0, accd = 0;
+    for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
+        const int s = 2 * p + parity;
+        if ((s % lad) + 1 >= lad) continue;
+        const int ra = rep_of_slot[base + s], rb = rep_of_slot[base + s + 1];
+        const float Ta = slots[base + s].T, Tb = slots[base + s + 1].T;
+        const uint4 o = philox((uint32_t)s, 0u, D->interval, 2u, D->key0, D->key1);
+        const float lhs = logf(u01(o.x));
+        const float rhs = (1.0f / Ta - 1.0f / Tb) * (reps[base + ra].E - reps[base + rb].E);
+        ++tried;
+        if (lhs < rhs) {
+            rep_of_slot[base + s] = rb;
+            rep_of_slot[base + s + 1] = ra;
+            reps[base + ra].slot = s + 1;
+            reps[base + rb].slot = s;
+            ++accd;
+        }
+    }
+    if (tried) atomicAdd(&D->ex_tried, tried);
+    if (accd) atomicAdd(&D->ex_acc, accd);
+    if (tid == 0) zero_slot = R;
+    __syncthreads();
+
+    // 3. zero-overlap test: coldest replica with FP32 energy exactly zero.
+    //    Descriptor fields are read into registers before any thread writes.
+    const double L0 = D->L;
+    const int had_arch = D->arch_valid;
+    const double L_arch = D->L_arch;
+    for (int s = tid; s < R; s += kLadderThreads)
+        if (reps[base + rep_of_slot[base + s]].E == 0.0f) atomicMin(&zero_slot, s);
+    __syncthreads();
+    double scale = 1.0, Ln = L0;
+    bool archive = false;
+    if (zero_slot < R) {
+        const int r = rep_of_slot[base + zero_slot];
+        const float4* src = state + D->state_off + (size_t)r * n;
+        for (int i = tid; i < n; i += kLadderThreads) {
+            const float4 v = src[i];
+            const double c = v.z, s = v.w, h = 1.0 / sqrt(c * c + s * s);
+            x[i] = v.x; y[i] = v.y; cc[i] = c * h; ss[i] = s * h;
+        }
+        __syncthreads();
+        const double e64 = total64<kLadderThreads>(x, y, cc, ss, n, L0, tmp);
+        if (e64 == 0.0) {
+            archive = !had_arch || L0 < L_arch;
+            if (archive)
+                for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
+            Ln = fmax(L0 - (double)D->shrink, sqrt((double)n));
+            if (Ln < L0) scale = Ln / L0;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        if (archive) {
+            D->arch_valid = 1;
+            D->L_arch = L0;
+            D->feasible += 1;
+        }
+        if (Ln < L0) {
+            D->L = Ln;
+            D->Lf = (float)Ln;
+            D->shrinks += 1;
+        }
+        D->scale = (float)scale;
+        D->sweeps_done += ran;
+        D->interval += 1u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_finalize: FP64 polish + certified side + export, one instance per block.
+// src: n float4 (x, y, cos, sin) per instance at FP64 box side Lsrc.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPolishThreads) void k_finalize(const FinalizeJob* __restrict__ jobs,
+                                                             FinalizeOut* __restrict__ outs,
+                                                             double* __restrict__ q_out) {
+    __shared__ double x[kMaxN], y[kMaxN], a[kMaxN], cc[kMaxN], ss[kMaxN];
+    __shared__ double tmp[kPolishThreads];
+    __shared__ double step_sh;
+    constexpr int NT = kPolishThreads, KPT = kPolishKPT;
+    const FinalizeJob job = jobs[blockIdx.x];
+    FinalizeOut* out = outs + blockIdx.x;
+    const int n = job.n, tid = threadIdx.x;
+    const double L = job.L;
+    for (int i = tid; i < n; i += NT) {
+        double xi, yi, ai;
+        if (job.src32) {
+            const float4 v = job.src32[i];
+            xi = v.x; yi = v.y;
+            ai = wrap_quarter(atan2((double)v.w, (double)v.z));
+        } else {
+            xi = job.src64[3 * i]; yi = job.src64[3 * i + 1]; ai = job.src64[3 * i + 2];
+        }
+        x[i] = xi; y[i] = yi; a[i] = ai;
+        cc[i] = cos(ai); ss[i] = sin(ai);
+    }
+    __syncthreads();
+    double energy = total64<NT>(x, y, cc, ss, n, L, tmp);
+    const double e_before = energy;
+
+    // Polak-Ribiere+ conjugate gradient, cuda/packing.cu:624-687.
+    double g[KPT][3], gold[KPT][3], dir[KPT][3], saved[KPT][3];
+#pragma unroll
+    for (int k = 0; k < KPT; ++k)
+        for (int d = 0; d < 3; ++d) { gold[k][d] = 0.0; dir[k][d] = 0.0; g[k][d] = 0.0; saved[k][d] = 0.0; }
+    if (tid == 0) step_sh = 0.04;
+    __syncthreads();
+    int iters = 0;
+    for (int it = 0; it < job.iters; ++it) {
+        if (energy < 1e-24) break;
+        double gg = 0.0, dgg = 0.0;
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) {
+            const int i = tid + k * NT;
+            if (i >= n) break;
+            double gx = 0, gy = 0, ga = 0;
+            const double ci = cc[i], si = ss[i], xi = x[i], yi = y[i];
+            const double ext = 0.5 * (fabs(ci) + fabs(si));
+            const double dR = .5 * (-sgn_dead(ci) * si + sgn_dead(si) * ci);
+            if (xi < ext) { const double u = ext - xi; gx -= 2 * u; ga += 2 * u * dR; }
+            else if (xi > L - ext) { const double u = xi + ext - L; gx += 2 * u; ga += 2 * u * dR; }
+            if (yi < ext) { const double u = ext - yi; gy -= 2 * u; ga += 2 * u * dR; }
+            else if (yi > L - ext) { const double u = yi + ext - L; gy += 2 * u; ga += 2 * u * dR; }
+            for (int j = 0; j < n; ++j)
+                if (j != i) pair64_grad(x[j] - xi, y[j] - yi, ci, si, cc[j], ss[j], gx, gy, ga);
+            g[k][0] = gx; g[k][1] = gy; g[k][2] = ga;
+            for (int d = 0; d < 3; ++d) {
+                gg += gold[k][d] * gold[k][d];
+                dgg += (g[k][d] - gold[k][d]) * g[k][d];
+            }
+        }
+        gg = block_sum_d<NT>(gg, tmp);
+        dgg = block_sum_d<NT>(dgg, tmp);
+        const double beta = (it == 0 || gg < 1e-30 || it % 25 == 0) ? 0.0 : fmax(0.0, dgg / gg);
+        double nrm = 0.0;
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) {
+            const int i = tid + k * NT;
+            if (i >= n) break;
+            for (int d = 0; d < 3; ++d) {
+                dir[k][d] = -g[k][d] + beta * dir[k][d];
+                gold[k][d] = g[k][d];
+                nrm += dir[k][d] * dir[k][d];
+            }
+            saved[k][0] = x[i]; saved[k][1] = y[i]; saved[k][2] = a[i];
+        }
+        if (block_sum_d<NT>(nrm, tmp) < 1e-28) break;
+        iters = it + 1;
+        double alpha = step_sh;
+        bool ok = false;
+        for (int ls = 0; ls < 5; ++ls) {
+#pragma unroll
+            for (int k = 0; k < KPT; ++k) {
+                const int i = tid + k * NT;
+                if (i >= n) break;
+                x[i] = saved[k][0] + alpha * dir[k][0];
+                y[i] = saved[k][1] + alpha * dir[k][1];
+                a[i] = wrap_quarter(saved[k][2] + alpha * dir[k][2]);
+                cc[i] = cos(a[i]);
+                ss[i] = sin(a[i]);
+            }
+            __syncthreads();
+            const double ne = total64<NT>(x, y, cc, ss, n, L, tmp);
+            if (ne < energy) {
+                energy = ne;
+                if (tid == 0) step_sh = fmin(0.15, alpha * 1.25);
+                ok = true;
+                break;
+            }
+            alpha *= 0.45;
+        }
+        if (!ok) {
+#pragma unroll
+            for (int k = 0; k < KPT; ++k) {
+                const int i = tid + k * NT;
+                if (i >= n) break;
+                x[i] = saved[k][0]; y[i] = saved[k][1]; a[i] = saved[k][2];
+                cc[i] = cos(a[i]); ss[i] = sin(a[i]);
+                dir[k][0] = dir[k][1] = dir[k][2] = 0.0;
+            }
+            if (tid == 0) step_sh = fmax(1e-7, step_sh * 0.5);
+        }
+        __syncthreads();
+    }
+
+    // Certified side, cuda/packing.cu:254-297.
+    const double tau = 2.6e-8;
+    double lam = 1.0;
+    for (int i = tid; i < n; i += NT)
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = x[i] - x[j], dy = y[i] - y[j];
+            if (dx * dx + dy * dy >= 2.05) continue;
+            const double R = 0.5 + 0.5 * (fabs(cc[i] * cc[j] + ss[i] * ss[j]) +
+                                          fabs(-ss[i] * cc[j] + cc[i] * ss[j]));
+            const double v = fmax(fmax(fabs(dx * cc[i] + dy * ss[i]), fabs(-dx * ss[i] + dy * cc[i])),
+                                  fmax(fabs(dx * cc[j] + dy * ss[j]), fabs(-dx * ss[j] + dy * cc[j])));
+            const double need = v > 1e-12 ? (R + tau) / v : 1e9;
+            lam = fmax(lam, need);
+        }
+    lam = block_max_d<NT>(lam, tmp);
+    double lox = 1e9, hix = -1e9, loy = 1e9, hiy = -1e9;
+    const double lx = lam > 1.008 ? 1.0 : lam;
+    for (int i = tid; i < n; i += NT) {
+        const double r = 0.5 * (fabs(cc[i]) + fabs(ss[i]));
+        lox = fmin(lox, x[i] * lx - r); hix = fmax(hix, x[i] * lx + r);
+        loy = fmin(loy, y[i] * lx - r); hiy = fmax(hiy, y[i] * lx + r);
+    }
+    lox = block_min_d<NT>(lox, tmp);
+    hix = block_max_d<NT>(hix, tmp);
+    loy = block_min_d<NT>(loy, tmp);
+    hiy = block_max_d<NT>(hiy, tmp);
+    const double side = fmax(hix - lox, hiy - loy);
+    double* qo = q_out + job.out_off;
+    for (int i = tid; i < n; i += NT) {
+        qo[3 * i] = x[i] * lx - lox;
+        qo[3 * i + 1] = y[i] * lx - loy;
+        qo[3 * i + 2] = a[i];
+    }
+    if (tid == 0) {
+        out->e64_before = e_before;
+        out->e64_after = energy;
+        out->lambda = lam;
+        out->L_cert = lam > 1.008 ? 1e9 : side;
+        out->span = side;
+        out->iters = iters;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parity probes.
+// ---------------------------------------------------------------------------
+__global__ void k_depth32(int npairs, const float* __restrict__ in, float* __restrict__ depth,
+                          float* __restrict__ d2out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs) return;
+    const float* p = in + 8 * (size_t)k;
+    const float dx = p[0], dy = p[1];
+    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    const float g = sat_gap(dx, dy, p[2], p[3], p[4], p[5]);
+    depth[k] = (d2 < 2.0f && g > 0.0f) ? g : 0.0f;
+    d2out[k] = depth2(dx, dy, p[2], p[3], p[4], p[5]);
+}
+
+__global__ void k_cellcoord(int count, const float* __restrict__ x, float shift, float w, int lo,
+                            int hi, int* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) out[k] = cellcoord(x[k], shift, w, lo, hi);
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers.
+// ---------------------------------------------------------------------------
+hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
+                        InstDesc* insts, RepState* reps, const SlotState* slots, float4* state) {
+    hipLaunchKernelGGL(k_sweep, dim3(ntiles), dim3(kSweepThreads), lds, st, tiles, insts, reps,
+                       slots, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
+                         SlotState* slots, int* rep_of_slot, float4* state, float4* arch) {
+    hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), 0, st, insts, reps, slots,
+                       rep_of_slot, state, arch);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t st, int njobs, const FinalizeJob* jobs, FinalizeOut* outs,
+                           double* q_out) {
+    hipLaunchKernelGGL(k_finalize, dim3(njobs), dim3(kPolishThreads), 0, st, jobs, outs, q_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_depth32(hipStream_t st, int npairs, const float* in, float* depth, float* d2) {
+    hipLaunchKernelGGL(k_depth32, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, in, depth,
+                       d2);
+    return hipGetLastError();
+}
+
+hipError_t launch_cellcoord(hipStream_t st, int count, const float* x, float shift, float w, int lo,
+                            int hi, int* out) {
+    hipLaunchKernelGGL(k_cellcoord, dim3((count + 255) / 256), dim3(256), 0, st, count, x, shift, w,
+                       lo, hi, out);
+    return hipGetLastError();
+}
+
+const void* sweep_kernel_ptr() { return reinterpret_cast<const void*>(k_sweep); }
+
+hipError_t sweep_kernel_attrs(int* max_lds) {
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(k_sweep));
+    if (e == hipSuccess && max_lds) *max_lds = attr.maxDynamicSharedSizeBytes;
+    return e;
+}
+
+}  // namespace sis

@@ -1,0 +1,137 @@
+// C ABI parity entry points: run the hot-path kernels on caller data so the
+// tests can compare them with the CPU oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/sis_abi.h"
+#include "sis_host.h"
+
+using namespace sis;
+
+extern "C" {
+
+int sis_dev_depth32(int npairs, const float* in, float* depth, float* depth2) {
+    if (npairs < 0 || (npairs > 0 && (!in || !depth || !depth2))) return fail(SIS_EINVAL, "bad arguments");
+    if (npairs == 0) return SIS_OK;
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    DevBuf<float> d_in, d_o, d_o2;
+    SIS_HIPCHECK(d_in.alloc(8 * (size_t)npairs));
+    SIS_HIPCHECK(d_o.alloc(npairs));
+    SIS_HIPCHECK(d_o2.alloc(npairs));
+    SIS_HIPCHECK(hipMemcpy(d_in, in, sizeof(float) * 8 * (size_t)npairs, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_depth32(nullptr, npairs, d_in, d_o, d_o2));
+    SIS_HIPCHECK(hipMemcpy(depth, d_o, sizeof(float) * npairs, hipMemcpyDeviceToHost));
+    SIS_HIPCHECK(hipMemcpy(depth2, d_o2, sizeof(float) * npairs, hipMemcpyDeviceToHost));
+    return SIS_OK;
+}
+
+int sis_dev_cellcoord(int count, const float* x, float shift, float w, int lo, int hi, int* out) {
+    if (count < 0 || (count > 0 && (!x || !out)) || !(w > 0) || lo > hi)
+        return fail(SIS_EINVAL, "bad arguments");
+    if (count == 0) return SIS_OK;
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    DevBuf<float> d_x;
+    DevBuf<int> d_o;
+    SIS_HIPCHECK(d_x.alloc(count));
+    SIS_HIPCHECK(d_o.alloc(count));
+    SIS_HIPCHECK(hipMemcpy(d_x, x, sizeof(float) * count, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_cellcoord(nullptr, count, d_x, shift, w, lo, hi, d_o));
+    SIS_HIPCHECK(hipMemcpy(out, d_o, sizeof(int) * count, hipMemcpyDeviceToHost));
+    return SIS_OK;
+}
+
+int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
+                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
+                   uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,
+                   uint64_t counts[3]) {
+    if (n < 1 || n > kMaxN || R < 1 || R > 4096 || !(L > 0 && L <= 64) || !(cell >= kMinCell) ||
+        sweeps < 1 || group < 0 || !inv_t_log2e || !step_xy || !step_th || !xycs)
+        return fail(SIS_EINVAL, "bad arguments");
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    const int M = grid_cells(L, cell) + 4;
+    if (M > 255) return fail(SIS_EINVAL, "cell grid too wide");
+    int G = group > 0 ? std::min(group, R) : 1;
+    while (G > 1 && sweep_lds_bytes(n, M, G) > 65536) --G;
+    const size_t lds = sweep_lds_bytes(n, M, G);
+    if (lds > 65536) return fail(SIS_EINVAL, "replica does not fit LDS");
+    for (size_t k = 0; k < (size_t)n * R; ++k)
+        if (!(xycs[4 * k] >= 0.0f && xycs[4 * k] <= L && xycs[4 * k + 1] >= 0.0f && xycs[4 * k + 1] <= L))
+            return fail(SIS_EINVAL, "centres must lie in [0, L]");
+    InstDesc D{};
+    D.n = n; D.R = R; D.ladder = R; D.G = G; D.state_off = 0; D.rep_off = 0; D.M = M;
+    D.spe = sweeps; D.sweeps_total = (long long)sweep0 + sweeps; D.sweeps_done = sweep0;
+    D.L = L; D.L_arch = L; D.Lf = L; D.scale = 1.0f; D.w = cell; D.key0 = key0; D.key1 = key1;
+    std::vector<Tile> tiles;
+    for (int r0 = 0; r0 < R; r0 += G) tiles.push_back(Tile{0, r0, std::min(G, R - r0), 0});
+    std::vector<RepState> reps(R);
+    std::vector<SlotState> slots(R);
+    for (int r = 0; r < R; ++r) {
+        reps[r] = RepState{0.0f, r, 0u, 0u};
+        slots[r] = SlotState{inv_t_log2e[r], step_xy[r], step_th[r], 1.0f};
+    }
+    DevBuf<InstDesc> d_inst;
+    DevBuf<Tile> d_tiles;
+    DevBuf<RepState> d_reps;
+    DevBuf<SlotState> d_slots;
+    DevBuf<float4> d_state;
+    SIS_HIPCHECK(d_inst.alloc(1));
+    SIS_HIPCHECK(d_tiles.alloc(tiles.size()));
+    SIS_HIPCHECK(d_reps.alloc(R));
+    SIS_HIPCHECK(d_slots.alloc(R));
+    SIS_HIPCHECK(d_state.alloc((size_t)n * R));
+    SIS_HIPCHECK(hipMemcpy(d_inst, &D, sizeof(InstDesc), hipMemcpyHostToDevice));
+    SIS_HIPCHECK(hipMemcpy(d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+    SIS_HIPCHECK(hipMemcpy(d_reps, reps.data(), sizeof(RepState) * R, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(hipMemcpy(d_slots, slots.data(), sizeof(SlotState) * R, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(hipMemcpy(d_state, xycs, sizeof(float4) * (size_t)n * R, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_sweep(nullptr, (int)tiles.size(), lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+    SIS_HIPCHECK(hipDeviceSynchronize());
+    SIS_HIPCHECK(hipMemcpy(xycs, d_state, sizeof(float4) * (size_t)n * R, hipMemcpyDeviceToHost));
+    SIS_HIPCHECK(hipMemcpy(reps.data(), d_reps, sizeof(RepState) * R, hipMemcpyDeviceToHost));
+    SIS_HIPCHECK(hipMemcpy(&D, d_inst, sizeof(InstDesc), hipMemcpyDeviceToHost));
+    for (int r = 0; r < R; ++r) {
+        if (accepted) accepted[r] = (int)reps[r].acc;
+        if (energy) energy[r] = reps[r].E;
+    }
+    if (counts) {
+        counts[0] = D.proposals;
+        counts[1] = D.cand;
+        counts[2] = D.near;
+    }
+    return SIS_OK;
+}
+
+int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,
+                   double* energy_after, double* lam, double* L_cert, double* q_export) {
+    if (n < 1 || n > kMaxN || !(L > 0 && L <= 64) || iters < 0 || !q) return fail(SIS_EINVAL, "bad arguments");
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    DevBuf<double> d_q, d_out;
+    DevBuf<FinalizeJob> d_job;
+    DevBuf<FinalizeOut> d_fo;
+    SIS_HIPCHECK(d_q.alloc(3 * (size_t)n));
+    SIS_HIPCHECK(d_out.alloc(3 * (size_t)n));
+    SIS_HIPCHECK(d_job.alloc(1));
+    SIS_HIPCHECK(d_fo.alloc(1));
+    SIS_HIPCHECK(hipMemcpy(d_q, q, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    FinalizeJob j{};
+    j.n = n; j.iters = iters; j.L = L; j.src32 = nullptr; j.src64 = d_q; j.out_off = 0;
+    SIS_HIPCHECK(hipMemcpy(d_job, &j, sizeof(j), hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_finalize(nullptr, 1, d_job, d_fo, d_out));
+    SIS_HIPCHECK(hipDeviceSynchronize());
+    FinalizeOut fo;
+    SIS_HIPCHECK(hipMemcpy(&fo, d_fo, sizeof(fo), hipMemcpyDeviceToHost));
+    if (q_export) SIS_HIPCHECK(hipMemcpy(q_export, d_out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (energy_before) *energy_before = fo.e64_before;
+    if (energy_after) *energy_after = fo.e64_after;
+    if (lam) *lam = fo.lambda;
+    if (L_cert) *L_cert = fo.L_cert;
+    return SIS_OK;
+}
+
+}  // extern "C"

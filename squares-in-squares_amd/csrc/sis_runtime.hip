@@ -1,0 +1,627 @@
+// Host runtime: resident instances in HBM, replica tiles, the plain and the
+// batched (HIP graph) launch schedules, FP64 finalisation, the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sis_abi.h"
+#include "sis_host.h"
+#include "sis_internal.h"
+#include "sis_kernels.h"
+
+namespace sis {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) {
+    t_err = msg;
+    return code;
+}
+
+#define HIPCHECK(x)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(SIS_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device set-up.
+// ---------------------------------------------------------------------------
+static std::mutex g_init_mu;
+static DeviceInfo g_dev;
+
+int ensure_device(int device) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_dev.id >= 0 && (device < 0 || device == g_dev.id)) {
+        HIPCHECK(hipSetDevice(g_dev.id));
+        return SIS_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return fail(SIS_EDEVICE, "no HIP device available");
+    const int id = device < 0 ? 0 : device;
+    if (id >= count) return fail(SIS_EINVAL, "device index out of range");
+    HIPCHECK(hipSetDevice(id));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, id));
+    g_dev.id = id;
+    g_dev.cus = prop.multiProcessorCount;
+    g_dev.lds_per_block = (size_t)prop.sharedMemPerBlock;
+    g_dev.name = prop.name;
+    g_dev.arch = prop.gcnArchName;
+    return SIS_OK;
+}
+
+const DeviceInfo& device_info() { return g_dev; }
+
+// ---------------------------------------------------------------------------
+// Parameters.
+// ---------------------------------------------------------------------------
+void default_params(sis_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->tmin = 1e-5;
+    p->tmax = 5e-2;
+    p->move_sigma = 0.05;
+    p->rot_sigma = 0.05;
+    p->target_accept = 0.3;
+    p->shrink = 1e-4;
+    p->cell = 0.7075;
+    p->init_jitter = 0.0;
+    p->sweeps = 1000;
+    p->replicas = 64;
+    p->ladder = 0;
+    p->sweeps_per_exchange = 10;
+    p->replicas_per_group = 0;
+    p->polish_iters = 200;
+    p->init_mode = 0;
+    p->flags = 0;
+    p->rng_seed = 7;
+}
+
+int validate(const sis_instance* in, const sis_params* p) {
+    if (!in || !p) return fail(SIS_EINVAL, "null instance or params");
+    if (in->n < 1 || in->n > kMaxN) return fail(SIS_EINVAL, "n must be 1..1000");
+    if (!std::isfinite(in->L) || in->L <= 0 || in->L > 64) return fail(SIS_EINVAL, "L must be in (0, 64]");
+    if (in->q)
+        for (int k = 0; k < 3 * in->n; ++k)
+            if (!std::isfinite(in->q[k])) return fail(SIS_EINVAL, "non-finite seed coordinate");
+    if (!(p->tmin > 0) || !(p->tmax >= p->tmin) || !std::isfinite(p->tmax))
+        return fail(SIS_EINVAL, "need 0 < tmin <= tmax");
+    if (!(p->move_sigma > 0) || !(p->rot_sigma > 0)) return fail(SIS_EINVAL, "step sigmas must be > 0");
+    if (!(p->target_accept > 0 && p->target_accept < 1)) return fail(SIS_EINVAL, "target_accept in (0,1)");
+    if (!(p->shrink >= 0 && p->shrink <= 0.1)) return fail(SIS_EINVAL, "shrink in [0, 0.1]");
+    if (!((float)p->cell >= kMinCell && p->cell <= 4.0)) return fail(SIS_EINVAL, "cell width in [0.7075, 4]");
+    if (p->replicas < 1 || p->replicas > 4096) return fail(SIS_EINVAL, "replicas must be 1..4096");
+    if (p->ladder < 0 || (p->ladder > 0 && p->replicas % p->ladder != 0))
+        return fail(SIS_EINVAL, "ladder must divide replicas");
+    if (p->sweeps < 0 || p->sweeps > (1ll << 31)) return fail(SIS_EINVAL, "sweeps out of range");
+    if (p->sweeps_per_exchange < 1 || p->sweeps_per_exchange > 100000)
+        return fail(SIS_EINVAL, "sweeps_per_exchange out of range");
+    if (p->replicas_per_group < 0 || p->replicas_per_group > 256)
+        return fail(SIS_EINVAL, "replicas_per_group out of range");
+    if (p->polish_iters < 0 || p->polish_iters > 100000) return fail(SIS_EINVAL, "polish_iters out of range");
+    if (!(p->init_jitter >= 0 && p->init_jitter < 1)) return fail(SIS_EINVAL, "init_jitter in [0,1)");
+    const float w = (float)p->cell;
+    const int M = grid_cells((float)in->L, w) + 4;
+    if (M > 255) return fail(SIS_EINVAL, "cell grid wider than 255 cells: raise the cell width");
+    return SIS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side initial states (Philox purpose 3; same generator as the kernels).
+// ---------------------------------------------------------------------------
+static inline void philox_host(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                               uint32_t k1, uint32_t out[4]) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+static inline double u01d(uint32_t hi, uint32_t lo) {
+    const uint64_t k = (((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11)) & ((1ull << 52) - 1);
+    return ((double)k + 0.5) * (1.0 / 4503599627370496.0);
+}
+
+void initial_state(const InstSpec& s, std::vector<float4>& out) {
+    const int n = s.n, R = s.p.replicas;
+    const uint32_t k0 = (uint32_t)s.p.rng_seed, k1 = (uint32_t)(s.p.rng_seed >> 32);
+    out.resize((size_t)n * R);
+    const double L = s.L;
+    for (int r = 0; r < R; ++r)
+        for (int i = 0; i < n; ++i) {
+            double x, y, a;
+            uint32_t o[4], o2[4];
+            philox_host((uint32_t)i, (uint32_t)r, 0u, 3u, k0, k1, o);
+            philox_host((uint32_t)i, (uint32_t)r, 1u, 3u, k0, k1, o2);
+            if (!s.q.empty() && s.p.init_mode == 0) {
+                const double j = s.p.init_jitter;
+                x = s.q[3 * i] + j * (2 * u01d(o[0], o[1]) - 1);
+                y = s.q[3 * i + 1] + j * (2 * u01d(o[2], o[3]) - 1);
+                a = s.q[3 * i + 2] + j * (2 * u01d(o2[0], o2[1]) - 1);
+            } else {
+                const double lo = std::min(0.5, 0.5 * L), span = std::max(0.0, L - 2 * lo);
+                x = lo + span * u01d(o[0], o[1]);
+                y = lo + span * u01d(o[2], o[3]);
+                a = 1.5707963267948966 * u01d(o2[0], o2[1]);
+            }
+            x = std::min(std::max(x, 0.0), L);
+            y = std::min(std::max(y, 0.0), L);
+            out[(size_t)r * n + i] = make_float4((float)x, (float)y, (float)std::cos(a), (float)std::sin(a));
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Tile planning.
+// ---------------------------------------------------------------------------
+int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
+    const DeviceInfo& dev = device_info();
+    const float w = (float)s.p.cell;
+    pl.M = grid_cells((float)s.L, w) + 4;
+    pl.ladder = s.p.ladder > 0 ? s.p.ladder : s.p.replicas;
+    pl.spe = s.p.sweeps_per_exchange;
+    const size_t budget = std::min<size_t>(64 * 1024, dev.lds_per_block ? dev.lds_per_block : 65536);
+    int G = s.p.replicas_per_group;
+    if (G <= 0) {
+        // Fill the 64 lanes of a colour group (about G*n/13 movers) but keep
+        // at least two tiles per compute unit across the whole batch.
+        int g_lane = 1;
+        while (g_lane < 64 && g_lane * s.n < 13 * 64) g_lane *= 2;
+        const int cus = std::max(1, dev.cus);
+        int g_occ = 64;
+        while (g_occ > 1 && (long long)batch_replicas / g_occ < 2ll * cus) g_occ /= 2;
+        G = std::max(1, std::min(g_lane, g_occ));
+    }
+    G = std::min(G, s.p.replicas);
+    while (G > 1 && (sweep_lds_bytes(s.n, pl.M, G) > budget || G * s.n > 65535)) --G;
+    if (sweep_lds_bytes(s.n, pl.M, G) > budget)
+        return fail(SIS_EINVAL, "one replica does not fit the LDS budget");
+    pl.G = G;
+    pl.lds = sweep_lds_bytes(s.n, pl.M, G);
+    pl.ntiles = (s.p.replicas + G - 1) / G;
+    return SIS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch: a set of instances resident in device memory.
+// ---------------------------------------------------------------------------
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    for (void* p : {(void*)d_inst, (void*)d_tiles, (void*)d_reps, (void*)d_slots, (void*)d_ros,
+                    (void*)d_state, (void*)d_arch, (void*)d_jobs, (void*)d_fout, (void*)d_qout})
+        if (p) (void)hipFree(p);
+    d_inst = nullptr; d_tiles = nullptr; d_reps = nullptr; d_slots = nullptr; d_ros = nullptr;
+    d_state = nullptr; d_arch = nullptr; d_jobs = nullptr; d_fout = nullptr; d_qout = nullptr;
+    for (hipEvent_t e : {ev0, ev1, evs0, evs1})
+        if (e) (void)hipEventDestroy(e);
+    ev0 = ev1 = evs0 = evs1 = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+int Batch::build(std::vector<InstSpec>&& in) {
+    specs = std::move(in);
+    const int ni = (int)specs.size();
+    if (ni < 1) return fail(SIS_EINVAL, "empty batch");
+    long long total_reps = 0;
+    for (auto& s : specs) total_reps += s.p.replicas;
+    plans.resize(ni);
+    h_inst.assign(ni, InstDesc{});
+    h_tiles.clear();
+    size_t nsq = 0, nrep = 0, narch = 0;
+    max_lds = 0;
+    max_intervals = 0;
+    for (int k = 0; k < ni; ++k) {
+        const InstSpec& s = specs[k];
+        Plan& pl = plans[k];
+        int rc = plan_instance(s, (int)std::min<long long>(total_reps, 1 << 30), pl);
+        if (rc) return rc;
+        InstDesc& D = h_inst[k];
+        D.n = s.n; D.R = s.p.replicas; D.ladder = pl.ladder; D.G = pl.G;
+        D.state_off = (int)nsq; D.rep_off = (int)nrep; D.arch_off = (int)narch;
+        D.M = pl.M; D.spe = pl.spe;
+        D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
+        D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
+        D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
+        D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
+        nsq += (size_t)s.n * s.p.replicas;
+        nrep += s.p.replicas;
+        narch += s.n;
+        max_lds = std::max(max_lds, pl.lds);
+        const long long iv = (s.p.sweeps + pl.spe - 1) / pl.spe;
+        max_intervals = std::max(max_intervals, iv);
+        for (int r0 = 0; r0 < s.p.replicas; r0 += pl.G)
+            h_tiles.push_back(Tile{k, r0, std::min(pl.G, s.p.replicas - r0), 0});
+    }
+    if (nsq > (size_t)INT32_MAX) return fail(SIS_EINVAL, "batch too large");
+    // heaviest tiles first: the dispatcher hands them out while CUs free up
+    std::stable_sort(h_tiles.begin(), h_tiles.end(), [&](const Tile& a, const Tile& b) {
+        return (long long)a.g * specs[a.inst].n > (long long)b.g * specs[b.inst].n;
+    });
+    n_sq = nsq; n_rep = nrep; n_arch = narch;
+
+    // host initial state, slots, ladders
+    std::vector<float4> h_state(nsq);
+    std::vector<RepState> h_reps(nrep);
+    std::vector<SlotState> h_slots(nrep);
+    std::vector<int> h_ros(nrep);
+    for (int k = 0; k < ni; ++k) {
+        const InstSpec& s = specs[k];
+        const InstDesc& D = h_inst[k];
+        std::vector<float4> st;
+        initial_state(s, st);
+        std::copy(st.begin(), st.end(), h_state.begin() + D.state_off);
+        const int lad = D.ladder;
+        for (int r = 0; r < D.R; ++r) {
+            const int pos = r % lad;
+            const double T = lad == 1 ? s.p.tmin
+                                      : s.p.tmin * std::pow(s.p.tmax / s.p.tmin, (double)pos / (lad - 1));
+            SlotState& sl = h_slots[D.rep_off + r];
+            sl.T = (float)T;
+            sl.inv_t_log2e = (float)(1.4426950408889634 / T);
+            sl.step_xy = (float)(std::sqrt(3.0) * s.p.move_sigma);
+            sl.step_th = (float)std::min(0.7794229, std::sqrt(3.0) * s.p.rot_sigma);
+            h_reps[D.rep_off + r] = RepState{0.0f, r, 0u, 0u};
+            h_ros[D.rep_off + r] = r;
+        }
+    }
+
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&ev0));
+    HIPCHECK(hipEventCreate(&ev1));
+    HIPCHECK(hipEventCreate(&evs0));
+    HIPCHECK(hipEventCreate(&evs1));
+    HIPCHECK(hipMalloc(&d_inst, sizeof(InstDesc) * ni));
+    HIPCHECK(hipMalloc(&d_tiles, sizeof(Tile) * h_tiles.size()));
+    HIPCHECK(hipMalloc(&d_reps, sizeof(RepState) * nrep));
+    HIPCHECK(hipMalloc(&d_slots, sizeof(SlotState) * nrep));
+    HIPCHECK(hipMalloc(&d_ros, sizeof(int) * nrep));
+    HIPCHECK(hipMalloc(&d_state, sizeof(float4) * nsq));
+    HIPCHECK(hipMalloc(&d_arch, sizeof(float4) * narch));
+    HIPCHECK(hipMemcpyAsync(d_inst, h_inst.data(), sizeof(InstDesc) * ni, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_tiles, h_tiles.data(), sizeof(Tile) * h_tiles.size(), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_reps, h_reps.data(), sizeof(RepState) * nrep, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_slots, h_slots.data(), sizeof(SlotState) * nrep, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_ros, h_ros.data(), sizeof(int) * nrep, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_state, h_state.data(), sizeof(float4) * nsq, hipMemcpyHostToDevice, stream));
+    std::vector<float4> h_arch(narch);
+    for (int k = 0; k < ni; ++k)
+        std::copy(h_state.begin() + h_inst[k].state_off,
+                  h_state.begin() + h_inst[k].state_off + specs[k].n, h_arch.begin() + h_inst[k].arch_off);
+    HIPCHECK(hipMemcpyAsync(d_arch, h_arch.data(), sizeof(float4) * narch, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (max_lds > 65536)
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(sweep_kernel_ptr()),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds));
+    return SIS_OK;
+}
+
+int Batch::enqueue_interval() {
+    HIPCHECK(launch_sweep(stream, (int)h_tiles.size(), max_lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+    HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
+    return SIS_OK;
+}
+
+// Plain schedule: one interval at a time, host waits on the instance status
+// after every interval (the reference's per-epoch blocking copy,
+// cuda/packing.cu:831).  The sweep kernel of every interval is timed with its
+// own event pair on the launch stream.
+int Batch::run_plain(long long intervals) {
+    HIPCHECK(hipEventRecord(ev0, stream));
+    for (long long iv = 0; iv < intervals; ++iv) {
+        HIPCHECK(hipEventRecord(evs0, stream));
+        HIPCHECK(launch_sweep(stream, (int)h_tiles.size(), max_lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+        HIPCHECK(hipEventRecord(evs1, stream));
+        HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
+        HIPCHECK(hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * specs.size(),
+                                hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        float sms = 0;
+        HIPCHECK(hipEventElapsedTime(&sms, evs0, evs1));
+        sweep_ms += sms;
+        sweep_launches += 1;
+    }
+    HIPCHECK(hipEventRecord(ev1, stream));
+    HIPCHECK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    device_ms += ms;
+    last_ms = ms;
+    intervals_done += intervals;
+    return SIS_OK;
+}
+
+int Batch::run(long long intervals, int mode) {
+    if (intervals <= 0) intervals = std::max(0ll, max_intervals - intervals_done);
+    if (intervals == 0) {
+        last_ms = 0;
+        return SIS_OK;
+    }
+    if (mode == 0) return run_plain(intervals);
+    const int per = (int)std::min<long long>(intervals, 50);
+    int rc = run_graph(per, intervals / per);
+    if (rc) return rc;
+    const double graph_ms = last_ms;
+    const long long rest = intervals % per;
+    if (rest) {
+        HIPCHECK(hipEventRecord(ev0, stream));
+        for (long long k = 0; k < rest; ++k)
+            if ((rc = enqueue_interval())) return rc;
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        device_ms += ms;
+        last_ms = graph_ms + ms;
+        intervals_done += rest;
+    }
+    return SIS_OK;
+}
+
+// Batched schedule: `per_graph` intervals captured once and replayed
+// `launches` times.  No host synchronisation in between.
+int Batch::run_graph(int per_graph, long long launches) {
+    per_graph = std::max(1, per_graph);
+    if (!graph_exec || graph_intervals != per_graph) {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < per_graph; ++k) {
+            int rc = enqueue_interval();
+            if (rc) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+        }
+        HIPCHECK(hipStreamEndCapture(stream, &graph));
+        HIPCHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        graph_intervals = per_graph;
+    }
+    HIPCHECK(hipEventRecord(ev0, stream));
+    for (long long k = 0; k < launches; ++k) HIPCHECK(hipGraphLaunch(graph_exec, stream));
+    HIPCHECK(hipEventRecord(ev1, stream));
+    HIPCHECK(hipEventSynchronize(ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    device_ms += ms;
+    last_ms = ms;
+    intervals_done += launches * per_graph;
+    return SIS_OK;
+}
+
+// FP64 polish + certified side of each instance's best configuration.
+int Batch::finalize(std::vector<FinalRecord>& rec) {
+    const int ni = (int)specs.size();
+    HIPCHECK(hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * ni, hipMemcpyDeviceToHost, stream));
+    std::vector<RepState> h_reps(n_rep);
+    HIPCHECK(hipMemcpyAsync(h_reps.data(), d_reps, sizeof(RepState) * n_rep, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    std::vector<FinalizeJob> jobs(ni);
+    rec.assign(ni, FinalRecord{});
+    long long qoff = 0;
+    for (int k = 0; k < ni; ++k) {
+        const InstDesc& D = h_inst[k];
+        FinalizeJob& j = jobs[k];
+        j.n = D.n;
+        j.iters = specs[k].p.polish_iters;
+        j.src64 = nullptr;
+        j.out_off = qoff;
+        qoff += 3ll * D.n;
+        int best = 0;
+        for (int r = 1; r < D.R; ++r)
+            if (h_reps[D.rep_off + r].E < h_reps[D.rep_off + best].E) best = r;
+        if (D.arch_valid) {
+            j.src32 = d_arch + D.arch_off;
+            j.L = D.L_arch;
+            rec[k].e32 = 0.0;
+        } else {
+            j.src32 = d_state + D.state_off + (size_t)best * D.n;
+            j.L = D.L;
+            rec[k].e32 = h_reps[D.rep_off + best].E;
+        }
+        rec[k].target_L = j.L;
+    }
+    for (void* p : {(void*)d_jobs, (void*)d_fout, (void*)d_qout})
+        if (p) (void)hipFree(p);
+    d_jobs = nullptr;
+    d_fout = nullptr;
+    d_qout = nullptr;
+    HIPCHECK(hipMalloc(&d_jobs, sizeof(FinalizeJob) * ni));
+    HIPCHECK(hipMalloc(&d_fout, sizeof(FinalizeOut) * ni));
+    HIPCHECK(hipMalloc(&d_qout, sizeof(double) * std::max<long long>(1, qoff)));
+    HIPCHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(FinalizeJob) * ni, hipMemcpyHostToDevice, stream));
+    HIPCHECK(launch_finalize(stream, ni, d_jobs, d_fout, d_qout));
+    std::vector<FinalizeOut> fo(ni);
+    std::vector<double> q(std::max<long long>(1, qoff));
+    HIPCHECK(hipMemcpyAsync(fo.data(), d_fout, sizeof(FinalizeOut) * ni, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(q.data(), d_qout, sizeof(double) * q.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int k = 0; k < ni; ++k) {
+        const InstDesc& D = h_inst[k];
+        FinalRecord& r = rec[k];
+        r.q.assign(q.begin() + jobs[k].out_off, q.begin() + jobs[k].out_off + 3ll * D.n);
+        r.e64_before = fo[k].e64_before;
+        r.e64_after = fo[k].e64_after;
+        r.lambda = fo[k].lambda;
+        r.L_cert = fo[k].L_cert;
+        r.desc = D;
+    }
+    return SIS_OK;
+}
+
+void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out) {
+    const InstDesc& D = r.desc;
+    if (out->q_out) std::copy(r.q.begin(), r.q.end(), out->q_out);
+    out->target_L = r.target_L;
+    out->L_cert = r.L_cert;
+    out->lambda = r.lambda;
+    out->e32 = r.e32;
+    out->e64_before = r.e64_before;
+    out->e64_after = r.e64_after;
+    out->e_gap = std::fabs(r.e32 - r.e64_before);
+    out->proposals = D.proposals;
+    out->accepted = D.accepted;
+    out->sweeps = (uint64_t)D.sweeps_done;
+    out->exchanges_tried = D.ex_tried;
+    out->exchanges_accepted = D.ex_acc;
+    out->candidates = D.cand;
+    out->near_pairs = D.near;
+    out->shrinks = D.shrinks;
+    out->feasible = D.arch_valid ? 1 : 0;
+    out->replicas = D.R;
+    out->status = SIS_OK;
+    out->seconds = wall_s;
+    out->device_seconds = b.device_ms * 1e-3;
+    (void)k;
+}
+
+InstSpec make_spec(const sis_instance* in, const sis_params* p) {
+    InstSpec s;
+    s.n = in->n;
+    s.L = in->L;
+    if (in->q) s.q.assign(in->q, in->q + 3 * in->n);
+    s.p = *p;
+    return s;
+}
+
+}  // namespace sis
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace sis;
+
+extern "C" {
+
+int sis_init(int device) { return ensure_device(device); }
+
+const char* sis_last_error(void) { return t_err.c_str(); }
+
+int sis_device_info(int* compute_units, char* name, int name_len) {
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    const DeviceInfo& d = device_info();
+    if (compute_units) *compute_units = d.cus;
+    if (name && name_len > 0) {
+        std::strncpy(name, d.name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return SIS_OK;
+}
+
+void sis_default_params(sis_params* p) {
+    if (p) default_params(p);
+}
+
+int sis_solve(const sis_instance* inst, const sis_params* params, sis_result* out) {
+    if (!out) return fail(SIS_EINVAL, "null result");
+    int rc = validate(inst, params);
+    if (rc) return out->status = rc;
+    if ((rc = ensure_device(-1))) return out->status = rc;
+    auto t0 = std::chrono::steady_clock::now();
+    Batch b;
+    std::vector<InstSpec> v;
+    v.push_back(make_spec(inst, params));
+    if ((rc = b.build(std::move(v)))) return out->status = rc;
+    if ((rc = b.run(0, 0))) return out->status = rc;
+    std::vector<FinalRecord> rec;
+    if ((rc = b.finalize(rec))) return out->status = rc;
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_result(rec[0], b, 0, wall, out);
+    return SIS_OK;
+}
+
+int sis_solve_batch(const sis_instance* insts, int count, const sis_params* params, sis_result* out) {
+    if (!insts || !params || !out || count < 1) return fail(SIS_EINVAL, "bad batch arguments");
+    int rc;
+    for (int k = 0; k < count; ++k)
+        if ((rc = validate(insts + k, params + k))) {
+            for (int j = 0; j < count; ++j) out[j].status = rc;
+            return rc;
+        }
+    if ((rc = ensure_device(-1))) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    Batch b;
+    std::vector<InstSpec> v;
+    for (int k = 0; k < count; ++k) v.push_back(make_spec(insts + k, params + k));
+    if ((rc = b.build(std::move(v)))) return rc;
+    if ((rc = b.run(0, 1))) return rc;
+    std::vector<FinalRecord> rec;
+    if ((rc = b.finalize(rec))) return rc;
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int k = 0; k < count; ++k) fill_result(rec[k], b, k, wall, out + k);
+    return SIS_OK;
+}
+
+struct sis_batch {
+    sis::Batch b;
+    std::chrono::steady_clock::time_point t0;
+};
+
+sis_batch* sis_batch_create(const sis_instance* insts, int count, const sis_params* params) {
+    if (!insts || !params || count < 1) {
+        fail(SIS_EINVAL, "bad batch arguments");
+        return nullptr;
+    }
+    for (int k = 0; k < count; ++k)
+        if (validate(insts + k, params + k)) return nullptr;
+    if (ensure_device(-1)) return nullptr;
+    auto* h = new sis_batch;
+    h->t0 = std::chrono::steady_clock::now();
+    std::vector<InstSpec> v;
+    for (int k = 0; k < count; ++k) v.push_back(make_spec(insts + k, params + k));
+    if (h->b.build(std::move(v))) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+int sis_batch_run(sis_batch* h, int64_t intervals, int mode, double* sweep_ms, int64_t* sweep_launches,
+                  double* total_ms) {
+    if (!h || (mode != 0 && mode != 1)) return fail(SIS_EINVAL, "bad batch handle or mode");
+    const double s0 = h->b.sweep_ms;
+    const long long l0 = h->b.sweep_launches;
+    int rc = h->b.run(intervals, mode);
+    if (rc) return rc;
+    if (sweep_ms) *sweep_ms = h->b.sweep_ms - s0;
+    if (sweep_launches) *sweep_launches = h->b.sweep_launches - l0;
+    if (total_ms) *total_ms = h->b.last_ms;
+    return SIS_OK;
+}
+
+int sis_batch_finalize(sis_batch* h, sis_result* out) {
+    if (!h || !out) return fail(SIS_EINVAL, "bad batch handle");
+    std::vector<FinalRecord> rec;
+    int rc = h->b.finalize(rec);
+    if (rc) return rc;
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - h->t0).count();
+    for (size_t k = 0; k < rec.size(); ++k) fill_result(rec[k], h->b, (int)k, wall, out + k);
+    return SIS_OK;
+}
+
+int sis_batch_destroy(sis_batch* h) {
+    delete h;
+    return SIS_OK;
+}
+
+}  // extern "C"

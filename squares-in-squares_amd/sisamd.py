@@ -1,0 +1,318 @@
+"""Python host mirror of the PT-SA engine's C ABI (include/sis_abi.h).
+
+The reference drives its search as a process (``packing seed.txt epochs steps
+chains rng_seed out.json``, cuda/packing.cu:768-868) from Python scripts such
+as cuda/cpp_loop.py:47-80 and cuda/demo.py:84-86.  This module offers the same
+operations in-process on top of ``lib/libsisamd.so``: ``solve`` (plain form),
+``solve_batch`` (batched HIP-graph form), the serving loop, the certifier and
+the device parity probes.  Every call goes to the HIP library; there is no
+CPU fallback, and a missing library raises ``LibraryMissing``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsisamd.so")
+PACKING_BIN = os.path.join(HERE, "bin", "packing")
+
+SIS_OK, SIS_PENDING, SIS_EINVAL, SIS_EDEVICE = 0, 1, 2, 3
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class SisError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sis error {code}: {msg}")
+        self.code = code
+
+
+class SisInstance(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("L", ctypes.c_double), ("q", ctypes.POINTER(ctypes.c_double))]
+
+
+class SisParams(ctypes.Structure):
+    _fields_ = [
+        ("tmin", ctypes.c_double), ("tmax", ctypes.c_double),
+        ("move_sigma", ctypes.c_double), ("rot_sigma", ctypes.c_double),
+        ("target_accept", ctypes.c_double), ("shrink", ctypes.c_double),
+        ("cell", ctypes.c_double), ("init_jitter", ctypes.c_double),
+        ("sweeps", ctypes.c_int64), ("replicas", ctypes.c_int), ("ladder", ctypes.c_int),
+        ("sweeps_per_exchange", ctypes.c_int), ("replicas_per_group", ctypes.c_int),
+        ("polish_iters", ctypes.c_int), ("init_mode", ctypes.c_int), ("flags", ctypes.c_int),
+        ("rng_seed", ctypes.c_uint64),
+    ]
+
+
+class SisResult(ctypes.Structure):
+    _fields_ = [
+        ("q_out", ctypes.POINTER(ctypes.c_double)),
+        ("target_L", ctypes.c_double), ("L_cert", ctypes.c_double), ("lambda_", ctypes.c_double),
+        ("e32", ctypes.c_double), ("e64_before", ctypes.c_double), ("e64_after", ctypes.c_double),
+        ("e_gap", ctypes.c_double),
+        ("proposals", ctypes.c_uint64), ("accepted", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
+        ("exchanges_tried", ctypes.c_uint64), ("exchanges_accepted", ctypes.c_uint64),
+        ("candidates", ctypes.c_uint64), ("near_pairs", ctypes.c_uint64),
+        ("shrinks", ctypes.c_int), ("feasible", ctypes.c_int), ("replicas", ctypes.c_int),
+        ("status", ctypes.c_int), ("seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
+    ]
+
+
+class SisServerCfg(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_double), ("max_instances", ctypes.c_int),
+        ("intervals_per_round", ctypes.c_int), ("certifier_threads", ctypes.c_int),
+        ("certify", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsisamd.so (built in-tree by ``make -C squares-in-squares_amd``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} not built: run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    dp, fp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.sis_init.argtypes = [ctypes.c_int]
+    L.sis_last_error.restype = ctypes.c_char_p
+    L.sis_device_info.argtypes = [ip, ctypes.c_char_p, ctypes.c_int]
+    L.sis_default_params.argtypes = [ctypes.POINTER(SisParams)]
+    L.sis_default_params.restype = None
+    L.sis_solve.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.POINTER(SisResult)]
+    L.sis_solve_batch.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams),
+                                  ctypes.POINTER(SisResult)]
+    L.sis_batch_create.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams)]
+    L.sis_batch_create.restype = ctypes.c_void_p
+    L.sis_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, dp,
+                                ctypes.POINTER(ctypes.c_int64), dp]
+    L.sis_batch_finalize.argtypes = [ctypes.c_void_p, ctypes.POINTER(SisResult)]
+    L.sis_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.sis_server_create.argtypes = [ctypes.POINTER(SisServerCfg)]
+    L.sis_server_create.restype = ctypes.c_void_p
+    L.sis_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), u64p]
+    L.sis_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(SisResult)]
+    L.sis_ticket_times.argtypes = [ctypes.c_void_p, ctypes.c_uint64, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
+    L.sis_server_destroy.argtypes = [ctypes.c_void_p]
+    L.sis_certify_strings.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.c_char_p, ctypes.c_int]
+    L.sis_export_certify.argtypes = [ctypes.c_int, dp, dp, ctypes.c_char_p, ctypes.c_int]
+    L.sis_dev_depth32.argtypes = [ctypes.c_int, fp, fp, fp]
+    L.sis_dev_cellcoord.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                    ctypes.c_int, ip]
+    L.sis_dev_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                 fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                 fp, ip, fp, u64p]
+    L.sis_dev_polish.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+    _lib = L
+    return L
+
+
+def _err(code: int):
+    raise SisError(code, lib().sis_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def default_params(**kw) -> SisParams:
+    p = SisParams()
+    lib().sis_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def params_dict(p: SisParams) -> dict:
+    return {name: getattr(p, name) for name, _ in SisParams._fields_}
+
+
+def device_info() -> tuple[int, str]:
+    cus = ctypes.c_int(0)
+    name = ctypes.create_string_buffer(256)
+    rc = lib().sis_device_info(ctypes.byref(cus), name, 256)
+    if rc:
+        _err(rc)
+    return cus.value, name.value.decode()
+
+
+@dataclass
+class Result:
+    q: np.ndarray
+    fields: dict = field(default_factory=dict)
+
+    def __getattr__(self, k):
+        try:
+            return self.fields[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+
+def _result(r: SisResult, q: np.ndarray) -> Result:
+    f = {name: getattr(r, name) for name, _ in SisResult._fields_ if name != "q_out"}
+    f["lambda"] = f.pop("lambda_")
+    return Result(q=q.copy(), fields=f)
+
+
+def _instance(n: int, L: float, q) -> tuple[SisInstance, np.ndarray | None]:
+    if q is None:
+        return SisInstance(n, L, None), None
+    arr = np.ascontiguousarray(np.asarray(q, dtype=np.float64).reshape(n, 3))
+    return SisInstance(n, L, _ptr(arr, ctypes.c_double)), arr
+
+
+def solve(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Result:
+    """Plain form: one instance, host sync per exchange interval."""
+    p = params if params is not None else default_params(**kw)
+    inst, keep = _instance(n, L, q)
+    out = np.zeros((n, 3), dtype=np.float64)
+    r = SisResult()
+    r.q_out = _ptr(out, ctypes.c_double)
+    rc = lib().sis_solve(ctypes.byref(inst), ctypes.byref(p), ctypes.byref(r))
+    if rc:
+        _err(rc)
+    del keep
+    return _result(r, out)
+
+
+def solve_batch(instances, params) -> list[Result]:
+    """Batched form: instances = [(n, L, q|None)], params = SisParams or list."""
+    k = len(instances)
+    if isinstance(params, SisParams):
+        params = [params] * k
+    insts = (SisInstance * k)()
+    ps = (SisParams * k)(*params)
+    keep, outs = [], []
+    res = (SisResult * k)()
+    for j, (n, L, q) in enumerate(instances):
+        insts[j], arr = _instance(n, L, q)
+        keep.append(arr)
+        o = np.zeros((n, 3), dtype=np.float64)
+        outs.append(o)
+        res[j].q_out = _ptr(o, ctypes.c_double)
+    rc = lib().sis_solve_batch(insts, k, ps, res)
+    if rc:
+        _err(rc)
+    return [_result(res[j], outs[j]) for j in range(k)]
+
+
+class BatchSession:
+    """Resident instances run step-wise (the epoch loop of cuda/packing.cu:823-853)."""
+
+    def __init__(self, instances, params):
+        k = len(instances)
+        if isinstance(params, SisParams):
+            params = [params] * k
+        self.k = k
+        self.ns = [n for n, _, _ in instances]
+        insts = (SisInstance * k)()
+        keep = []
+        for j, (n, L, q) in enumerate(instances):
+            insts[j], arr = _instance(n, L, q)
+            keep.append(arr)
+        ps = (SisParams * k)(*params)
+        self.h = lib().sis_batch_create(insts, k, ps)
+        if not self.h:
+            raise SisError(SIS_EINVAL, lib().sis_last_error().decode(errors="replace"))
+
+    def run(self, intervals: int, mode: int = 0):
+        """Returns (sweep_kernel_ms, sweep_launches, total_ms)."""
+        sm, tm = ctypes.c_double(), ctypes.c_double()
+        nl = ctypes.c_int64()
+        rc = lib().sis_batch_run(self.h, intervals, mode, ctypes.byref(sm), ctypes.byref(nl), ctypes.byref(tm))
+        if rc:
+            _err(rc)
+        return sm.value, nl.value, tm.value
+
+    def finalize(self) -> list[Result]:
+        res = (SisResult * self.k)()
+        outs = []
+        for j, n in enumerate(self.ns):
+            o = np.zeros((n, 3), dtype=np.float64)
+            outs.append(o)
+            res[j].q_out = _ptr(o, ctypes.c_double)
+        rc = lib().sis_batch_finalize(self.h, res)
+        if rc:
+            _err(rc)
+        return [_result(res[j], outs[j]) for j in range(self.k)]
+
+    def close(self):
+        if self.h:
+            lib().sis_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dev_depth32(pairs: np.ndarray):
+    """pairs: (N, 6) float32 rows dx, dy, ci, si, cj, sj -> (depth, depth^2)."""
+    N = pairs.shape[0]
+    buf = np.zeros((N, 8), dtype=np.float32)
+    buf[:, :6] = pairs
+    d = np.zeros(N, dtype=np.float32)
+    d2 = np.zeros(N, dtype=np.float32)
+    rc = lib().sis_dev_depth32(N, _ptr(buf, ctypes.c_float), _ptr(d, ctypes.c_float), _ptr(d2, ctypes.c_float))
+    if rc:
+        _err(rc)
+    return d, d2
+
+
+def dev_cellcoord(x: np.ndarray, shift: float, w: float, lo: int, hi: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.shape[0], dtype=np.int32)
+    rc = lib().sis_dev_cellcoord(x.shape[0], _ptr(x, ctypes.c_float), shift, w, lo, hi, _ptr(out, ctypes.c_int))
+    if rc:
+        _err(rc)
+    return out
+
+
+def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, step_th, key0: int,
+               key1: int, sweep0: int, sweeps: int, group: int = 1):
+    """xycs: (R, n, 4) float32 (x, y, cos, sin); returns (state, accepted, energy, counts)."""
+    R, n, _ = xycs.shape
+    st = np.ascontiguousarray(xycs, dtype=np.float32).copy()
+    ik = np.ascontiguousarray(np.broadcast_to(np.asarray(inv_t_log2e, np.float32), (R,)))
+    sx = np.ascontiguousarray(np.broadcast_to(np.asarray(step_xy, np.float32), (R,)))
+    sth = np.ascontiguousarray(np.broadcast_to(np.asarray(step_th, np.float32), (R,)))
+    acc = np.zeros(R, dtype=np.int32)
+    en = np.zeros(R, dtype=np.float32)
+    counts = np.zeros(3, dtype=np.uint64)
+    rc = lib().sis_dev_sweeps(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
+                              _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps,
+                              _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int), _ptr(en, ctypes.c_float),
+                              _ptr(counts, ctypes.c_uint64))
+    if rc:
+        _err(rc)
+    return st, acc, en, counts
+
+
+def dev_polish(q: np.ndarray, L: float, iters: int):
+    """FP64 CG polish + certified side; returns dict with energies, lambda, L_cert, export."""
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3).copy()
+    n = q.shape[0]
+    eb, ea, lam, lc = (ctypes.c_double() for _ in range(4))
+    out = np.zeros_like(q)
+    rc = lib().sis_dev_polish(n, L, iters, _ptr(q, ctypes.c_double), ctypes.byref(eb), ctypes.byref(ea),
+                              ctypes.byref(lam), ctypes.byref(lc), _ptr(out, ctypes.c_double))
+    if rc:
+        _err(rc)
+    return dict(e_before=eb.value, e_after=ea.value, lam=lam.value, L_cert=lc.value, q=out)

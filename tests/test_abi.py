@@ -1,0 +1,124 @@
+"""The C-ABI library and the `packing` CLI without a GPU: the library loads,
+exports every entry point include/sis_abi.h declares, validates input with the
+reference's error codes, and fails loudly (SIS_EDEVICE) when no device exists."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import PKG, ROOT
+
+HEADER = ROOT / "include" / "sis_abi.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sis_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(sis):
+    names = declared_functions()
+    assert len(names) >= 15
+    lib = ctypes.CDLL(str(PKG / "lib" / "libsisamd.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_default_params(sis):
+    p = sis.default_params()
+    assert p.tmin == 1e-5 and p.tmax == 5e-2
+    assert p.move_sigma == 0.05 and p.rot_sigma == 0.05 and p.target_accept == 0.3
+    assert p.shrink == 1e-4 and p.sweeps_per_exchange == 10 and p.polish_iters == 200
+    assert abs(p.cell - 0.7075) < 1e-12
+
+
+@pytest.mark.parametrize("n,L,kw", [
+    (0, 5.0, {}), (1001, 40.0, {}), (5, 0.0, {}), (5, 65.0, {}), (5, float("nan"), {}),
+    (5, 5.0, {"tmin": 0.0}), (5, 5.0, {"tmin": 1e-2, "tmax": 1e-3}), (5, 5.0, {"replicas": 0}),
+    (5, 5.0, {"replicas": 12, "ladder": 5}), (5, 5.0, {"cell": 0.5}), (5, 5.0, {"sweeps_per_exchange": 0}),
+])
+def test_invalid_input_is_einval(sis, n, L, kw):
+    with pytest.raises(sis.SisError) as e:
+        sis.solve(n, L, None, **kw)
+    assert e.value.code == sis.SIS_EINVAL
+
+
+def test_nonfinite_seed_is_einval(sis):
+    q = [[0.5, 0.5, 0.0], [float("nan"), 1.5, 0.0]]
+    with pytest.raises(sis.SisError) as e:
+        sis.solve(2, 3.0, q)
+    assert e.value.code == sis.SIS_EINVAL
+
+
+def _has_device(sis) -> bool:
+    return sis.lib().sis_init(0) == 0
+
+
+def test_no_device_fails_loudly(sis):
+    if _has_device(sis):
+        pytest.skip("a HIP device is present; covered by the GPU tests")
+    with pytest.raises(sis.SisError) as e:
+        sis.solve(5, 3.0, None, sweeps=10, replicas=8)
+    assert e.value.code == sis.SIS_EDEVICE
+
+
+# -- the `packing` CLI keeps the reference's process contract -----------------
+PACKING = PKG / "bin" / "packing"
+
+
+def run_packing(args, env=None, cwd=None):
+    e = dict(os.environ)
+    for k in list(e):
+        if k.startswith("PACK_"):
+            del e[k]
+    if env:
+        e.update(env)
+    return subprocess.run([str(PACKING), *args], capture_output=True, text=True, env=e, cwd=cwd, timeout=120)
+
+
+def test_cli_usage_exit_2():
+    r = run_packing(["only", "three", "args"])
+    assert r.returncode == 2 and "packing seed.txt" in r.stderr
+
+
+@pytest.mark.parametrize("args", [
+    ["-1", "0", "0"], ["100001", "0", "0"], ["0", "4097", "0"], ["0", "0", "12"], ["0", "0", "4104"],
+])
+def test_cli_argument_ranges(tmp_path, args):
+    seed = tmp_path / "seed.txt"
+    seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
+    r = run_packing([str(seed), *args, "42", str(tmp_path / "o.json")])
+    assert r.returncode == 2
+
+
+@pytest.mark.parametrize("text", ["", "0 3\n", "1001 3\n", "2 0\n", "2 65\n", "2 3\n0.5 0.5 0\n1.5 0.5\n",
+                                  "2 3\n0.5 nan 0\n1.5 0.5 0\n"])
+def test_cli_seed_file_validation(tmp_path, text):
+    seed = tmp_path / "seed.txt"
+    seed.write_text(text)
+    r = run_packing([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")])
+    assert r.returncode == 2
+
+
+@pytest.mark.parametrize("name,value", [("PACK_TMIN", "0"), ("PACK_TMAX", "2"), ("PACK_MOVE", "0.5"),
+                                        ("PACK_ROTATION", "abc"), ("PACK_SHRINK", "1"),
+                                        ("PACK_AUTOTUNE", "3"), ("PACK_MAX_CHAINS", "4")])
+def test_cli_env_knobs_range_checked(tmp_path, name, value):
+    seed = tmp_path / "seed.txt"
+    seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
+    r = run_packing([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")], env={name: value})
+    assert r.returncode == 2
+    assert f"invalid {name}" in r.stderr
+
+
+def test_cli_tmax_below_tmin(tmp_path):
+    seed = tmp_path / "seed.txt"
+    seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
+    r = run_packing([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")],
+                    env={"PACK_TMIN": "1e-3", "PACK_TMAX": "1e-4"})
+    assert r.returncode == 2

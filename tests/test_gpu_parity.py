@@ -1,0 +1,248 @@
+"""GPU parity: the gfx950 kernels, called through the C ABI, against the CPU
+oracle on the reference's corpora and seeded configurations."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import f32, f64, ptr
+
+pytestmark = pytest.mark.gpu
+
+LOG2E = 1.4426950408889634
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _corpus_pairs(q):
+    """All ordered pairs of every selftest block as (dx, dy, ci, si, cj, sj)."""
+    B, n, _ = q.shape
+    c = np.cos(q[:, :, 2]).astype(np.float32)
+    s = np.sin(q[:, :, 2]).astype(np.float32)
+    rows = []
+    for b in range(B):
+        dx = (q[b, None, :, 0] - q[b, :, None, 0]).astype(np.float32)  # x_j - x_i
+        dy = (q[b, None, :, 1] - q[b, :, None, 1]).astype(np.float32)
+        ci = np.broadcast_to(c[b, :, None], (n, n))
+        si = np.broadcast_to(s[b, :, None], (n, n))
+        cj = np.broadcast_to(c[b, None, :], (n, n))
+        sj = np.broadcast_to(s[b, None, :], (n, n))
+        rows.append(np.stack([dx, dy, ci, si, cj, sj], axis=-1).reshape(-1, 6))
+    return np.ascontiguousarray(np.concatenate(rows), dtype=np.float32)
+
+
+def test_depth32_bit_exact_and_fp64_tolerance(sis, oracle, corpus):
+    """250 000 selftest pairs: device == oracle restatement bit for bit; both
+    within 2e-6 of the FP64 axis oracle (cuda/tests/selftest.cu:53)."""
+    q = corpus["selftest"]
+    pairs = _corpus_pairs(q)
+    say("depth32", pairs.shape)
+    d, d2 = sis.dev_depth32(pairs)
+    L = oracle.lib
+    idx = np.random.default_rng(0).choice(len(pairs), 40000, replace=False)
+    for k in idx:
+        row = [float(v) for v in pairs[k]]
+        assert d[k] == np.float32(L.o_depth32_cs(*row))
+        assert d2[k] == np.float32(L.o_depth2_32_cs(*row))
+    worst = 0.0
+    B, n, _ = q.shape
+    for b in range(0, B, 10):
+        for i in range(n):
+            for j in range(n):
+                if i == j:
+                    continue
+                pi = np.ascontiguousarray(q[b, i]); pj = np.ascontiguousarray(q[b, j])
+                ref = L.o_sat_oracle_depth_f(ptr(pi, f32), ptr(pj, f32))
+                worst = max(worst, abs(float(d[(b * n + i) * n + j]) - ref))
+    assert worst <= 2e-6, worst
+    # edge fixtures of selftest.cu:38: touch / separated / overlapping
+    assert d[1] == 0.0 and d[2] == 0.0 and d[3] > 0.0
+
+
+def test_cell_binning_bit_exact(sis, oracle):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-3, 40, 20000), np.arange(0, 32, 0.25), [0.0, 2.0, 1.9999999, 31.999998]])
+    x = x.astype(np.float32)
+    say("cellcoord", x.shape)
+    ref = sis.dev_cellcoord(x, 0.0, 2.0, 0, 15)  # the reference's binning
+    for k in range(len(x)):
+        assert ref[k] == min(15, max(0, math.floor(float(x[k]) / 2)))
+    for shift, w in ((0.0, 0.7075), (0.31, 0.7075), (0.7, 0.75), (0.05, 1.0)):
+        out = sis.dev_cellcoord(x, shift, w, 0, 60)
+        for k in range(0, len(x), 7):
+            assert out[k] == oracle.lib.o_cellcoord(x[k], shift, w, 0, 60)
+
+
+def _seeded_state(rng, R, n, L):
+    x = rng.uniform(0.2, L - 0.2, (R, n))
+    y = rng.uniform(0.2, L - 0.2, (R, n))
+    a = rng.uniform(0, math.pi / 2, (R, n))
+    st = np.stack([x, y, np.cos(a), np.sin(a)], axis=-1).astype(np.float32)
+    return np.ascontiguousarray(st)
+
+
+def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
+    st = st.copy()
+    R, n, _ = st.shape
+    acc = np.zeros(R, np.int64)
+    cnt = np.zeros(3, np.uint64)
+    en = np.zeros(R, np.float32)
+    for r in range(R):
+        x, y, c, s = (np.ascontiguousarray(st[r, :, k]) for k in range(4))
+        for sw in range(sweep0, sweep0 + sweeps):
+            a, cc = oracle.sweep_model(x, y, c, s, L, w, sxy[r], sth[r], ik[r], key0, key1, r, sw)
+            acc[r] += a
+            cnt += cc
+        st[r, :, 0], st[r, :, 1], st[r, :, 2], st[r, :, 3] = x, y, c, s
+        en[r] = oracle.energy32(x, y, c, s, L, w)
+    return st, acc, en, cnt
+
+
+@pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (120, 12.5, 3, 3, 2)])
+def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
+    """k_sweep == the CPU model of the same colour-checkerboard sweep: same
+    states, accept counts and canonical energies, bit for bit."""
+    rng = np.random.default_rng(n)
+    st = _seeded_state(rng, R, n, L)
+    T = np.geomspace(1e-3, 1e-1, R)
+    ik = (LOG2E / T).astype(np.float32)
+    sxy = np.full(R, 0.12, np.float32)
+    sth = np.full(R, 0.1, np.float32)
+    say("sweeps", n, L, R, sweeps, group)
+    dst, dacc, den, dcnt = sis.dev_sweeps(st, L, 0.7075, ik, sxy, sth, 7, 0, 11, sweeps, group=group)
+    mst, macc, men, mcnt = _model_run(oracle, st, L, 0.7075, ik, sxy, sth, 7, 0, 11, sweeps)
+    assert np.array_equal(dacc, macc)
+    assert np.array_equal(dst, mst)
+    assert np.array_equal(den, men)
+    assert dcnt[0] == R * n * sweeps and np.array_equal(dcnt, mcnt)
+
+
+def test_sweep_energy_vs_fp64_recompute(sis, oracle):
+    """Energy after device sweeps equals the FP64 all-pairs recompute of the
+    same state within 1e-5 relative (cuda/tests/globaltest.cu:49)."""
+    rng = np.random.default_rng(4)
+    R, n, L = 8, 80, 10.0
+    st = _seeded_state(rng, R, n, L)
+    T = np.geomspace(1e-4, 5e-2, R)
+    ik = (LOG2E / T).astype(np.float32)
+    say("energy-vs-fp64", R, n)
+    for rnd in range(3):
+        st, acc, en, _ = sis.dev_sweeps(st, L, 0.7075, ik, np.full(R, 0.1, np.float32), np.full(R, 0.08, np.float32),
+                                        3, 1, rnd * 4, 4, group=4)
+        for r in range(R):
+            x, y, c, s = (np.ascontiguousarray(st[r, :, k], np.float64) for k in range(4))
+            h = 1 / np.sqrt(c * c + s * s)
+            c, s = np.ascontiguousarray(c * h), np.ascontiguousarray(s * h)
+            e64 = oracle.lib.o_total64_cs(n, ptr(x, f64), ptr(y, f64), ptr(c, f64), ptr(s, f64), L)
+            assert abs(en[r] - e64) <= 1e-5 * max(1.0, e64), (r, en[r], e64)
+
+
+def test_replicas_per_group_do_not_change_results(sis):
+    """Determinism: the tile shape (replicas per workgroup) never changes any
+    replica's arithmetic."""
+    rng = np.random.default_rng(9)
+    R, n, L = 12, 40, 7.5
+    st = _seeded_state(rng, R, n, L)
+    ik = (LOG2E / np.geomspace(1e-3, 1e-1, R)).astype(np.float32)
+    outs = []
+    for g in (1, 4, 12):
+        say("group", g)
+        outs.append(sis.dev_sweeps(st, L, 0.7075, ik, np.full(R, 0.1, np.float32), np.full(R, 0.1, np.float32),
+                                   5, 6, 0, 8, group=g))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_globaltest_grid_n1000(sis, oracle):
+    """cuda/tests/globaltest.cu:6-38 analogue: n=1000 grid at pitch 1.000001,
+    L=32.0001, 8 replicas, sweeps, energy vs FP64 host recompute (1e-5)."""
+    n, R, L = 1000, 8, 32.0001
+    i = np.arange(n)
+    st = np.zeros((R, n, 4), np.float32)
+    st[:, :, 0] = (i % 32 + 0.5) * 1.000001
+    st[:, :, 1] = (i // 32 + 0.5) * 1.000001
+    st[:, :, 2] = 1.0
+    ik = (LOG2E / np.geomspace(1e-7, 1e-2, R)).astype(np.float32)
+    say("globaltest n=1000")
+    st2, acc, en, cnt = sis.dev_sweeps(st, L, 0.7075, ik, np.full(R, 0.015, np.float32),
+                                       np.full(R, 0.02, np.float32), 142, 0, 0, 3, group=1)
+    assert cnt[0] == 3 * n * R
+    for r in range(R):
+        x, y, c, s = (np.ascontiguousarray(st2[r, :, k], np.float64) for k in range(4))
+        h = 1 / np.sqrt(c * c + s * s)
+        c, s = np.ascontiguousarray(c * h), np.ascontiguousarray(s * h)
+        e64 = oracle.lib.o_total64_cs(n, ptr(x, f64), ptr(y, f64), ptr(c, f64), ptr(s, f64), L)
+        assert abs(en[r] - e64) <= 1e-5, (r, en[r], e64)
+
+
+def test_polish_and_certified_side_match_oracle(sis, oracle):
+    rng = np.random.default_rng(2)
+    for n, k in ((16, 4), (30, 6)):
+        grid = np.zeros((n, 3))
+        oracle.lib.o_grid_packing(n, ptr(grid, f64))
+        q = grid.copy()
+        q[:, :2] = q[:, :2] * 0.99 + rng.uniform(-0.01, 0.01, (n, 2))
+        q[:, 2] = rng.uniform(0, 0.03, n)
+        L = k * 0.995
+        say("polish", n)
+        dev = sis.dev_polish(q, L, 200)
+        oq, oe, _ = oracle.polish64(q, L, 200)
+        assert abs(dev["e_before"] - oracle.total64(q, L)) <= 1e-12 * max(1, dev["e_before"])
+        assert dev["e_after"] <= dev["e_before"]
+        assert abs(dev["e_after"] - oe) <= 1e-9 * max(1e-12, oe) + 1e-20
+        side, lam, out = oracle.certified_side(oq)
+        assert abs(dev["lam"] - lam) <= 1e-9
+        if side < 1e8:
+            assert abs(dev["L_cert"] - side) <= 1e-9
+            assert np.max(np.abs(dev["q"] - out)) <= 1e-9
+
+
+def test_solve_n11_certifies(sis, oracle, ref_certifier):
+    """End to end (BASELINE config 1 shape on the GPU): the exported packing is
+    overlap-free in FP64 and passes the reference's interval certifier after
+    the dilation ladder of cuda/demo.py:31."""
+    from refcert import certify_q
+    say("solve n=11")
+    r = sis.solve(11, 5.0, None, replicas=8, tmin=1e-4, tmax=1e-1, sweeps=4000, sweeps_per_exchange=10,
+                  shrink=1e-3, rng_seed=42, init_mode=1)
+    assert r.proposals == 8 * 11 * 4000
+    assert r.feasible == 1 and r.shrinks > 0
+    assert math.sqrt(11) - 1e-7 <= r.L_cert < 5.0
+    q = r.q
+    for i in range(11):
+        for j in range(i + 1, 11):
+            assert oracle.lib.o_sat_oracle64(ptr(q[i].copy(), f64), ptr(q[j].copy(), f64)) == 0.0
+    dil, L50 = certify_q(ref_certifier, q)
+    assert dil is not None
+    assert float(L50) <= r.L_cert * dil + 1e-9
+
+
+def test_batch_equals_alone(sis):
+    """Alone vs in the mix: each instance's result is bit-identical whether it
+    runs alone (plain) or inside a batch with others of different n."""
+    insts = [(13, 5.0, None), (37, 8.0, None), (60, 9.5, None)]
+    ps = [sis.default_params(replicas=r, sweeps=s, rng_seed=seed, init_mode=1, tmin=1e-4, tmax=3e-2)
+          for r, s, seed in ((16, 300, 1), (32, 200, 2), (8, 250, 3))]
+    say("batch")
+    mix = sis.solve_batch(insts, ps)
+    for (n, L, q), p, m in zip(insts, ps, mix):
+        say("alone", n)
+        a = sis.solve(n, L, q, params=p)
+        assert np.array_equal(a.q, m.q)
+        for k in ("proposals", "accepted", "shrinks", "exchanges_accepted", "L_cert", "target_L", "e64_after"):
+            assert a.fields[k] == m.fields[k], k
+
+
+def test_equal_temperatures_accept_every_exchange(sis):
+    """Forced exchange (cuda/tests/globaltest.cu:39-46): with tmin == tmax the
+    criterion is log u < 0, so every attempted swap is taken."""
+    say("forced exchange")
+    r = sis.solve(20, 6.0, None, replicas=16, tmin=1e-2, tmax=1e-2, sweeps=200, sweeps_per_exchange=5,
+                  init_mode=1, shrink=0.0)
+    assert r.exchanges_tried == 20 * 8 + 20 * 7
+    assert r.exchanges_accepted == r.exchanges_tried
