@@ -63,6 +63,55 @@ def cpu_baseline(target_s: float = 12.0):
                       f"PT-SA in {r['seconds']:.2f} s on {threads} threads"}
 
 
+def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
+    """Mixed-n stream (512 instances, n ~ U{20..200} with seed 3, 64 replicas
+    each): all instances in one HIP-graph schedule vs the plain form (one
+    instance per launch sequence, host sync per interval) on a sample of the
+    same stream.  Returns proposals/s of both and the sweep-kernel roofline of
+    the batched form."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    ns = rng.integers(20, 201, n_inst)
+    insts = [(int(n), float(math.ceil(math.sqrt(n)) + 1), None) for n in ns]
+    params = [sis.default_params(tmin=1e-5, tmax=5e-2, replicas=64, sweeps=sweeps, sweeps_per_exchange=SPE,
+                                 rng_seed=1000 + k, init_mode=1, polish_iters=200) for k in range(n_inst)]
+    sess = sis.BatchSession(insts, params)
+    warm = 5
+    sess.run(warm, mode=1)
+    t0 = time.perf_counter()
+    _, _, dev_ms = sess.run(0, mode=1)
+    wall = time.perf_counter() - t0
+    res = sess.finalize()
+    sess.close()
+    timed = sum(n * 64 * (sweeps - warm * SPE) for n, _, _ in insts)
+    batched = timed / wall
+    props = sum(r.proposals for r in res)
+    c_per = sum(r.candidates for r in res) / props
+    p_per = sum(r.near_pairs for r in res) / props
+    feas = sum(r.feasible for r in res)
+    # plain form on a sample of the same stream
+    plain_props, plain_wall, sweep_ms = 0, 0.0, 0.0
+    for k in range(plain_sample):
+        s = sis.BatchSession([insts[k]], params[k])
+        t1 = time.perf_counter()
+        sm, _, _ = s.run(0, mode=0)
+        plain_wall += time.perf_counter() - t1
+        plain_props += insts[k][0] * 64 * sweeps
+        sweep_ms += sm
+        s.close()
+    plain = plain_props / plain_wall
+    return {
+        "workload": f"{n_inst} instances, n ~ U{{20..200}} (seed 3), 64 replicas each, {sweeps} sweeps each, "
+                    "exchange every 10 sweeps; batched = one HIP-graph schedule over all instances",
+        "batched_proposals_per_s": batched, "plain_proposals_per_s": plain, "ratio": batched / plain,
+        "plain_sample_instances": plain_sample, "timed_proposals": timed, "wall_s": wall,
+        "device_ms": dev_ms, "feasible_instances": feas,
+        "flop_per_proposal": 12.0 * c_per + 28.0 * p_per + 45.0,
+        "achieved_tflops_wall": timed * (12.0 * c_per + 28.0 * p_per + 45.0) / wall / 1e12,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +119,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batched-instances", type=int, default=512)
+    ap.add_argument("--batched-sweeps", type=int, default=400)
+    ap.add_argument("--plain-sample", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -138,6 +191,8 @@ def main():
             "exchange_rate": res.exchanges_accepted / max(1, res.exchanges_tried),
         },
     }
+    if not a.no_batched:
+        line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -59,7 +59,8 @@ struct SlotState {              // per global temperature slot
 };
 
 // LDS of one sweep tile (G replicas of n squares, M x M cells).
-__host__ __device__ inline int grid_words(int M) { return (M * M + 3) & ~1; }
+// Cell starts per replica: M*M + 2 u16, padded to 16 bytes.
+__host__ __device__ inline int grid_words(int M) { return (M * M + 2 + 7) & ~7; }
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     size_t gn = (size_t)G * n;
     size_t b = 16 * gn;                         // float4 x, y, cos, sin

@@ -105,24 +105,41 @@ __device__ __forceinline__ unsigned add_u16(uint16_t* base, int idx, unsigned v)
     return (atomicAdd(w, v << sh) >> sh) & 0xFFFFu;
 }
 
-// Inclusive prefix sum of K u16 values by one wavefront: contiguous chunk per
-// lane, wave scan of the chunk totals, second pass writes.
+// Inclusive prefix sum of the u16 array a[0..K) (K a multiple of 8, 16-byte
+// aligned) by one wavefront: each lane owns a contiguous run of 16-byte
+// chunks, sums them, the lane totals are scanned across the wave, and a
+// second pass writes the prefixes.  Totals stay below 65536.
 __device__ void wave_scan_u16(uint16_t* a, int K) {
     const int lane = threadIdx.x;
-    const int chunk = (K + kSweepThreads - 1) / kSweepThreads;
-    const int b = lane * chunk, e = min(K, b + chunk);
+    uint4* v = reinterpret_cast<uint4*>(a);
+    const int nv = K >> 3;
+    const int per = (nv + kSweepThreads - 1) / kSweepThreads;
+    const int b = min(nv, lane * per), e = min(nv, b + per);
     unsigned sum = 0;
-    for (int k = b; k < e; ++k) sum += a[k];
+    for (int k = b; k < e; ++k) {
+        const uint4 w = v[k];
+        sum += (w.x & 0xFFFFu) + (w.x >> 16) + (w.y & 0xFFFFu) + (w.y >> 16) + (w.z & 0xFFFFu) +
+               (w.z >> 16) + (w.w & 0xFFFFu) + (w.w >> 16);
+    }
     unsigned incl = sum;
 #pragma unroll
     for (int off = 1; off < kSweepThreads; off <<= 1) {
-        unsigned v = __shfl_up(incl, off, kSweepThreads);
-        if (lane >= off) incl += v;
+        const unsigned t = __shfl_up(incl, off, kSweepThreads);
+        if (lane >= off) incl += t;
     }
     unsigned run = incl - sum;
     for (int k = b; k < e; ++k) {
-        run += a[k];
-        a[k] = (uint16_t)run;
+        uint4 w = v[k];
+        unsigned p0 = run + (w.x & 0xFFFFu), p1 = p0 + (w.x >> 16);
+        unsigned p2 = p1 + (w.y & 0xFFFFu), p3 = p2 + (w.y >> 16);
+        unsigned p4 = p3 + (w.z & 0xFFFFu), p5 = p4 + (w.z >> 16);
+        unsigned p6 = p5 + (w.w & 0xFFFFu), p7 = p6 + (w.w >> 16);
+        run = p7;
+        w.x = p0 | (p1 << 16);
+        w.y = p2 | (p3 << 16);
+        w.z = p4 | (p5 << 16);
+        w.w = p6 | (p7 << 16);
+        v[k] = w;
     }
 }
 
@@ -168,11 +185,14 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
 
 // Cell lists of every replica of the tile in canonical order (ascending
 // square index inside a cell), with the occupancy rank of each square.
+// One prefix sum runs over the cell counts of all G replicas back to back, so
+// A holds tile-global positions into `sorted` (replica g's squares occupy
+// sorted[g*n, (g+1)*n), each replica contributing exactly n counts).
 __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) {
     const int lane = threadIdx.x;
     const int gw = grid_words(M);
-    unsigned* A32 = reinterpret_cast<unsigned*>(t.A);
-    for (int k = lane; k < (G * gw) / 2; k += kSweepThreads) A32[k] = 0u;
+    uint4* A128 = reinterpret_cast<uint4*>(t.A);
+    for (int k = lane; k < (G * gw) / 8; k += kSweepThreads) A128[k] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     for (int q = lane; q < G * n; q += kSweepThreads) {
         const int g = q / n;
@@ -183,14 +203,14 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) 
         add_u16(t.A + g * gw, ky * M + kx + 2, 1u);
     }
     __syncthreads();
-    for (int g = 0; g < G; ++g) wave_scan_u16(t.A + g * gw, M * M + 2);
+    wave_scan_u16(t.A, G * gw);
     __syncthreads();
     for (int q = lane; q < G * n; q += kSweepThreads) {
         const int g = q / n, i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
         const unsigned slot = add_u16(t.A + g * gw, c + 1, 1u);
-        t.sorted[g * n + slot] = (uint16_t)i;
+        t.sorted[slot] = (uint16_t)i;
     }
     __syncthreads();
     for (int q = lane; q < G * n; q += kSweepThreads) {
@@ -200,7 +220,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) 
         const uint16_t* Ag = t.A + g * gw;
         const int a = Ag[c], b = Ag[c + 1];
         int r = 0;
-        for (int k = a; k < b; ++k) r += t.sorted[g * n + k] < i;
+        for (int k = a; k < b; ++k) r += t.sorted[k] < i;
         t.rank[q] = (uint16_t)r;
     }
     __syncthreads();
@@ -208,7 +228,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) 
         const int g = q / n, i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
-        t.sorted[g * n + t.A[g * gw + c] + t.rank[q]] = (uint16_t)i;
+        t.sorted[t.A[g * gw + c] + t.rank[q]] = (uint16_t)i;
     }
     __syncthreads();
 }
@@ -238,7 +258,6 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     rot32(p.z, p.w, t.sth[g] * ft, nc, ns);
     float dE = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
     const uint16_t* Ag = t.A + g * gw;
-    const uint16_t* sg = t.sorted + g * n;
     const float4* pg = t.pos + g * n;
     unsigned cnt = 0, near = 0;
 #pragma unroll 1
@@ -246,7 +265,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         const int row = (ky + oy) * M + kx;
         const int a = Ag[row - 2], b = Ag[row + 3];
         for (int k = a; k < b; ++k) {
-            const int j = sg[k];
+            const int j = t.sorted[k];
             if (j == i) continue;
             const float4 v = pg[j];
             const float dxn = v.x - nx, dyn = v.y - ny;
@@ -285,14 +304,13 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const int ck = t.cellk[q];
         const int kx = ck & 255, ky = ck >> 8;
         const uint16_t* Ag = t.A + g * gw;
-        const uint16_t* sg = t.sorted + g * n;
         const float4* pg = t.pos + g * n;
         float ei = wall32(p.x, p.y, p.z, p.w, L);
         for (int oy = -2; oy <= 2; ++oy) {
             const int row = (ky + oy) * M + kx;
             const int a = Ag[row - 2], b = Ag[row + 3];
             for (int k = a; k < b; ++k) {
-                const int j = sg[k];
+                const int j = t.sorted[k];
                 if (j <= i) continue;
                 const float4 v = pg[j];
                 ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);

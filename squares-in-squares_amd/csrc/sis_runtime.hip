@@ -54,7 +54,7 @@ int ensure_device(int device) {
     g_dev.id = id;
     g_dev.cus = prop.multiProcessorCount;
     g_dev.lds_per_block = (size_t)prop.sharedMemPerBlock;
-    g_dev.name = prop.name;
+    g_dev.name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
     g_dev.arch = prop.gcnArchName;
     return SIS_OK;
 }
@@ -170,8 +170,11 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
     pl.M = grid_cells((float)s.L, w) + 4;
     pl.ladder = s.p.ladder > 0 ? s.p.ladder : s.p.replicas;
     pl.spe = s.p.sweeps_per_exchange;
-    const size_t budget = std::min<size_t>(64 * 1024, dev.lds_per_block ? dev.lds_per_block : 65536);
+    const size_t hard = std::min<size_t>(64 * 1024, dev.lds_per_block ? dev.lds_per_block : 65536);
     int G = s.p.replicas_per_group;
+    // Automatic tiles aim at 20 KB of LDS so that eight tiles (eight waves)
+    // share a 160 KB compute unit; an explicit group size may use up to 64 KB.
+    const size_t budget = G > 0 ? hard : std::min<size_t>(hard, 20 * 1024);
     if (G <= 0) {
         // Fill the 64 lanes of a colour group (about G*n/13 movers) but keep
         // at least two tiles per compute unit across the whole batch.
@@ -184,8 +187,8 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
     }
     G = std::min(G, s.p.replicas);
     while (G > 1 && (sweep_lds_bytes(s.n, pl.M, G) > budget || G * s.n > 65535)) --G;
-    if (sweep_lds_bytes(s.n, pl.M, G) > budget)
-        return fail(SIS_EINVAL, "one replica does not fit the LDS budget");
+    if (sweep_lds_bytes(s.n, pl.M, G) > hard)
+        return fail(SIS_EINVAL, "one replica does not fit the LDS of a workgroup");
     pl.G = G;
     pl.lds = sweep_lds_bytes(s.n, pl.M, G);
     pl.ntiles = (s.p.replicas + G - 1) / G;

@@ -18,11 +18,5 @@ int sis_ticket_times(sis_server*, uint64_t, double*, double*, double*, char*, in
     return sis::fail(SIS_EINVAL, "server not available in this build");
 }
 int sis_server_destroy(sis_server*) { return SIS_OK; }
-int sis_certify_strings(int, const char* const*, const char* const*, const char* const*, char*, int) {
-    return sis::fail(SIS_EINVAL, "certifier not available in this build");
-}
-int sis_export_certify(int, const double*, double*, char*, int) {
-    return sis::fail(SIS_EINVAL, "certifier not available in this build");
-}
 
 }  // extern "C"
