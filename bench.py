@@ -112,6 +112,76 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
     }
 
 
+def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_sample: int, seed: int = 5):
+    """Serving stream: Poisson arrivals at `rate`/s for `duration` s; n
+    log-uniform in 10..1000; sweeps U(5e4, 5e5) x sweep_scale; replicas
+    U{32, 64, ..., 256}; half the requests with a 200-iteration FP64 polish,
+    half without; every result certified exactly.  Reports p50/p99 time to a
+    certified L and aggregate proposals/s, and the plain form (requests one
+    at a time, host sync per interval) on a sample of the same stream."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        n = int(round(math.exp(rng.uniform(math.log(10), math.log(1000)))))
+        sweeps = max(SPE, int(rng.uniform(5e4, 5e5) * sweep_scale) // SPE * SPE)
+        R = int(rng.integers(1, 9)) * 32
+        polish = 200 if rng.random() < 0.5 else 0
+        reqs.append((t, n, sweeps, R, polish, 100 + len(reqs)))
+    cpu = max(1, min(16, os.cpu_count() or 1))
+    srv = sis.Server(max_instances=1024, intervals_per_round=5, certifier_threads=cpu, certify=True,
+                     hbm_budget_bytes=200e9)
+    tickets = []
+    t0 = time.perf_counter()
+    for (ta, n, sweeps, R, polish, s) in reqs:
+        while time.perf_counter() - t0 < ta:
+            time.sleep(0.0005)
+        L0 = float(math.ceil(math.sqrt(n)) + 1)
+        tickets.append(srv.submit(n, L0, None, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s,
+                                  init_mode=1, tmin=1e-5, tmax=5e-2))
+    results = {}
+    while len(results) < len(tickets):
+        for tk in tickets:
+            if tk not in results:
+                r = srv.poll(tk)
+                if r is not None:
+                    results[tk] = r
+        time.sleep(0.002)
+    wall = time.perf_counter() - t0
+    lat, cert_ok = [], 0
+    for tk in tickets:
+        _, _, tc, _ = srv.times(tk)
+        lat.append(tc)
+        cert_ok += results[tk].status == 0
+    srv.close()
+    props = sum(results[tk].proposals for tk in tickets)
+    lat = np.array(lat)
+    # plain form on a sample of the same requests
+    pp, pw = 0, 0.0
+    for (ta, n, sweeps, R, polish, s) in reqs[:plain_sample]:
+        L0 = float(math.ceil(math.sqrt(n)) + 1)
+        t1 = time.perf_counter()
+        r = sis.solve(n, L0, None, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=1,
+                      tmin=1e-5, tmax=5e-2)
+        pw += time.perf_counter() - t1
+        pp += r.proposals
+    return {
+        "workload": f"Poisson arrivals {rate}/s for {duration} s ({len(reqs)} requests), n log-uniform 10..1000, "
+                    f"sweeps U(5e4,5e5) x {sweep_scale}, replicas U{{32..256}}, polish 0/200 mixed, certified",
+        "requests": len(reqs), "certified": cert_ok, "served_proposals_per_s": props / wall,
+        "p50_time_to_certified_L_s": float(np.percentile(lat, 50)),
+        "p99_time_to_certified_L_s": float(np.percentile(lat, 99)),
+        "makespan_s": wall, "plain_proposals_per_s": pp / pw if pw else None,
+        "ratio_served_vs_plain": (props / wall) / (pp / pw) if pw else None,
+        "plain_sample_requests": plain_sample,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +193,10 @@ def main():
     ap.add_argument("--batched-instances", type=int, default=512)
     ap.add_argument("--batched-sweeps", type=int, default=400)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--no-served", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-sweep-scale", type=float, default=0.02)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -193,6 +267,8 @@ def main():
     }
     if not a.no_batched:
         line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
+    if not a.no_served:
+        line["served"] = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale, 4)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(line), flush=True)

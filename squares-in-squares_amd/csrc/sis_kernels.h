@@ -20,8 +20,10 @@ struct FinalizeOut {
 
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
                         InstDesc* insts, RepState* reps, const SlotState* slots, float4* state);
+// list: ninst instance indices into insts, or null for insts[0..ninst).
 hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
-                         SlotState* slots, int* rep_of_slot, float4* state, float4* arch);
+                         SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
+                         const int* list = nullptr);
 hipError_t launch_finalize(hipStream_t st, int njobs, const FinalizeJob* jobs, FinalizeOut* outs,
                            double* q_out);
 hipError_t launch_depth32(hipStream_t st, int npairs, const float* in, float* depth, float* d2);

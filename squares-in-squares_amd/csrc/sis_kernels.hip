@@ -560,9 +560,11 @@ __device__ double total64(const double* x, const double* y, const double* c, con
 }
 
 // ---------------------------------------------------------------------------
-// k_ladder: one workgroup per instance, after every sweep launch.
+// k_ladder: one workgroup per instance (insts[list[b]], or insts[b] without a
+// list), after every sweep launch.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict__ insts,
+                                                           const int* __restrict__ list,
                                                            RepState* __restrict__ reps,
                                                            SlotState* __restrict__ slots,
                                                            int* __restrict__ rep_of_slot,
@@ -571,7 +573,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     __shared__ double x[kMaxN], y[kMaxN], cc[kMaxN], ss[kMaxN];
     __shared__ double tmp[kLadderThreads];
     __shared__ int zero_slot;
-    InstDesc* D = insts + blockIdx.x;
+    InstDesc* D = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
     const long long left = D->sweeps_total - D->sweeps_done;
     if (left <= 0) return;
     const int ran = (int)min((long long)D->spe, left);
@@ -851,8 +853,9 @@ hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tile
 }
 
 hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
-                         SlotState* slots, int* rep_of_slot, float4* state, float4* arch) {
-    hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), 0, st, insts, reps, slots,
+                         SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
+                         const int* list) {
+    hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), 0, st, insts, list, reps, slots,
                        rep_of_slot, state, arch);
     return hipGetLastError();
 }

@@ -263,6 +263,73 @@ class BatchSession:
             pass
 
 
+class Server:
+    """Continuous-batching request server (sis_server_*)."""
+
+    def __init__(self, max_instances=256, intervals_per_round=10, certifier_threads=8, certify=True,
+                 hbm_budget_bytes=200e9):
+        cfg = SisServerCfg(hbm_budget_bytes, max_instances, intervals_per_round, certifier_threads, int(certify))
+        self.h = lib().sis_server_create(ctypes.byref(cfg))
+        if not self.h:
+            raise SisError(SIS_EDEVICE, lib().sis_last_error().decode(errors="replace"))
+        self.ns = {}
+
+    def submit(self, n, L, q=None, params: SisParams | None = None, **kw) -> int:
+        p = params if params is not None else default_params(**kw)
+        inst, keep = _instance(n, L, q)
+        t = ctypes.c_uint64()
+        rc = lib().sis_submit(self.h, ctypes.byref(inst), ctypes.byref(p), ctypes.byref(t))
+        if rc:
+            _err(rc)
+        del keep
+        self.ns[t.value] = n
+        return t.value
+
+    def poll(self, ticket: int):
+        """Result when finished, else None."""
+        n = self.ns[ticket]
+        out = np.zeros((n, 3), dtype=np.float64)
+        r = SisResult()
+        r.q_out = _ptr(out, ctypes.c_double)
+        rc = lib().sis_poll(self.h, ticket, ctypes.byref(r))
+        if rc == SIS_PENDING:
+            return None
+        if rc:
+            _err(rc)
+        return _result(r, out)
+
+    def times(self, ticket: int):
+        """(queued_s, device_done_s, certified_s, L50 or rejection reason)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        buf = ctypes.create_string_buffer(128)
+        rc = lib().sis_ticket_times(self.h, ticket, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), buf, 128)
+        if rc not in (SIS_OK, SIS_PENDING):
+            _err(rc)
+        return a.value, b.value, c.value, buf.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().sis_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def export_certify(q: np.ndarray):
+    """Dilation ladder + exact certificate; returns (dilation, L50) or raises."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    d = ctypes.c_double()
+    buf = ctypes.create_string_buffer(64)
+    rc = lib().sis_export_certify(q.shape[0], _ptr(q, ctypes.c_double), ctypes.byref(d), buf, 64)
+    if rc:
+        _err(rc)
+    return d.value, buf.value.decode()
+
+
 def dev_depth32(pairs: np.ndarray):
     """pairs: (N, 6) float32 rows dx, dy, ci, si, cj, sj -> (depth, depth^2)."""
     N = pairs.shape[0]

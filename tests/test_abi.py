@@ -122,3 +122,11 @@ def test_cli_tmax_below_tmin(tmp_path):
     r = run_packing([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")],
                     env={"PACK_TMIN": "1e-3", "PACK_TMAX": "1e-4"})
     assert r.returncode == 2
+
+
+def test_server_without_device_fails_loudly(sis):
+    if _has_device(sis):
+        pytest.skip("a HIP device is present; covered by the GPU tests")
+    with pytest.raises(sis.SisError) as e:
+        sis.Server(max_instances=4)
+    assert e.value.code == sis.SIS_EDEVICE

@@ -246,3 +246,33 @@ def test_equal_temperatures_accept_every_exchange(sis):
                   init_mode=1, shrink=0.0)
     assert r.exchanges_tried == 20 * 8 + 20 * 7
     assert r.exchanges_accepted == r.exchanges_tried
+
+
+def test_packing_cli_end_to_end(tmp_path, ref_certifier):
+    """`packing seed.txt epochs steps chains rng_seed out.json` keeps the raw
+    JSON contract (cuda/packing.cu:859-865) and its output certifies through
+    the export gate of cuda/demo.py:25-40."""
+    import json
+    import subprocess
+
+    from conftest import PKG
+    from refcert import certify_q
+    n = 11
+    w = math.isqrt(n - 1) + 1
+    f = 1.00002
+    rows = [((i % w + 0.5) * f, (i // w + 0.5) * f, 0.0) for i in range(n)]
+    seed = tmp_path / "seed11.txt"
+    seed.write_text(f"{n} {w * f:.12g}\n" + "".join(f"{x:.12g} {y:.12g} {a:.12g}\n" for x, y, a in rows))
+    out = tmp_path / "raw11.json"
+    say("packing cli")
+    r = subprocess.run([str(PKG / "bin" / "packing"), str(seed), "20", "32", "64", "42", str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    raw = json.loads(out.read_text())
+    for k in ("n", "target_L", "seconds", "chain_steps", "proposals", "feasible_checkpoints", "shrinks", "chains",
+              "sm_count", "blocks_per_sm", "wave_chains", "device", "q"):
+        assert k in raw, k
+    assert raw["n"] == n and len(raw["q"]) == n and raw["chains"] >= 64 and raw["chains"] % 8 == 0
+    assert raw["target_L"] <= w * f + 1e-12
+    dil, L50 = certify_q(ref_certifier, np.asarray(raw["q"]))
+    assert dil is not None
