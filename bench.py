@@ -112,13 +112,29 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
     }
 
 
-def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_sample: int, seed: int = 5):
+def grid_seed(n: int):
+    """Certified start of a request: the half-integer grid of side
+    ceil(sqrt(n)) with centres and side dilated by 1.00002, as the reference
+    seeds its searches (cuda/demo.py:14-23)."""
+    import numpy as np
+
+    w = math.isqrt(n - 1) + 1
+    i = np.arange(n)
+    q = np.zeros((n, 3))
+    q[:, 0] = (i % w + 0.5) * 1.00002
+    q[:, 1] = (i // w + 0.5) * 1.00002
+    return float(w * 1.00002), q
+
+
+def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seconds: float, seed: int = 5):
     """Serving stream: Poisson arrivals at `rate`/s for `duration` s; n
     log-uniform in 10..1000; sweeps U(5e4, 5e5) x sweep_scale; replicas
     U{32, 64, ..., 256}; half the requests with a 200-iteration FP64 polish,
-    half without; every result certified exactly.  Reports p50/p99 time to a
-    certified L and aggregate proposals/s, and the plain form (requests one
-    at a time, host sync per interval) on a sample of the same stream."""
+    half without; every request starts from its certified grid seed and every
+    result is certified exactly.  Reports p50/p99 time to a certified L and
+    aggregate proposals/s, and the plain form (requests one at a time, host
+    sync per interval) on the same requests in arrival order for about
+    `plain_seconds`."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -141,9 +157,9 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_samp
     for (ta, n, sweeps, R, polish, s) in reqs:
         while time.perf_counter() - t0 < ta:
             time.sleep(0.0005)
-        L0 = float(math.ceil(math.sqrt(n)) + 1)
-        tickets.append(srv.submit(n, L0, None, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s,
-                                  init_mode=1, tmin=1e-5, tmax=5e-2))
+        L0, q0 = grid_seed(n)
+        tickets.append(srv.submit(n, L0, q0, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s,
+                                  init_mode=0, tmin=1e-5, tmax=5e-2))
     results = {}
     while len(results) < len(tickets):
         for tk in tickets:
@@ -153,32 +169,39 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_samp
                     results[tk] = r
         time.sleep(0.002)
     wall = time.perf_counter() - t0
-    lat, cert_ok = [], 0
+    lat, cert_ok, Ls = [], 0, {}
     for tk in tickets:
-        _, _, tc, _ = srv.times(tk)
+        _, _, tc, L50 = srv.times(tk)
         lat.append(tc)
         cert_ok += results[tk].status == 0
+        Ls[tk] = L50
     srv.close()
     props = sum(results[tk].proposals for tk in tickets)
     lat = np.array(lat)
-    # plain form on a sample of the same requests
-    pp, pw = 0, 0.0
-    for (ta, n, sweeps, R, polish, s) in reqs[:plain_sample]:
-        L0 = float(math.ceil(math.sqrt(n)) + 1)
+    # plain form on the same requests in arrival order; identical results
+    pp, pw, k, same = 0, 0.0, 0, 0
+    while k < len(reqs) and pw < plain_seconds:
+        ta, n, sweeps, R, polish, s = reqs[k]
+        L0, q0 = grid_seed(n)
         t1 = time.perf_counter()
-        r = sis.solve(n, L0, None, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=1,
+        r = sis.solve(n, L0, q0, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=0,
                       tmin=1e-5, tmax=5e-2)
         pw += time.perf_counter() - t1
         pp += r.proposals
+        same += bool(np.array_equal(r.q, results[tickets[k]].q) and r.L_cert == results[tickets[k]].L_cert)
+        k += 1
+    served = props / wall
     return {
         "workload": f"Poisson arrivals {rate}/s for {duration} s ({len(reqs)} requests), n log-uniform 10..1000, "
-                    f"sweeps U(5e4,5e5) x {sweep_scale}, replicas U{{32..256}}, polish 0/200 mixed, certified",
-        "requests": len(reqs), "certified": cert_ok, "served_proposals_per_s": props / wall,
+                    f"sweeps U(5e4,5e5) x {sweep_scale}, replicas U{{32..256}}, polish 0/200 mixed, grid seeds, "
+                    "every result certified",
+        "requests": len(reqs), "certified": cert_ok, "served_proposals_per_s": served,
         "p50_time_to_certified_L_s": float(np.percentile(lat, 50)),
         "p99_time_to_certified_L_s": float(np.percentile(lat, 99)),
         "makespan_s": wall, "plain_proposals_per_s": pp / pw if pw else None,
-        "ratio_served_vs_plain": (props / wall) / (pp / pw) if pw else None,
-        "plain_sample_requests": plain_sample,
+        "ratio_served_vs_plain": served / (pp / pw) if pw else None,
+        "plain_requests": k, "plain_results_identical": same,
+        "fraction_of_fp32_peak": None,
     }
 
 
@@ -196,7 +219,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-sweep-scale", type=float, default=0.02)
+    ap.add_argument("--serve-sweep-scale", type=float, default=0.05)
+    ap.add_argument("--plain-seconds", type=float, default=8.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -268,7 +292,7 @@ def main():
     if not a.no_batched:
         line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
     if not a.no_served:
-        line["served"] = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale, 4)
+        line["served"] = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale, a.plain_seconds)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(line), flush=True)
