@@ -63,7 +63,7 @@ def cpu_baseline(target_s: float = 12.0):
                       f"PT-SA in {r['seconds']:.2f} s on {threads} threads"}
 
 
-def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
+def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool = True):
     """Mixed-n stream (512 instances, n ~ U{20..200} with seed 3, 64 replicas
     each): all instances in one HIP-graph schedule vs the plain form (one
     instance per launch sequence, host sync per interval) on a sample of the
@@ -76,15 +76,24 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
     insts = [(int(n), float(math.ceil(math.sqrt(n)) + 1), None) for n in ns]
     params = [sis.default_params(tmin=1e-5, tmax=5e-2, replicas=64, sweeps=sweeps, sweeps_per_exchange=SPE,
                                  rng_seed=1000 + k, init_mode=1, polish_iters=200) for k in range(n_inst)]
+    tune = None
+    if autotune:
+        # autotuner on a 64-instance slice of the stream, 20 probe sweeps per candidate
+        tune = sis.autotune(insts[:64], params[:64], probe_sweeps=20)
+        for p in params:
+            p.replicas_per_group = tune["replicas_per_group"]
+            p.sweeps_per_exchange = tune["sweeps_per_exchange"]
+            p.cell = tune["cell"]
+    spe = params[0].sweeps_per_exchange
     sess = sis.BatchSession(insts, params)
-    warm = 5
+    warm = max(1, 50 // spe)
     sess.run(warm, mode=1)
     t0 = time.perf_counter()
     _, _, dev_ms = sess.run(0, mode=1)
     wall = time.perf_counter() - t0
     res = sess.finalize()
     sess.close()
-    timed = sum(n * 64 * (sweeps - warm * SPE) for n, _, _ in insts)
+    timed = sum(n * 64 * (sweeps - warm * spe) for n, _, _ in insts)
     batched = timed / wall
     props = sum(r.proposals for r in res)
     c_per = sum(r.candidates for r in res) / props
@@ -100,13 +109,14 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int):
         plain_props += insts[k][0] * 64 * sweeps
         sweep_ms += sm
         s.close()
-    plain = plain_props / plain_wall
+    plain = plain_props / plain_wall if plain_wall > 0 else None
     return {
         "workload": f"{n_inst} instances, n ~ U{{20..200}} (seed 3), 64 replicas each, {sweeps} sweeps each, "
                     "exchange every 10 sweeps; batched = one HIP-graph schedule over all instances",
-        "batched_proposals_per_s": batched, "plain_proposals_per_s": plain, "ratio": batched / plain,
+        "batched_proposals_per_s": batched, "plain_proposals_per_s": plain,
+        "ratio": batched / plain if plain else None,
         "plain_sample_instances": plain_sample, "timed_proposals": timed, "wall_s": wall,
-        "device_ms": dev_ms, "feasible_instances": feas,
+        "device_ms": dev_ms, "feasible_instances": feas, "autotuned": tune,
         "flop_per_proposal": 12.0 * c_per + 28.0 * p_per + 45.0,
         "achieved_tflops_wall": timed * (12.0 * c_per + 28.0 * p_per + 45.0) / wall / 1e12,
     }
@@ -217,6 +227,8 @@ def main():
     ap.add_argument("--batched-sweeps", type=int, default=400)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-served", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true", help="one 256 MiB device copy (profiling)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-sweep-scale", type=float, default=0.05)
@@ -227,9 +239,25 @@ def main():
               "(instances shard across GPUs with no inter-GPU traffic; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_calibration:
+        # profiling aid: one device copy of exactly 256 MiB (read 256 MiB, write
+        # 256 MiB) to scale the FETCH_SIZE / WRITE_SIZE counters
+        import torch
+
+        x = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
+        return
+
     import sisamd as sis
 
     cus, dev = sis.device_info()
+    if a.batched_only:
+        # profiling aid: the batched leg alone, no plain sample, no CPU leg
+        print(json.dumps(batched_leg(sis, a.batched_instances, a.batched_sweeps, 0, autotune=False)), flush=True)
+        return
     total_steps = a.warmup + a.steps
     p = c2_params(sis, total_steps * SWEEPS_PER_STEP)
     sess = sis.BatchSession([(N, L0, None)], p)

@@ -128,6 +128,23 @@ int sis_batch_run(sis_batch* b, int64_t intervals, int mode, double* sweep_ms, i
 int sis_batch_finalize(sis_batch* b, sis_result* out);
 int sis_batch_destroy(sis_batch* b);
 
+/* Autotuning (no reference counterpart: the reference picks chains from an
+ * occupancy query, cuda/packing.cu:788-799, and its autotune.py runs a fixed
+ * launch matrix, cuda/autotune.py:8-20).  Runs `probe_sweeps` sweeps of the
+ * given batch for every candidate of replicas per workgroup (0 = LDS-budget
+ * automatic, 16, 32, 64), sweeps per exchange (5, 10, 20) and cell width
+ * (0.7075, 0.75, 0.8) and reports the fastest by HIP-event time.  Replicas per
+ * workgroup never change results; the other two change the trajectory. */
+typedef struct sis_tuning {
+    int replicas_per_group;
+    int sweeps_per_exchange;
+    double cell;
+    double proposals_per_s;     /* of the winner during its probe */
+    int candidates_tried;
+} sis_tuning;
+int sis_autotune(const sis_instance* insts, int count, const sis_params* params, int probe_sweeps,
+                 sis_tuning* out);
+
 /* Serving: requests of any n admitted while others run (continuous batching).
  * Replaces cuda/cpp_loop.py:82-106 (ThreadPoolExecutor + VRAM admission). */
 sis_server* sis_server_create(const sis_server_cfg* cfg);

@@ -146,8 +146,11 @@ void o_ref_local32(int n, const float* q, float L, float* local_out, float* full
 /* operations in this order, with explicit fma, so results agree bit for bit. */
 /* ------------------------------------------------------------------------ */
 
+/* clamp(floor((x + shift) * (1/w))) with the correctly rounded reciprocal;
+ * w = 2 gives exactly 0.5 and the reference's floor(x / 2). */
 int o_cellcoord(float x, float shift, float w, int lo, int hi) {
-    int k = (int)floorf((x + shift) / w);
+    const float inv_w = 1.0f / w;
+    int k = (int)floorf((x + shift) * inv_w);
     return k < lo ? lo : (k > hi ? hi : k);
 }
 
@@ -759,22 +762,27 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
                 continue;
             float nc, ns;
             o_rot32(c[i], s[i], cfg->step_th * ft, &nc, &ns);
-            float dE = o_wall32_cs(nx, ny, nc, ns, L) - o_wall32_cs(x[i], y[i], c[i], s[i], L);
+            const float wdiff = o_wall32_cs(nx, ny, nc, ns, L) - o_wall32_cs(x[i], y[i], c[i], s[i], L);
+            /* stencil rows in order, CSR order inside a row; the flat index f
+             * (self included) selects one of four partial sums */
+            float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            int f = 0;
             for (int oy = -2; oy <= 2; ++oy) {
                 const int row = (g.ky[i] + oy) * g.M;
                 const int a = g.start[row + g.kx[i] - 2], b = g.start[row + g.kx[i] + 3];
-                for (int k = a; k < b; ++k) {
+                for (int k = a; k < b; ++k, ++f) {
                     const int j = g.sorted[k];
                     if (j == i) continue;
                     const float dxn = x[j] - nx, dyn = y[j] - ny;
                     const float dxo = x[j] - x[i], dyo = y[j] - y[i];
                     const float en = o_depth2_32_cs(dxn, dyn, nc, ns, c[j], s[j]);
                     const float eo = o_depth2_32_cs(dxo, dyo, c[i], s[i], c[j], s[j]);
-                    dE += en - eo;
+                    part[f & 3] += en - eo;
                     C++;
                     P += (fmaf(dxn, dxn, dyn * dyn) < 2.0f) + (fmaf(dxo, dxo, dyo * dyo) < 2.0f);
                 }
             }
+            const float dE = wdiff + ((part[0] + part[1]) + (part[2] + part[3]));
             const float u = o_u01f(o[3]);
             if (dE <= 0.0f || u < exp2f(-dE * cfg->inv_t_log2e)) {
                 x[i] = nx; y[i] = ny; c[i] = nc; s[i] = ns;

@@ -67,6 +67,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
     b += 2 * gn * 4;                            // sorted, cell coords, rank/key, items
     b = (b + 15) & ~(size_t)15;
+    b += (size_t)G * 8;                         // per-replica occupied-group mask
     b += (size_t)G * 6 * 4;                     // per-replica shift x/y, steps, 1/T, accepts
     b += 2 * (kGroups + 2) * 4;                 // group counts and starts
     return b;

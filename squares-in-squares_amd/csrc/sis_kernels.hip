@@ -43,8 +43,11 @@ __device__ __forceinline__ float u01(uint32_t r) {
 // ---------------------------------------------------------------------------
 // FP32 geometry.  (dx, dy) = centre j - centre i; (c, s) = cos/sin of angles.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int cellcoord(float x, float shift, float w, int lo, int hi) {
-    int k = (int)floorf((x + shift) / w);
+// Cell coordinate clamp(floor((x + shift) * (1/w)), lo, hi).  The reciprocal
+// is the correctly rounded 1/w; for w = 2 it is exactly 0.5, which makes this
+// the reference's floor(x / 2) bit for bit.
+__device__ __forceinline__ int cellcoord(float x, float shift, float inv_w, int lo, int hi) {
+    int k = (int)floorf((x + shift) * inv_w);
     return k < lo ? lo : (k > hi ? hi : k);
 }
 
@@ -158,6 +161,7 @@ struct TileLds {
     int* acc;
     int* gcnt;        // kGroups + 2
     int* gstart;      // kGroups + 2
+    unsigned long long* gmask;  // per replica: which colour x rank groups are occupied
 };
 
 __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
@@ -171,6 +175,8 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     t.items = t.rank + gn;
     size_t off = (size_t)(reinterpret_cast<unsigned char*>(t.items + gn) - smem);
     off = (off + 15) & ~(size_t)15;
+    t.gmask = reinterpret_cast<unsigned long long*>(smem + off);
+    off += 8 * (size_t)G;
     float* f = reinterpret_cast<float*>(smem + off);
     t.shx = f;
     t.shy = f + G;
@@ -188,7 +194,7 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
 // One prefix sum runs over the cell counts of all G replicas back to back, so
 // A holds tile-global positions into `sorted` (replica g's squares occupy
 // sorted[g*n, (g+1)*n), each replica contributing exactly n counts).
-__device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) {
+__device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv_w) {
     const int lane = threadIdx.x;
     const int gw = grid_words(M);
     uint4* A128 = reinterpret_cast<uint4*>(t.A);
@@ -197,8 +203,8 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) 
     for (int q = lane; q < G * n; q += kSweepThreads) {
         const int g = q / n;
         const float4 p = t.pos[q];
-        const int kx = cellcoord(p.x, t.shx[g], w, 0, m - 1) + 2;
-        const int ky = cellcoord(p.y, t.shy[g], w, 0, m - 1) + 2;
+        const int kx = cellcoord(p.x, t.shx[g], inv_w, 0, m - 1) + 2;
+        const int ky = cellcoord(p.y, t.shy[g], inv_w, 0, m - 1) + 2;
         t.cellk[q] = (uint16_t)(kx | (ky << 8));
         add_u16(t.A + g * gw, ky * M + kx + 2, 1u);
     }
@@ -233,10 +239,51 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float w) 
     __syncthreads();
 }
 
-// One Metropolis proposal for square i of tile replica g.
-__device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float w,
+// The 5x5-cell stencil around (kx, ky) as one flat candidate sequence: five
+// row ranges of the CSR cell list, visited row by row and, inside a row, in
+// ascending CSR position.  One flat loop per lane costs the wave the largest
+// per-lane total instead of the sum of per-row maxima.
+struct Stencil {
+    int a0, o1, o2, o3, o4, c1, c2, c3, c4, total;
+    __device__ __forceinline__ Stencil(const uint16_t* A, int M, int kx, int ky) {
+        int a[5], l[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const int row = (ky + r - 2) * M + kx;
+            a[r] = A[row - 2];
+            l[r] = A[row + 3] - a[r];
+        }
+        c1 = l[0];
+        c2 = c1 + l[1];
+        c3 = c2 + l[2];
+        c4 = c3 + l[3];
+        total = c4 + l[4];
+        a0 = a[0];
+        o1 = a[1] - c1;
+        o2 = a[2] - c2;
+        o3 = a[3] - c3;
+        o4 = a[4] - c4;
+    }
+    // CSR position of flat candidate f: a select chain (empty rows are skipped
+    // because a later row with the same cumulative start overrides them)
+    __device__ __forceinline__ int at(int f) const {
+        int off = a0;
+        off = f >= c1 ? o1 : off;
+        off = f >= c2 ? o2 : off;
+        off = f >= c3 ? o3 : off;
+        off = f >= c4 ? o4 : off;
+        return f + off;
+    }
+};
+
+// One Metropolis proposal for square i of tile replica g, computed by `lpi`
+// consecutive lanes (1, 2 or 4; this lane is `sub`).  The stencil energy
+// change is four partial sums over the flat candidate index f (f mod 4),
+// each in increasing f, combined as (s0 + s1) + (s2 + s3): the same bits
+// whatever the number of lanes sharing the work.
+__device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
                                         float L, int r0, uint32_t sweep, uint32_t k0,
-                                        uint32_t k1, unsigned long long& cC,
+                                        uint32_t k1, int lpi, int sub, unsigned long long& cC,
                                         unsigned long long& cP) {
     const int g = q / n, i = q - g * n;
     const int gw = grid_words(M);
@@ -251,37 +298,63 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     const float nx = p.x + sxy * fx;
     const float ny = p.y + sxy * fy;
     if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) return;
-    if (cellcoord(nx, t.shx[g], w, 0, m - 1) + 2 != kx ||
-        cellcoord(ny, t.shy[g], w, 0, m - 1) + 2 != ky)
+    if (cellcoord(nx, t.shx[g], inv_w, 0, m - 1) + 2 != kx ||
+        cellcoord(ny, t.shy[g], inv_w, 0, m - 1) + 2 != ky)
         return;
     float nc, ns;
     rot32(p.z, p.w, t.sth[g] * ft, nc, ns);
-    float dE = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
-    const uint16_t* Ag = t.A + g * gw;
+    const float wdiff = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
+    const Stencil sc(t.A + g * gw, M, kx, ky);
     const float4* pg = t.pos + g * n;
     unsigned cnt = 0, near = 0;
-#pragma unroll 1
-    for (int oy = -2; oy <= 2; ++oy) {
-        const int row = (ky + oy) * M + kx;
-        const int a = Ag[row - 2], b = Ag[row + 3];
-        for (int k = a; k < b; ++k) {
-            const int j = t.sorted[k];
-            if (j == i) continue;
-            const float4 v = pg[j];
-            const float dxn = v.x - nx, dyn = v.y - ny;
-            const float dxo = v.x - p.x, dyo = v.y - p.y;
-            const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
-            const float eo = depth2(dxo, dyo, p.z, p.w, v.z, v.w);
-            dE += en - eo;
-            ++cnt;
-            near += (__builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f) +
-                    (__builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+    // Energy change against flat candidate f, added to `acc`.  The mover
+    // itself contributes an exact +0 (no partial sum can hold -0, so this
+    // equals skipping it).
+    auto pair_delta = [&](int f, float& acc) {
+        const int j = t.sorted[sc.at(f)];
+        const float4 v = pg[j];
+        const float dxn = v.x - nx, dyn = v.y - ny;
+        const float dxo = v.x - p.x, dyo = v.y - p.y;
+        const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
+        const float eo = depth2(dxo, dyo, p.z, p.w, v.z, v.w);
+        const bool other = j != i;
+        acc += other ? en - eo : 0.0f;
+        cnt += other;
+        near += other ? (__builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f) +
+                            (__builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f)
+                      : 0u;
+    };
+    // partial j accumulates f = j (mod 4); one specialised loop per lane count
+    float sum;
+    if (lpi == 1) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        for (int f0 = 0; f0 < sc.total; f0 += 4) {
+            pair_delta(f0, s0);
+            if (f0 + 1 < sc.total) pair_delta(f0 + 1, s1);
+            if (f0 + 2 < sc.total) pair_delta(f0 + 2, s2);
+            if (f0 + 3 < sc.total) pair_delta(f0 + 3, s3);
         }
+        sum = (s0 + s1) + (s2 + s3);
+    } else if (lpi == 2) {
+        // sub 0 owns partials 0, 1; sub 1 owns 2, 3
+        float sa = 0.0f, sb = 0.0f;
+        for (int f0 = 2 * sub; f0 < sc.total; f0 += 4) {
+            pair_delta(f0, sa);
+            if (f0 + 1 < sc.total) pair_delta(f0 + 1, sb);
+        }
+        const float h = sa + sb;
+        sum = h + __shfl_xor(h, 1, kSweepThreads);
+    } else {
+        float sa = 0.0f;
+        for (int f = sub; f < sc.total; f += 4) pair_delta(f, sa);
+        const float h = sa + __shfl_xor(sa, 1, kSweepThreads);
+        sum = h + __shfl_xor(h, 2, kSweepThreads);
     }
     cC += cnt;
     cP += near;
+    const float dE = wdiff + sum;
     const float u = u01(o.w);
-    if (dE <= 0.0f || u < exp2f(-dE * t.ik[g])) {
+    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * t.ik[g]))) {
         t.pos[q] = make_float4(nx, ny, nc, ns);
         atomicAdd(&t.acc[g], 1);
     }
@@ -290,12 +363,12 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
 // Canonical FP32 energy of every tile replica: per-square wall + pairs with
 // larger index inside the 5x5 stencil, then a fixed pairwise tree over the
 // square index, so the bits do not depend on the tile shape.
-__device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float w, float L,
+__device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float inv_w, float L,
                             float* out /* G floats, LDS or global */) {
     const int lane = threadIdx.x;
     for (int g = lane; g < G; g += kSweepThreads) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
     __syncthreads();
-    bin_tile(t, n, G, M, m, w);
+    bin_tile(t, n, G, M, m, inv_w);
     float* e = reinterpret_cast<float*>(t.rank);   // rank + items: 4 bytes per square
     const int gw = grid_words(M);
     for (int q = lane; q < G * n; q += kSweepThreads) {
@@ -303,18 +376,14 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
         const int kx = ck & 255, ky = ck >> 8;
-        const uint16_t* Ag = t.A + g * gw;
+        const Stencil sc(t.A + g * gw, M, kx, ky);
         const float4* pg = t.pos + g * n;
         float ei = wall32(p.x, p.y, p.z, p.w, L);
-        for (int oy = -2; oy <= 2; ++oy) {
-            const int row = (ky + oy) * M + kx;
-            const int a = Ag[row - 2], b = Ag[row + 3];
-            for (int k = a; k < b; ++k) {
-                const int j = t.sorted[k];
-                if (j <= i) continue;
-                const float4 v = pg[j];
-                ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
-            }
+        for (int f = 0; f < sc.total; ++f) {
+            const int j = t.sorted[sc.at(f)];
+            if (j <= i) continue;
+            const float4 v = pg[j];
+            ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
         }
         e[q] = ei;
     }
@@ -347,6 +416,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict_
     if (nsw <= 0) return;
     const int n = D->n, G = tl.g, M = D->M, lane = threadIdx.x;
     const float L = D->Lf, w = D->w, scale = D->scale;
+    const float inv_w = 1.0f / w;
     const int m = grid_cells(L, w);
     const uint32_t k0 = D->key0, k1 = D->key1;
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
@@ -378,8 +448,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict_
             t.shy[g] = w * u01(o.y);
         }
         for (int k = lane; k < kGroups + 2; k += kSweepThreads) t.gcnt[k] = 0;
+        for (int g = lane; g < G; g += kSweepThreads) t.gmask[g] = 0ull;
         __syncthreads();
-        bin_tile(t, n, G, M, m, w);
+        bin_tile(t, n, G, M, m, inv_w);
         // group key: occupancy rank x colour; ranks >= kRankCap go serial
         for (int q = lane; q < G * n; q += kSweepThreads) {
             const int ck = t.cellk[q];
@@ -387,7 +458,18 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict_
             const int r = t.rank[q];
             const int key = r < kRankCap ? r * 9 + colour : kGroups;
             t.rank[q] = (uint16_t)key;
-            atomicAdd(&t.gcnt[key], 1);
+            atomicOr(&t.gmask[q / n], 1ull << key);
+        }
+        __syncthreads();
+        // level = position of the square's group among its replica's occupied
+        // groups: each replica walks only its own non-empty groups, in key
+        // order, and one pass serves level l of every replica of the tile
+        for (int q = lane; q < G * n; q += kSweepThreads) {
+            const int key = t.rank[q];
+            const int lvl =
+                key < kGroups ? __popcll(t.gmask[q / n] & ((1ull << key) - 1ull)) : kGroups;
+            t.rank[q] = (uint16_t)lvl;
+            atomicAdd(&t.gcnt[lvl], 1);
         }
         __syncthreads();
         if (lane == 0) {
@@ -402,27 +484,34 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict_
         for (int k = lane; k <= kGroups; k += kSweepThreads) t.gcnt[k] = t.gstart[k];
         __syncthreads();
         for (int q = lane; q < G * n; q += kSweepThreads) {
-            const int key = t.rank[q];
-            if (key < kGroups) t.items[atomicAdd(&t.gcnt[key], 1)] = (uint16_t)q;
+            const int lvl = t.rank[q];
+            if (lvl < kGroups) t.items[atomicAdd(&t.gcnt[lvl], 1)] = (uint16_t)q;
         }
         __syncthreads();
-        for (int gi = 0; gi < kGroups; ++gi) {
-            const int a = t.gstart[gi], b = t.gstart[gi + 1];
-            for (int it = a + lane; it < b; it += kSweepThreads)
-                propose(t, t.items[it], n, M, m, w, L, tl.r0, sweep, k0, k1, cC, cP);
+        for (int lv = 0; lv < kGroups; ++lv) {
+            const int a = t.gstart[lv], b = t.gstart[lv + 1];
+            if (a == b) break;  // levels are dense: the first empty one is the last
+            // sparse levels give each mover 2 or 4 lanes
+            const int lpi = b - a > 32 ? 1 : (b - a > 16 ? 2 : 4);
+            const int per = kSweepThreads / lpi;
+            const int slot = lane / lpi, sub = lane - slot * lpi;
+            for (int base = a; base < b; base += per)
+                if (base + slot < b)
+                    propose(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1, lpi,
+                            sub, cC, cP);
             __syncthreads();
         }
         if (t.gstart[kGroups + 1] > t.gstart[kGroups]) {
             if (lane == 0)
                 for (int q = 0; q < G * n; ++q)
                     if (t.rank[q] == kGroups)
-                        propose(t, q, n, M, m, w, L, tl.r0, sweep, k0, k1, cC, cP);
+                        propose(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
             __syncthreads();
         }
     }
 
     float* Eout = t.sth;  // reuse: steps are no longer needed
-    tile_energy(t, n, G, M, m, w, L, Eout);
+    tile_energy(t, n, G, M, m, inv_w, L, Eout);
     for (int q = lane; q < G * n; q += kSweepThreads) gstate[q] = t.pos[q];
     unsigned long long accsum = 0;
     for (int g = lane; g < G; g += kSweepThreads) {
@@ -839,7 +928,7 @@ __global__ void k_depth32(int npairs, const float* __restrict__ in, float* __res
 __global__ void k_cellcoord(int count, const float* __restrict__ x, float shift, float w, int lo,
                             int hi, int* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) out[k] = cellcoord(x[k], shift, w, lo, hi);
+    if (k < count) out[k] = cellcoord(x[k], shift, 1.0f / w, lo, hi);
 }
 
 // ---------------------------------------------------------------------------

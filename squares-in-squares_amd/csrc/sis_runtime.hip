@@ -575,6 +575,55 @@ int sis_solve_batch(const sis_instance* insts, int count, const sis_params* para
     return SIS_OK;
 }
 
+int sis_autotune(const sis_instance* insts, int count, const sis_params* params, int probe_sweeps,
+                 sis_tuning* out) {
+    if (!insts || !params || !out || count < 1 || probe_sweeps < 1) return fail(SIS_EINVAL, "bad autotune arguments");
+    int rc;
+    for (int k = 0; k < count; ++k)
+        if ((rc = validate(insts + k, params + k))) return rc;
+    if ((rc = ensure_device(-1))) return rc;
+    const int groups[4] = {0, 16, 32, 64};
+    const int spes[3] = {5, 10, 20};
+    const double cells[3] = {0.7075, 0.75, 0.8};
+    double best = -1;
+    int tried = 0;
+    *out = sis_tuning{};
+    for (double cell : cells)
+        for (int spe : spes)
+            for (int G : groups) {
+                std::vector<InstSpec> v;
+                long long props = 0;
+                for (int k = 0; k < count; ++k) {
+                    InstSpec s = make_spec(insts + k, params + k);
+                    s.p.replicas_per_group = G;
+                    s.p.sweeps_per_exchange = spe;
+                    s.p.cell = cell;
+                    // whole intervals only, so every candidate does the same work
+                    s.p.sweeps = (std::max(probe_sweeps, spe) / spe) * spe;
+                    if (validate(insts + k, &s.p)) goto next;
+                    props += (long long)s.n * s.p.replicas * s.p.sweeps;
+                    v.push_back(std::move(s));
+                }
+                {
+                    Batch b;
+                    if ((rc = b.build(std::move(v)))) return rc;
+                    if ((rc = b.run(0, 1))) return rc;
+                    const double rate = props / (b.device_ms * 1e-3);
+                    ++tried;
+                    if (rate > best) {
+                        best = rate;
+                        out->replicas_per_group = G;
+                        out->sweeps_per_exchange = spe;
+                        out->cell = cell;
+                        out->proposals_per_s = rate;
+                    }
+                }
+            next:;
+            }
+    out->candidates_tried = tried;
+    return tried ? SIS_OK : fail(SIS_EINVAL, "no candidate configuration was valid");
+}
+
 struct sis_batch {
     sis::Batch b;
     std::chrono::steady_clock::time_point t0;

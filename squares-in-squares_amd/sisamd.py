@@ -93,6 +93,8 @@ def lib() -> ctypes.CDLL:
     L.sis_solve.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.POINTER(SisResult)]
     L.sis_solve_batch.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams),
                                   ctypes.POINTER(SisResult)]
+    L.sis_autotune.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams), ctypes.c_int,
+                               ctypes.c_void_p]
     L.sis_batch_create.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams)]
     L.sis_batch_create.restype = ctypes.c_void_p
     L.sis_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, dp,
@@ -209,6 +211,30 @@ def solve_batch(instances, params) -> list[Result]:
     if rc:
         _err(rc)
     return [_result(res[j], outs[j]) for j in range(k)]
+
+
+class SisTuning(ctypes.Structure):
+    _fields_ = [("replicas_per_group", ctypes.c_int), ("sweeps_per_exchange", ctypes.c_int),
+                ("cell", ctypes.c_double), ("proposals_per_s", ctypes.c_double),
+                ("candidates_tried", ctypes.c_int)]
+
+
+def autotune(instances, params, probe_sweeps: int = 20) -> dict:
+    """Fastest (replicas per workgroup, sweeps per exchange, cell width) for a batch."""
+    k = len(instances)
+    if isinstance(params, SisParams):
+        params = [params] * k
+    insts = (SisInstance * k)()
+    keep = []
+    for j, (n, L, q) in enumerate(instances):
+        insts[j], arr = _instance(n, L, q)
+        keep.append(arr)
+    ps = (SisParams * k)(*params)
+    out = SisTuning()
+    rc = lib().sis_autotune(insts, k, ps, probe_sweeps, ctypes.byref(out))
+    if rc:
+        _err(rc)
+    return {name: getattr(out, name) for name, _ in SisTuning._fields_}
 
 
 class BatchSession:
