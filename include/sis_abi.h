@@ -131,10 +131,10 @@ int sis_batch_destroy(sis_batch* b);
 /* Autotuning (no reference counterpart: the reference picks chains from an
  * occupancy query, cuda/packing.cu:788-799, and its autotune.py runs a fixed
  * launch matrix, cuda/autotune.py:8-20).  Runs `probe_sweeps` sweeps of the
- * given batch for every candidate of replicas per workgroup (0 = LDS-budget
- * automatic, 16, 32, 64), sweeps per exchange (5, 10, 20) and cell width
- * (0.7075, 0.75, 0.8) and reports the fastest by HIP-event time.  Replicas per
- * workgroup never change results; the other two change the trajectory. */
+ * given batch for every candidate of replicas per workgroup (1, 2, 4, 16, 32,
+ * 64), sweeps per exchange (5, 10, 20) and cell width (0.7075, 0.8) and
+ * reports the fastest by HIP-event time.  Replicas per workgroup never change
+ * results; the other two change the trajectory. */
 typedef struct sis_tuning {
     int replicas_per_group;
     int sweeps_per_exchange;

@@ -327,12 +327,17 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     // partial j accumulates f = j (mod 4); one specialised loop per lane count
     float sum;
     if (lpi == 1) {
+        // one candidate per iteration, routed to its partial by select (keeps
+        // the register count, and with it the waves in flight, low)
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        for (int f0 = 0; f0 < sc.total; f0 += 4) {
-            pair_delta(f0, s0);
-            if (f0 + 1 < sc.total) pair_delta(f0 + 1, s1);
-            if (f0 + 2 < sc.total) pair_delta(f0 + 2, s2);
-            if (f0 + 3 < sc.total) pair_delta(f0 + 3, s3);
+        for (int f = 0; f < sc.total; ++f) {
+            float d = 0.0f;
+            pair_delta(f, d);
+            const int r = f & 3;
+            s0 = r == 0 ? s0 + d : s0;
+            s1 = r == 1 ? s1 + d : s1;
+            s2 = r == 2 ? s2 + d : s2;
+            s3 = r == 3 ? s3 + d : s3;
         }
         sum = (s0 + s1) + (s2 + s3);
     } else if (lpi == 2) {
