@@ -187,6 +187,9 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         Ls[tk] = L50
     srv.close()
     props = sum(results[tk].proposals for tk in tickets)
+    flops = sum(12.0 * results[tk].candidates + 28.0 * results[tk].near_pairs + 45.0 * results[tk].proposals
+                for tk in tickets)
+    offered = sum(n * R * sweeps for (_, n, sweeps, R, _, _) in reqs)
     lat = np.array(lat)
     # plain form on the same requests in arrival order; identical results
     pp, pw, k, same = 0, 0.0, 0, 0
@@ -205,13 +208,15 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         "workload": f"Poisson arrivals {rate}/s for {duration} s ({len(reqs)} requests), n log-uniform 10..1000, "
                     f"sweeps U(5e4,5e5) x {sweep_scale}, replicas U{{32..256}}, polish 0/200 mixed, grid seeds, "
                     "every result certified",
-        "requests": len(reqs), "certified": cert_ok, "served_proposals_per_s": served,
+        "requests": len(reqs), "certified": cert_ok,
+        "offered_proposals_per_s": offered / duration, "served_proposals_per_s": served,
         "p50_time_to_certified_L_s": float(np.percentile(lat, 50)),
         "p99_time_to_certified_L_s": float(np.percentile(lat, 99)),
         "makespan_s": wall, "plain_proposals_per_s": pp / pw if pw else None,
         "ratio_served_vs_plain": served / (pp / pw) if pw else None,
         "plain_requests": k, "plain_results_identical": same,
-        "fraction_of_fp32_peak": None,
+        "served_tflops": flops / wall / 1e12,
+        "fraction_of_fp32_peak": flops / wall / (PEAK_FP32_TFLOPS * 1e12),
     }
 
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,7 @@ class Batch {
 public:
     ~Batch();
     int build(std::vector<InstSpec>&& in);
+    int launch_tiles();
     int enqueue_interval();
     int run(long long intervals, int mode);
     int run_plain(long long intervals);
@@ -50,8 +52,10 @@ public:
     std::vector<InstSpec> specs;
     std::vector<Plan> plans;
     std::vector<InstDesc> h_inst;
-    std::vector<Tile> h_tiles;
-    size_t n_sq = 0, n_rep = 0, n_arch = 0, max_lds = 0;
+    std::vector<Tile> h_tiles;  // tiles of large instances (four wavefronts) first
+    int n_big = 0;              // number of those
+    size_t lds_big = 0, lds_small = 0;
+    size_t n_sq = 0, n_rep = 0, n_arch = 0;
     long long max_intervals = 0, intervals_done = 0;
     double device_ms = 0, last_ms = 0, sweep_ms = 0;
     long long sweep_launches = 0;
@@ -104,6 +108,22 @@ void initial_state(const InstSpec& s, std::vector<float4>& out);
 int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
 InstSpec make_spec(const sis_instance* in, const sis_params* p);
 void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);
-const void* sweep_kernel_ptr();
+const void* sweep_kernel_ptr(int threads);
+// Launch order of a tile set: tiles of instances with kBigN squares or more
+// (four-wavefront workgroups) first, then the rest, each part heaviest first
+// so the dispatcher hands the long tiles out while compute units free up.
+// Returns the number of large-instance tiles.
+template <class NOf>
+int order_tiles(std::vector<Tile>& tiles, NOf n_of) {
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const Tile& a, const Tile& b) {
+        const int na = n_of(a.inst), nb = n_of(b.inst);
+        const bool ba = na >= kBigN, bb = nb >= kBigN;
+        if (ba != bb) return ba;
+        return (long long)a.g * na > (long long)b.g * nb;
+    });
+    int big = 0;
+    while (big < (int)tiles.size() && n_of(tiles[big].inst) >= kBigN) ++big;
+    return big;
+}
 
 }  // namespace sis

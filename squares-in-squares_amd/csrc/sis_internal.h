@@ -9,6 +9,9 @@ constexpr int kMaxN = 1000;
 constexpr int kRankCap = 4;              // occupancy ranks with their own group
 constexpr int kGroups = 9 * kRankCap;    // colour x rank groups; one more = serial group
 constexpr int kSweepThreads = 64;        // sweep workgroup = one wavefront
+constexpr int kSweepThreadsBig = 256;    // four wavefronts for instances of kBigN squares or more
+constexpr int kBigN = 256;
+__host__ __device__ inline int sweep_threads(int n) { return n >= kBigN ? kSweepThreadsBig : kSweepThreads; }
 constexpr int kLadderThreads = 256;
 constexpr int kPolishThreads = 256;
 constexpr int kPolishKPT = (kMaxN + kPolishThreads - 1) / kPolishThreads;
@@ -61,15 +64,23 @@ struct SlotState {              // per global temperature slot
 // LDS of one sweep tile (G replicas of n squares, M x M cells).
 // Cell starts per replica: M*M + 2 u16, padded to 16 bytes.
 __host__ __device__ inline int grid_words(int M) { return (M * M + 2 + 7) & ~7; }
+// Small tiles draw the sweep's move variates ahead into LDS (their levels are
+// sparse, so several lanes share a move and would each draw them); large
+// tiles move one square per lane and draw inline, keeping their LDS, and with
+// it the tiles resident per compute unit, smaller.
+constexpr int kRndLdsMaxSquares = 512;
+__host__ __device__ inline bool sweep_rnd_in_lds(int gn) { return gn <= kRndLdsMaxSquares; }
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     size_t gn = (size_t)G * n;
     size_t b = 16 * gn;                         // float4 x, y, cos, sin
+    if (sweep_rnd_in_lds((int)gn)) b += 16 * gn;  // float4 move variates of the sweep
+    b += 16 * gn;                               // float4 positions in cell-list order
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
-    b += 2 * gn * 4;                            // sorted, cell coords, rank/key, items
+    b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot
     b = (b + 15) & ~(size_t)15;
     b += (size_t)G * 8;                         // per-replica occupied-group mask
-    b += (size_t)G * 6 * 4;                     // per-replica shift x/y, steps, 1/T, accepts
-    b += 2 * (kGroups + 2) * 4;                 // group counts and starts
+    b += (size_t)G * 8 * 4;                     // shift x/y (this and next sweep), steps, 1/T, accepts
+    b += 2 * (kGroups + 2) * 4;                 // level counts / cursors, level starts
     return b;
 }
 

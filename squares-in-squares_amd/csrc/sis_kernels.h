@@ -11,24 +11,36 @@ struct FinalizeJob {
     const float4* src32;        // n (x, y, cos, sin) or null
     const double* src64;        // n*3 (x, y, angle) when src32 is null
     long long out_off;          // double index into q_out
+    // When set, src32 is replica 0 of R consecutive replicas and the kernel
+    // takes the one with the lowest FP32 energy (the first on ties).
+    const RepState* reps;
+    int R, pad;
 };
 
 struct FinalizeOut {
     double e64_before, e64_after, lambda, L_cert, span;
-    int iters, pad;
+    int iters;
+    int best;                   // replica taken when the job named replicas
+    double e32;                 // its FP32 energy
 };
 
+// threads: kSweepThreads for instances below kBigN squares, kSweepThreadsBig
+// for the others (sweep_threads(n)); every tile of one launch uses the same.
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
-                        InstDesc* insts, RepState* reps, const SlotState* slots, float4* state);
+                        InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
+                        int threads);
 // list: ninst instance indices into insts, or null for insts[0..ninst).
 hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
                          const int* list = nullptr);
 hipError_t launch_finalize(hipStream_t st, int njobs, const FinalizeJob* jobs, FinalizeOut* outs,
                            double* q_out);
+// out[k] = insts[list[k]] for k < count (a kernel, so it sees every earlier
+// kernel's writes to the descriptors).
+hipError_t launch_gather_desc(hipStream_t st, int count, const InstDesc* insts, const int* list,
+                              InstDesc* out);
 hipError_t launch_depth32(hipStream_t st, int npairs, const float* in, float* depth, float* d2);
 hipError_t launch_cellcoord(hipStream_t st, int count, const float* x, float shift, float w,
                             int lo, int hi, int* out);
-hipError_t sweep_kernel_attrs(int* max_lds);
 
 }  // namespace sis

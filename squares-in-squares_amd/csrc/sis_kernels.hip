@@ -39,6 +39,15 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 __device__ __forceinline__ float u01(uint32_t r) {
     return ((float)(r >> 9) + 0.5f) * (1.0f / 8388608.0f);
 }
+// Move variates of square i of replica `rep` in sweep `sweep`: fx, fy, ftheta
+// uniform in [-1, 1) and the accept uniform (Philox counter: square, replica,
+// sweep, 0).
+__device__ __forceinline__ float4 move_variates(uint32_t i, uint32_t rep, uint32_t sweep, uint32_t k0,
+                                                uint32_t k1) {
+    const uint4 o = philox(i, rep, sweep, 0u, k0, k1);
+    return make_float4(__builtin_fmaf(2.0f, u01(o.x), -1.0f), __builtin_fmaf(2.0f, u01(o.y), -1.0f),
+                       __builtin_fmaf(2.0f, u01(o.z), -1.0f), u01(o.w));
+}
 
 // ---------------------------------------------------------------------------
 // FP32 geometry.  (dx, dy) = centre j - centre i; (c, s) = cos/sin of angles.
@@ -101,6 +110,34 @@ __device__ __forceinline__ void rot32(float c, float s, float t, float& co, floa
 // ---------------------------------------------------------------------------
 // LDS helpers.
 // ---------------------------------------------------------------------------
+// A sweep workgroup is one wavefront (kSweepThreads) or four (kSweepThreadsBig,
+// instances of kBigN squares or more).  A wavefront's LDS instructions execute
+// in issue order, so in a one-wavefront workgroup an LDS read sees every
+// earlier LDS write or atomic without a wait (which is also why the compiler
+// inserts none between them): its phase boundaries only stop the compiler
+// moving LDS accesses across them.  Four wavefronts need the workgroup barrier.
+static_assert(kSweepThreads == 64, "the small sweep workgroup must be one wavefront");
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+template <int NT>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (NT == 64)
+        wave_sync();
+    else
+        __syncthreads();
+}
+
+// Lane exchange inside aligned groups of four lanes (DPP quad permutes, no LDS
+// round trip): xor 1 swaps neighbours, xor 2 swaps pairs.
+__device__ __forceinline__ float quad_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_xor2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+
 // 16-bit counters packed two per 32-bit LDS word (counts never exceed 1000).
 __device__ __forceinline__ unsigned add_u16(uint16_t* base, int idx, unsigned v) {
     unsigned* w = reinterpret_cast<unsigned*>(base) + (idx >> 1);
@@ -111,9 +148,10 @@ __device__ __forceinline__ unsigned add_u16(uint16_t* base, int idx, unsigned v)
 // Inclusive prefix sum of the u16 array a[0..K) (K a multiple of 8, 16-byte
 // aligned) by one wavefront: each lane owns a contiguous run of 16-byte
 // chunks, sums them, the lane totals are scanned across the wave, and a
-// second pass writes the prefixes.  Totals stay below 65536.
+// second pass writes the prefixes.  Totals stay below 65536.  Called by one
+// wavefront of the workgroup.
 __device__ void wave_scan_u16(uint16_t* a, int K) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     uint4* v = reinterpret_cast<uint4*>(a);
     const int nv = K >> 3;
     const int per = (nv + kSweepThreads - 1) / kSweepThreads;
@@ -148,19 +186,25 @@ __device__ void wave_scan_u16(uint16_t* a, int K) {
 
 struct TileLds {
     float4* pos;
+    float4* rnd;      // per square: move variates fx, fy, ftheta in [-1, 1) and the accept
+                      // uniform; null when the tile draws them inline
+    float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k]
     uint16_t* A;      // per replica grid_words(M) cell starts
     uint16_t* sorted;
     uint16_t* cellk;  // kx | ky << 8
     uint16_t* rank;   // rank, then group key
     uint16_t* items;
-    float* shx;
+    uint16_t* slotq;  // cell-list position of each square
+    float* shx;       // grid shift of the current sweep
     float* shy;
+    float* shx_next;  // grid shift of the next sweep
+    float* shy_next;
     float* sxy;
     float* sth;
     float* ik;
     int* acc;
-    int* gcnt;        // kGroups + 2
-    int* gstart;      // kGroups + 2
+    int* gcnt;        // kGroups + 2: level counts, then scatter cursors
+    int* gstart;      // kGroups + 2: level starts (read by every wavefront of a large tile)
     unsigned long long* gmask;  // per replica: which colour x rank groups are occupied
 };
 
@@ -168,22 +212,32 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     TileLds t;
     const int gn = G * n;
     t.pos = reinterpret_cast<float4*>(smem);
-    t.A = reinterpret_cast<uint16_t*>(t.pos + gn);
+    float4* next = t.pos + gn;
+    t.rnd = nullptr;
+    if (sweep_rnd_in_lds(gn)) {
+        t.rnd = next;
+        next += gn;
+    }
+    t.spos = next;
+    t.A = reinterpret_cast<uint16_t*>(t.spos + gn);
     t.sorted = t.A + (size_t)G * grid_words(M);
     t.cellk = t.sorted + gn;
     t.rank = t.cellk + gn;
     t.items = t.rank + gn;
-    size_t off = (size_t)(reinterpret_cast<unsigned char*>(t.items + gn) - smem);
+    t.slotq = t.items + gn;
+    size_t off = (size_t)(reinterpret_cast<unsigned char*>(t.slotq + gn) - smem);
     off = (off + 15) & ~(size_t)15;
     t.gmask = reinterpret_cast<unsigned long long*>(smem + off);
     off += 8 * (size_t)G;
     float* f = reinterpret_cast<float*>(smem + off);
     t.shx = f;
     t.shy = f + G;
-    t.sxy = f + 2 * G;
-    t.sth = f + 3 * G;
-    t.ik = f + 4 * G;
-    t.acc = reinterpret_cast<int*>(f + 5 * G);
+    t.shx_next = f + 2 * G;
+    t.shy_next = f + 3 * G;
+    t.sxy = f + 4 * G;
+    t.sth = f + 5 * G;
+    t.ik = f + 6 * G;
+    t.acc = reinterpret_cast<int*>(f + 7 * G);
     t.gcnt = t.acc + G;
     t.gstart = t.gcnt + (kGroups + 2);
     return t;
@@ -194,13 +248,14 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
 // One prefix sum runs over the cell counts of all G replicas back to back, so
 // A holds tile-global positions into `sorted` (replica g's squares occupy
 // sorted[g*n, (g+1)*n), each replica contributing exactly n counts).
+template <int NT>
 __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv_w) {
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const int gw = grid_words(M);
     uint4* A128 = reinterpret_cast<uint4*>(t.A);
-    for (int k = lane; k < (G * gw) / 8; k += kSweepThreads) A128[k] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    for (int k = tid; k < (G * gw) / 8; k += NT) A128[k] = make_uint4(0u, 0u, 0u, 0u);
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
         const int g = q / n;
         const float4 p = t.pos[q];
         const int kx = cellcoord(p.x, t.shx[g], inv_w, 0, m - 1) + 2;
@@ -208,18 +263,18 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
         t.cellk[q] = (uint16_t)(kx | (ky << 8));
         add_u16(t.A + g * gw, ky * M + kx + 2, 1u);
     }
-    __syncthreads();
-    wave_scan_u16(t.A, G * gw);
-    __syncthreads();
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    tile_sync<NT>();
+    if (tid < 64) wave_scan_u16(t.A, G * gw);
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
         const int g = q / n, i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
         const unsigned slot = add_u16(t.A + g * gw, c + 1, 1u);
         t.sorted[slot] = (uint16_t)i;
     }
-    __syncthreads();
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
         const int g = q / n, i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
@@ -229,14 +284,17 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
         for (int k = a; k < b; ++k) r += t.sorted[k] < i;
         t.rank[q] = (uint16_t)r;
     }
-    __syncthreads();
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
         const int g = q / n, i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
-        t.sorted[t.A[g * gw + c] + t.rank[q]] = (uint16_t)i;
+        const int k = t.A[g * gw + c] + t.rank[q];
+        t.sorted[k] = (uint16_t)i;
+        t.spos[k] = t.pos[q];
+        t.slotq[q] = (uint16_t)k;
     }
-    __syncthreads();
+    tile_sync<NT>();
 }
 
 // The 5x5-cell stencil around (kx, ky) as one flat candidate sequence: five
@@ -244,7 +302,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
 // ascending CSR position.  One flat loop per lane costs the wave the largest
 // per-lane total instead of the sum of per-row maxima.
 struct Stencil {
-    int a0, o1, o2, o3, o4, c1, c2, c3, c4, total;
+    int a0, d1, d2, d3, d4, c1, c2, c3, c4, total;
     __device__ __forceinline__ Stencil(const uint16_t* A, int M, int kx, int ky) {
         int a[5], l[5];
 #pragma unroll
@@ -258,109 +316,132 @@ struct Stencil {
         c3 = c2 + l[2];
         c4 = c3 + l[3];
         total = c4 + l[4];
+        // row r's offset is o_r = a[r] - c_r (o_0 = a[0]); keep the steps
+        // o_r - o_{r-1}
         a0 = a[0];
-        o1 = a[1] - c1;
-        o2 = a[2] - c2;
-        o3 = a[3] - c3;
-        o4 = a[4] - c4;
+        d1 = (a[1] - c1) - a0;
+        d2 = (a[2] - c2) - (a[1] - c1);
+        d3 = (a[3] - c3) - (a[2] - c2);
+        d4 = (a[4] - c4) - (a[3] - c3);
     }
-    // CSR position of flat candidate f: a select chain (empty rows are skipped
-    // because a later row with the same cumulative start overrides them)
+    // CSR position of flat candidate f: f + o_K with K the last row whose
+    // cumulative start c_K <= f (the c are nondecreasing, so the steps of
+    // rows 1..K telescope to o_K and empty rows add nothing).  Written as
+    // conditional increments rather than a select among the row offsets,
+    // which the compiler would turn into an indexed table in scratch memory.
     __device__ __forceinline__ int at(int f) const {
         int off = a0;
-        off = f >= c1 ? o1 : off;
-        off = f >= c2 ? o2 : off;
-        off = f >= c3 ? o3 : off;
-        off = f >= c4 ? o4 : off;
+        off += f >= c1 ? d1 : 0;
+        off += f >= c2 ? d2 : 0;
+        off += f >= c3 ? d3 : 0;
+        off += f >= c4 ? d4 : 0;
         return f + off;
     }
 };
 
 // One Metropolis proposal for square i of tile replica g, computed by `lpi`
-// consecutive lanes (1, 2 or 4; this lane is `sub`).  The stencil energy
-// change is four partial sums over the flat candidate index f (f mod 4),
-// each in increasing f, combined as (s0 + s1) + (s2 + s3): the same bits
-// whatever the number of lanes sharing the work.
+// consecutive lanes (1, 2 or 4; this lane is `sub`).  The move variates come
+// from t.rnd (drawn for the whole sweep before the first level).  The stencil
+// energy change is four partial sums over the flat candidate index f
+// (f mod 4), each in increasing f, combined as (s0 + s1) + (s2 + s3): the
+// same bits whatever the number of lanes sharing the work.
 __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
-                                        float L, int r0, uint32_t sweep, uint32_t k0,
-                                        uint32_t k1, int lpi, int sub, unsigned long long& cC,
+                                        float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
+                                        int lpi, int sub, unsigned long long& cC,
                                         unsigned long long& cP) {
-    const int g = q / n, i = q - g * n;
+    const int g = q / n;
     const int gw = grid_words(M);
     const float4 p = t.pos[q];
+    const float4 r = t.rnd ? t.rnd[q] : move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
+    const int self = t.slotq[q];
     const int ck = t.cellk[q];
     const int kx = ck & 255, ky = ck >> 8;
-    const uint4 o = philox((uint32_t)i, (uint32_t)(r0 + g), sweep, 0u, k0, k1);
-    const float fx = __builtin_fmaf(2.0f, u01(o.x), -1.0f);
-    const float fy = __builtin_fmaf(2.0f, u01(o.y), -1.0f);
-    const float ft = __builtin_fmaf(2.0f, u01(o.z), -1.0f);
+    // the stencil depends only on the current cell: its loads go out first
+    const Stencil sc(t.A + g * gw, M, kx, ky);
     const float sxy = t.sxy[g];
-    const float nx = p.x + sxy * fx;
-    const float ny = p.y + sxy * fy;
+    const float nx = p.x + sxy * r.x;
+    const float ny = p.y + sxy * r.y;
     if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) return;
     if (cellcoord(nx, t.shx[g], inv_w, 0, m - 1) + 2 != kx ||
         cellcoord(ny, t.shy[g], inv_w, 0, m - 1) + 2 != ky)
         return;
     float nc, ns;
-    rot32(p.z, p.w, t.sth[g] * ft, nc, ns);
+    rot32(p.z, p.w, t.sth[g] * r.z, nc, ns);
     const float wdiff = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
-    const Stencil sc(t.A + g * gw, M, kx, ky);
-    const float4* pg = t.pos + g * n;
+    const int last = sc.total - 1;  // >= 0: the mover is in its own cell
     unsigned cnt = 0, near = 0;
-    // Energy change against flat candidate f, added to `acc`.  The mover
-    // itself contributes an exact +0 (no partial sum can hold -0, so this
-    // equals skipping it).
-    auto pair_delta = [&](int f, float& acc) {
-        const int j = t.sorted[sc.at(f)];
-        const float4 v = pg[j];
+    // Energy change against the square at cell-list position k.  The mover
+    // itself and positions past the end of the stencil contribute an exact
+    // +0 (no partial sum can hold -0, so this equals skipping them).
+    auto delta = [&](const float4& v, int k, bool valid) -> float {
         const float dxn = v.x - nx, dyn = v.y - ny;
         const float dxo = v.x - p.x, dyo = v.y - p.y;
         const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
         const float eo = depth2(dxo, dyo, p.z, p.w, v.z, v.w);
-        const bool other = j != i;
-        acc += other ? en - eo : 0.0f;
+        const bool other = valid && k != self;
         cnt += other;
         near += other ? (__builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f) +
                             (__builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f)
                       : 0u;
+        return other ? en - eo : 0.0f;
     };
-    // partial j accumulates f = j (mod 4); one specialised loop per lane count
-    float sum;
+    // Four candidates at once: their positions are read from the cell-list
+    // ordered copy in one go, one LDS round trip for four candidates.
+    auto quad = [&](int f0, int f1, int f2, int f3, float& d0, float& d1, float& d2, float& d3) {
+        const int k0 = sc.at(min(f0, last));
+        const int k1 = sc.at(min(f1, last));
+        const int k2 = sc.at(min(f2, last));
+        const int k3 = sc.at(min(f3, last));
+        const float4 v0 = t.spos[k0], v1 = t.spos[k1], v2 = t.spos[k2], v3 = t.spos[k3];
+        d0 = delta(v0, k0, f0 <= last);
+        d1 = delta(v1, k1, f1 <= last);
+        d2 = delta(v2, k2, f2 <= last);
+        d3 = delta(v3, k3, f3 <= last);
+    };
+    // partial k accumulates f = k (mod 4); one loop per lane count
+    float sum, d0, d1, d2, d3;
     if (lpi == 1) {
-        // one candidate per iteration, routed to its partial by select (keeps
-        // the register count, and with it the waves in flight, low)
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        for (int f = 0; f < sc.total; ++f) {
-            float d = 0.0f;
-            pair_delta(f, d);
-            const int r = f & 3;
-            s0 = r == 0 ? s0 + d : s0;
-            s1 = r == 1 ? s1 + d : s1;
-            s2 = r == 2 ? s2 + d : s2;
-            s3 = r == 3 ? s3 + d : s3;
+        for (int f = 0; f <= last; f += 4) {
+            quad(f, f + 1, f + 2, f + 3, d0, d1, d2, d3);
+            s0 += d0;
+            s1 += d1;
+            s2 += d2;
+            s3 += d3;
         }
         sum = (s0 + s1) + (s2 + s3);
     } else if (lpi == 2) {
         // sub 0 owns partials 0, 1; sub 1 owns 2, 3
         float sa = 0.0f, sb = 0.0f;
-        for (int f0 = 2 * sub; f0 < sc.total; f0 += 4) {
-            pair_delta(f0, sa);
-            if (f0 + 1 < sc.total) pair_delta(f0 + 1, sb);
+        for (int f = 2 * sub; f <= last; f += 8) {
+            quad(f, f + 1, f + 4, f + 5, d0, d1, d2, d3);
+            sa += d0;
+            sb += d1;
+            sa += d2;
+            sb += d3;
         }
         const float h = sa + sb;
-        sum = h + __shfl_xor(h, 1, kSweepThreads);
+        sum = h + quad_xor1(h);
     } else {
+        // sub k owns partial k
         float sa = 0.0f;
-        for (int f = sub; f < sc.total; f += 4) pair_delta(f, sa);
-        const float h = sa + __shfl_xor(sa, 1, kSweepThreads);
-        sum = h + __shfl_xor(h, 2, kSweepThreads);
+        for (int f = sub; f <= last; f += 16) {
+            quad(f, f + 4, f + 8, f + 12, d0, d1, d2, d3);
+            sa += d0;
+            sa += d1;
+            sa += d2;
+            sa += d3;
+        }
+        const float h = sa + quad_xor1(sa);
+        sum = h + quad_xor2(h);
     }
     cC += cnt;
     cP += near;
     const float dE = wdiff + sum;
-    const float u = u01(o.w);
-    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * t.ik[g]))) {
-        t.pos[q] = make_float4(nx, ny, nc, ns);
+    if (sub == 0 && (dE <= 0.0f || r.w < exp2f(-dE * t.ik[g]))) {
+        const float4 np = make_float4(nx, ny, nc, ns);
+        t.pos[q] = np;
+        t.spos[self] = np;
         atomicAdd(&t.acc[g], 1);
     }
 }
@@ -368,15 +449,16 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
 // Canonical FP32 energy of every tile replica: per-square wall + pairs with
 // larger index inside the 5x5 stencil, then a fixed pairwise tree over the
 // square index, so the bits do not depend on the tile shape.
+template <int NT>
 __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float inv_w, float L,
                             float* out /* G floats, LDS or global */) {
-    const int lane = threadIdx.x;
-    for (int g = lane; g < G; g += kSweepThreads) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
-    __syncthreads();
-    bin_tile(t, n, G, M, m, inv_w);
+    const int tid = threadIdx.x;
+    for (int g = tid; g < G; g += NT) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
+    tile_sync<NT>();
+    bin_tile<NT>(t, n, G, M, m, inv_w);
     float* e = reinterpret_cast<float*>(t.rank);   // rank + items: 4 bytes per square
     const int gw = grid_words(M);
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    for (int q = tid; q < G * n; q += NT) {
         const int g = q / n, i = q - g * n;
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
@@ -392,134 +474,161 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         }
         e[q] = ei;
     }
-    __syncthreads();
+    tile_sync<NT>();
     for (int s = 1; s < n; s <<= 1) {
-        for (int q = lane; q < G * n; q += kSweepThreads) {
+        for (int q = tid; q < G * n; q += NT) {
             const int g = q / n, i = q - g * n;
             if ((i & (2 * s - 1)) == 0 && i + s < n) e[q] += e[q + s];
         }
-        __syncthreads();
+        tile_sync<NT>();
     }
-    for (int g = lane; g < G; g += kSweepThreads) out[g] = e[g * n];
-    __syncthreads();
+    for (int g = tid; g < G; g += NT) out[g] = e[g * n];
+    tile_sync<NT>();
 }
 
 // ---------------------------------------------------------------------------
 // k_sweep: `spe` sweeps (or what is left of the budget) of every replica of
-// one tile.  One sweep = every square proposes exactly once.
+// one tile.  One sweep = every square proposes exactly once.  NT threads: one
+// wavefront, or four for large instances (the same arithmetic either way).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict__ tiles,
-                                                         InstDesc* __restrict__ insts,
-                                                         RepState* __restrict__ reps,
-                                                         const SlotState* __restrict__ slots,
-                                                         float4* __restrict__ state) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
+                                              InstDesc* __restrict__ insts,
+                                              RepState* __restrict__ reps,
+                                              const SlotState* __restrict__ slots,
+                                              float4* __restrict__ state) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Tile tl = tiles[blockIdx.x];
     InstDesc* D = insts + tl.inst;
     const long long left = D->sweeps_total - D->sweeps_done;
     const int nsw = (int)min((long long)D->spe, left);
     if (nsw <= 0) return;
-    const int n = D->n, G = tl.g, M = D->M, lane = threadIdx.x;
+    const int n = D->n, G = tl.g, M = D->M, tid = threadIdx.x, lane = tid & 63;
     const float L = D->Lf, w = D->w, scale = D->scale;
     const float inv_w = 1.0f / w;
     const int m = grid_cells(L, w);
     const uint32_t k0 = D->key0, k1 = D->key1;
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
-    const TileLds t = carve(smem, n, M, G);
+    TileLds t = carve(smem, n, M, G);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
-    for (int q = lane; q < G * n; q += kSweepThreads) {
+    for (int q = tid; q < G * n; q += NT) {
         float4 v = gstate[q];
         v.x = fminf(fmaxf(v.x * scale, 0.0f), L);
         v.y = fminf(fmaxf(v.y * scale, 0.0f), L);
         t.pos[q] = v;
     }
-    for (int g = lane; g < G; g += kSweepThreads) {
+    for (int g = tid; g < G; g += NT) {
         const RepState rs = reps[D->rep_off + tl.r0 + g];
         const SlotState ss = slots[D->rep_off + rs.slot];
         t.sxy[g] = ss.step_xy;
         t.sth[g] = ss.step_th;
         t.ik[g] = ss.inv_t_log2e;
         t.acc[g] = 0;
+        // grid shift of the first sweep (later ones are drawn a sweep ahead)
+        const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep0, 1u, k0, k1);
+        t.shx[g] = w * u01(o.x);
+        t.shy[g] = w * u01(o.y);
     }
-    __syncthreads();
+    tile_sync<NT>();
 
     unsigned long long cC = 0, cP = 0;
     for (int sw = 0; sw < nsw; ++sw) {
         const uint32_t sweep = sweep0 + (uint32_t)sw;
-        for (int g = lane; g < G; g += kSweepThreads) {
-            const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep, 1u, k0, k1);
-            t.shx[g] = w * u01(o.x);
-            t.shy[g] = w * u01(o.y);
-        }
-        for (int k = lane; k < kGroups + 2; k += kSweepThreads) t.gcnt[k] = 0;
-        for (int g = lane; g < G; g += kSweepThreads) t.gmask[g] = 0ull;
-        __syncthreads();
-        bin_tile(t, n, G, M, m, inv_w);
-        // group key: occupancy rank x colour; ranks >= kRankCap go serial
-        for (int q = lane; q < G * n; q += kSweepThreads) {
+        for (int k = tid; k < kGroups + 2; k += NT) t.gcnt[k] = 0;
+        for (int g = tid; g < G; g += NT) t.gmask[g] = 0ull;
+        tile_sync<NT>();
+        bin_tile<NT>(t, n, G, M, m, inv_w);
+        // group key: occupancy rank x colour; ranks >= kRankCap go serial.
+        // Small tiles draw the sweep's move variates here, one Philox block
+        // per square, while the LDS atomics of the key pass are in flight.
+        for (int q = tid; q < G * n; q += NT) {
+            const int g = q / n, i = q - g * n;
             const int ck = t.cellk[q];
             const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
             const int r = t.rank[q];
             const int key = r < kRankCap ? r * 9 + colour : kGroups;
             t.rank[q] = (uint16_t)key;
-            atomicOr(&t.gmask[q / n], 1ull << key);
+            atomicOr(&t.gmask[g], 1ull << key);
+            if (t.rnd) t.rnd[q] = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
         }
-        __syncthreads();
+        if (sw + 1 < nsw)
+            for (int g = tid; g < G; g += NT) {
+                const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep + 1u, 1u, k0, k1);
+                t.shx_next[g] = w * u01(o.x);
+                t.shy_next[g] = w * u01(o.y);
+            }
+        tile_sync<NT>();
         // level = position of the square's group among its replica's occupied
         // groups: each replica walks only its own non-empty groups, in key
         // order, and one pass serves level l of every replica of the tile
-        for (int q = lane; q < G * n; q += kSweepThreads) {
+        for (int q = tid; q < G * n; q += NT) {
             const int key = t.rank[q];
             const int lvl =
                 key < kGroups ? __popcll(t.gmask[q / n] & ((1ull << key) - 1ull)) : kGroups;
             t.rank[q] = (uint16_t)lvl;
             atomicAdd(&t.gcnt[lvl], 1);
         }
-        __syncthreads();
-        if (lane == 0) {
-            int run = 0;
-            for (int k = 0; k <= kGroups; ++k) {
-                t.gstart[k] = run;
-                run += t.gcnt[k];
+        tile_sync<NT>();
+        // level starts: one wave-wide exclusive scan of the level counts; lane
+        // l keeps the start of level l (lane kGroups + 1: the total), and the
+        // counts become the scatter cursors.  The other wavefronts of a large
+        // tile read the starts back from LDS.
+        static_assert(kGroups + 2 <= 64, "level starts must fit one wavefront");
+        int gs = 0;
+        if (tid < 64) {
+            const int c = lane <= kGroups ? t.gcnt[lane] : 0;
+            int incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int up = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += up;
             }
-            t.gstart[kGroups + 1] = run;
+            gs = incl - c;
+            if (lane <= kGroups) t.gcnt[lane] = gs;
+            if (NT > 64 && lane <= kGroups + 1) t.gstart[lane] = gs;
         }
-        __syncthreads();
-        for (int k = lane; k <= kGroups; k += kSweepThreads) t.gcnt[k] = t.gstart[k];
-        __syncthreads();
-        for (int q = lane; q < G * n; q += kSweepThreads) {
+        tile_sync<NT>();
+        if (NT > 64) gs = lane <= kGroups + 1 ? t.gstart[lane] : 0;
+        for (int q = tid; q < G * n; q += NT) {
             const int lvl = t.rank[q];
             if (lvl < kGroups) t.items[atomicAdd(&t.gcnt[lvl], 1)] = (uint16_t)q;
         }
-        __syncthreads();
+        tile_sync<NT>();
         for (int lv = 0; lv < kGroups; ++lv) {
-            const int a = t.gstart[lv], b = t.gstart[lv + 1];
+            const int a = __builtin_amdgcn_readlane(gs, lv);
+            const int b = __builtin_amdgcn_readlane(gs, lv + 1);
             if (a == b) break;  // levels are dense: the first empty one is the last
-            // sparse levels give each mover 2 or 4 lanes
-            const int lpi = b - a > 32 ? 1 : (b - a > 16 ? 2 : 4);
-            const int per = kSweepThreads / lpi;
-            const int slot = lane / lpi, sub = lane - slot * lpi;
+            // sparse levels give each mover 2 or 4 lanes (of one wavefront)
+            const int lpi = b - a > NT / 2 ? 1 : (b - a > NT / 4 ? 2 : 4);
+            const int per = NT / lpi;
+            const int slot = tid / lpi, sub = tid - slot * lpi;
             for (int base = a; base < b; base += per)
                 if (base + slot < b)
-                    propose(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1, lpi,
-                            sub, cC, cP);
-            __syncthreads();
+                    propose(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1,
+                            lpi, sub, cC, cP);
+            tile_sync<NT>();
         }
-        if (t.gstart[kGroups + 1] > t.gstart[kGroups]) {
-            if (lane == 0)
+        if (__builtin_amdgcn_readlane(gs, kGroups + 1) > __builtin_amdgcn_readlane(gs, kGroups)) {
+            if (tid == 0)
                 for (int q = 0; q < G * n; ++q)
                     if (t.rank[q] == kGroups)
                         propose(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
-            __syncthreads();
+            tile_sync<NT>();
         }
+        float* tmp = t.shx;  // the shift drawn during this sweep becomes current
+        t.shx = t.shx_next;
+        t.shx_next = tmp;
+        tmp = t.shy;
+        t.shy = t.shy_next;
+        t.shy_next = tmp;
     }
 
     float* Eout = t.sth;  // reuse: steps are no longer needed
-    tile_energy(t, n, G, M, m, inv_w, L, Eout);
-    for (int q = lane; q < G * n; q += kSweepThreads) gstate[q] = t.pos[q];
+    tile_energy<NT>(t, n, G, M, m, inv_w, L, Eout);
+    for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
     unsigned long long accsum = 0;
-    for (int g = lane; g < G; g += kSweepThreads) {
+    for (int g = tid; g < G; g += NT) {
         RepState* rs = reps + D->rep_off + tl.r0 + g;
         rs->E = Eout[g];
         rs->acc = (unsigned)t.acc[g];
@@ -527,15 +636,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(const Tile* __restrict_
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        cC += __shfl_xor(cC, off, kSweepThreads);
-        cP += __shfl_xor(cP, off, kSweepThreads);
-        accsum += __shfl_xor(accsum, off, kSweepThreads);
+        cC += __shfl_xor(cC, off, 64);
+        cP += __shfl_xor(cP, off, 64);
+        accsum += __shfl_xor(accsum, off, 64);
     }
-    if (lane == 0) {
+    if (lane == 0) {  // one set of counter atomics per wavefront
         atomicAdd(&D->cand, cC);
         atomicAdd(&D->near, cP);
         atomicAdd(&D->accepted, accsum);
-        atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
+        if (tid == 0) atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
     }
 }
 
@@ -766,15 +875,46 @@ __global__ __launch_bounds__(kPolishThreads) void k_finalize(const FinalizeJob* 
     __shared__ double x[kMaxN], y[kMaxN], a[kMaxN], cc[kMaxN], ss[kMaxN];
     __shared__ double tmp[kPolishThreads];
     __shared__ double step_sh;
+    __shared__ float best_e[kPolishThreads];
+    __shared__ int best_r[kPolishThreads];
     constexpr int NT = kPolishThreads, KPT = kPolishKPT;
     const FinalizeJob job = jobs[blockIdx.x];
     FinalizeOut* out = outs + blockIdx.x;
     const int n = job.n, tid = threadIdx.x;
     const double L = job.L;
+    const float4* src32 = job.src32;
+    if (job.reps) {
+        // lowest FP32 energy over the replicas, the lowest index on ties
+        float e = INFINITY;
+        int r = 0x7fffffff;
+        for (int k = tid; k < job.R; k += NT) {
+            const float ek = job.reps[k].E;
+            if (ek < e) { e = ek; r = k; }
+        }
+        best_e[tid] = e;
+        best_r[tid] = r;
+        __syncthreads();
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                const float eo = best_e[tid + s];
+                const int ro = best_r[tid + s];
+                if (eo < best_e[tid] || (eo == best_e[tid] && ro < best_r[tid])) {
+                    best_e[tid] = eo;
+                    best_r[tid] = ro;
+                }
+            }
+            __syncthreads();
+        }
+        src32 += (size_t)best_r[0] * n;
+        if (tid == 0) {
+            out->best = best_r[0];
+            out->e32 = best_e[0];
+        }
+    }
     for (int i = tid; i < n; i += NT) {
         double xi, yi, ai;
-        if (job.src32) {
-            const float4 v = job.src32[i];
+        if (src32) {
+            const float4 v = src32[i];
             xi = v.x; yi = v.y;
             ai = wrap_quarter(atan2((double)v.w, (double)v.z));
         } else {
@@ -940,9 +1080,16 @@ __global__ void k_cellcoord(int count, const float* __restrict__ x, float shift,
 // Launch wrappers.
 // ---------------------------------------------------------------------------
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
-                        InstDesc* insts, RepState* reps, const SlotState* slots, float4* state) {
-    hipLaunchKernelGGL(k_sweep, dim3(ntiles), dim3(kSweepThreads), lds, st, tiles, insts, reps,
-                       slots, state);
+                        InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
+                        int threads) {
+    if (threads == kSweepThreadsBig)
+        hipLaunchKernelGGL(k_sweep<kSweepThreadsBig>, dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
+                           tiles, insts, reps, slots, state);
+    else if (threads == kSweepThreads)
+        hipLaunchKernelGGL(k_sweep<kSweepThreads>, dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
+                           insts, reps, slots, state);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -951,6 +1098,22 @@ hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* r
                          const int* list) {
     hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), 0, st, insts, list, reps, slots,
                        rep_of_slot, state, arch);
+    return hipGetLastError();
+}
+
+// Copy of the descriptors named by `list` into out[0..count), 4-byte words.
+__global__ void k_gather_desc(const InstDesc* __restrict__ insts, const int* __restrict__ list,
+                              InstDesc* __restrict__ out) {
+    constexpr int kWords = (int)(sizeof(InstDesc) / 4);
+    static_assert(sizeof(InstDesc) % 4 == 0, "descriptor copied in 4-byte words");
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(insts + list[blockIdx.x]);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(out + blockIdx.x);
+    for (int k = threadIdx.x; k < kWords; k += blockDim.x) dst[k] = src[k];
+}
+
+hipError_t launch_gather_desc(hipStream_t st, int count, const InstDesc* insts, const int* list,
+                              InstDesc* out) {
+    hipLaunchKernelGGL(k_gather_desc, dim3(count), dim3(64), 0, st, insts, list, out);
     return hipGetLastError();
 }
 
@@ -973,13 +1136,9 @@ hipError_t launch_cellcoord(hipStream_t st, int count, const float* x, float shi
     return hipGetLastError();
 }
 
-const void* sweep_kernel_ptr() { return reinterpret_cast<const void*>(k_sweep); }
-
-hipError_t sweep_kernel_attrs(int* max_lds) {
-    hipFuncAttributes attr;
-    hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(k_sweep));
-    if (e == hipSuccess && max_lds) *max_lds = attr.maxDynamicSharedSizeBytes;
-    return e;
+const void* sweep_kernel_ptr(int threads) {
+    return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig>)
+                                       : reinterpret_cast<const void*>(k_sweep<kSweepThreads>);
 }
 
 }  // namespace sis

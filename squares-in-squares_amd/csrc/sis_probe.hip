@@ -56,9 +56,10 @@ int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* in
     const int M = grid_cells(L, cell) + 4;
     if (M > 255) return fail(SIS_EINVAL, "cell grid too wide");
     int G = group > 0 ? std::min(group, R) : 1;
-    while (G > 1 && sweep_lds_bytes(n, M, G) > 65536) --G;
+    const size_t hard = device_info().lds_per_block;
+    while (G > 1 && sweep_lds_bytes(n, M, G) > hard) --G;
     const size_t lds = sweep_lds_bytes(n, M, G);
-    if (lds > 65536) return fail(SIS_EINVAL, "replica does not fit LDS");
+    if (lds > hard) return fail(SIS_EINVAL, "replica does not fit LDS");
     for (size_t k = 0; k < (size_t)n * R; ++k)
         if (!(xycs[4 * k] >= 0.0f && xycs[4 * k] <= L && xycs[4 * k + 1] >= 0.0f && xycs[4 * k + 1] <= L))
             return fail(SIS_EINVAL, "centres must lie in [0, L]");
@@ -89,7 +90,8 @@ int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* in
     SIS_HIPCHECK(hipMemcpy(d_reps, reps.data(), sizeof(RepState) * R, hipMemcpyHostToDevice));
     SIS_HIPCHECK(hipMemcpy(d_slots, slots.data(), sizeof(SlotState) * R, hipMemcpyHostToDevice));
     SIS_HIPCHECK(hipMemcpy(d_state, xycs, sizeof(float4) * (size_t)n * R, hipMemcpyHostToDevice));
-    SIS_HIPCHECK(launch_sweep(nullptr, (int)tiles.size(), lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+    SIS_HIPCHECK(launch_sweep(nullptr, (int)tiles.size(), lds, d_tiles, d_inst, d_reps, d_slots, d_state,
+                              sweep_threads(n)));
     SIS_HIPCHECK(hipDeviceSynchronize());
     SIS_HIPCHECK(hipMemcpy(xycs, d_state, sizeof(float4) * (size_t)n * R, hipMemcpyDeviceToHost));
     SIS_HIPCHECK(hipMemcpy(reps.data(), d_reps, sizeof(RepState) * R, hipMemcpyDeviceToHost));

@@ -51,11 +51,15 @@ int ensure_device(int device) {
     HIPCHECK(hipSetDevice(id));
     hipDeviceProp_t prop;
     HIPCHECK(hipGetDeviceProperties(&prop, id));
-    g_dev.id = id;
+    // a sweep tile may use all the LDS a workgroup can hold (160 KB on gfx950)
+    for (int threads : {kSweepThreads, kSweepThreadsBig})
+        HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)prop.sharedMemPerBlock));
     g_dev.cus = prop.multiProcessorCount;
     g_dev.lds_per_block = (size_t)prop.sharedMemPerBlock;
     g_dev.name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
     g_dev.arch = prop.gcnArchName;
+    g_dev.id = id;  // last: a failed set-up is retried on the next call
     return SIS_OK;
 }
 
@@ -170,10 +174,11 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
     pl.M = grid_cells((float)s.L, w) + 4;
     pl.ladder = s.p.ladder > 0 ? s.p.ladder : s.p.replicas;
     pl.spe = s.p.sweeps_per_exchange;
-    const size_t hard = std::min<size_t>(64 * 1024, dev.lds_per_block ? dev.lds_per_block : 65536);
+    const size_t hard = dev.lds_per_block ? dev.lds_per_block : 65536;
     int G = s.p.replicas_per_group;
     // Automatic tiles aim at 20 KB of LDS so that eight tiles (eight waves)
-    // share a 160 KB compute unit; an explicit group size may use up to 64 KB.
+    // share a 160 KB compute unit; an explicit group size may use all the
+    // LDS of a workgroup.
     const size_t budget = G > 0 ? hard : std::min<size_t>(hard, 20 * 1024);
     if (G <= 0) {
         // The sweep is bound by LDS latency, not by lane fill: many small tiles
@@ -225,7 +230,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
     h_inst.assign(ni, InstDesc{});
     h_tiles.clear();
     size_t nsq = 0, nrep = 0, narch = 0;
-    max_lds = 0;
+    lds_big = lds_small = 0;
     max_intervals = 0;
     for (int k = 0; k < ni; ++k) {
         const InstSpec& s = specs[k];
@@ -243,17 +248,15 @@ int Batch::build(std::vector<InstSpec>&& in) {
         nsq += (size_t)s.n * s.p.replicas;
         nrep += s.p.replicas;
         narch += s.n;
-        max_lds = std::max(max_lds, pl.lds);
+        size_t& lds = s.n >= kBigN ? lds_big : lds_small;
+        lds = std::max(lds, pl.lds);
         const long long iv = (s.p.sweeps + pl.spe - 1) / pl.spe;
         max_intervals = std::max(max_intervals, iv);
         for (int r0 = 0; r0 < s.p.replicas; r0 += pl.G)
             h_tiles.push_back(Tile{k, r0, std::min(pl.G, s.p.replicas - r0), 0});
     }
     if (nsq > (size_t)INT32_MAX) return fail(SIS_EINVAL, "batch too large");
-    // heaviest tiles first: the dispatcher hands them out while CUs free up
-    std::stable_sort(h_tiles.begin(), h_tiles.end(), [&](const Tile& a, const Tile& b) {
-        return (long long)a.g * specs[a.inst].n > (long long)b.g * specs[b.inst].n;
-    });
+    n_big = order_tiles(h_tiles, [&](int k) { return specs[k].n; });
     n_sq = nsq; n_rep = nrep; n_arch = narch;
 
     // host initial state, slots, ladders
@@ -306,14 +309,24 @@ int Batch::build(std::vector<InstSpec>&& in) {
                   h_state.begin() + h_inst[k].state_off + specs[k].n, h_arch.begin() + h_inst[k].arch_off);
     HIPCHECK(hipMemcpyAsync(d_arch, h_arch.data(), sizeof(float4) * narch, hipMemcpyHostToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    if (max_lds > 65536)
-        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(sweep_kernel_ptr()),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds));
+    return SIS_OK;
+}
+
+// The sweep launches of one interval: four-wavefront tiles, then one-wavefront
+// tiles.
+int Batch::launch_tiles() {
+    const int nt = (int)h_tiles.size();
+    if (n_big > 0)
+        HIPCHECK(launch_sweep(stream, n_big, lds_big, d_tiles, d_inst, d_reps, d_slots, d_state,
+                              kSweepThreadsBig));
+    if (nt > n_big)
+        HIPCHECK(launch_sweep(stream, nt - n_big, lds_small, d_tiles + n_big, d_inst, d_reps, d_slots,
+                              d_state, kSweepThreads));
     return SIS_OK;
 }
 
 int Batch::enqueue_interval() {
-    HIPCHECK(launch_sweep(stream, (int)h_tiles.size(), max_lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+    if (int rc = launch_tiles()) return rc;
     HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
     return SIS_OK;
 }
@@ -326,7 +339,7 @@ int Batch::run_plain(long long intervals) {
     HIPCHECK(hipEventRecord(ev0, stream));
     for (long long iv = 0; iv < intervals; ++iv) {
         HIPCHECK(hipEventRecord(evs0, stream));
-        HIPCHECK(launch_sweep(stream, (int)h_tiles.size(), max_lds, d_tiles, d_inst, d_reps, d_slots, d_state));
+        if (int rc = launch_tiles()) return rc;
         HIPCHECK(hipEventRecord(evs1, stream));
         HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
         HIPCHECK(hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * specs.size(),

@@ -4,13 +4,16 @@
 // `packing` process per n, at most four in flight, admission by free VRAM and
 // RAM, cuda/cpp_loop.py:41-45, then the C++ certifier gates,
 // cuda/cpp_loop.py:17-28).  Here every admitted request keeps its replicas
-// resident in one device arena; each round runs `intervals_per_round`
-// sweep + exchange steps over the tiles of all resident requests at once, so
-// requests that arrive while others run are admitted as soon as memory frees
-// up and their work overlaps on the device.  Finished requests get the FP64
-// polish / certified side on the device and then the exact certificate on a
-// host thread pool.  A request's arithmetic depends only on the request
-// (its seed, n, parameters), never on what else is resident.
+// resident in one device arena.  Requests are split into size classes
+// ("lanes": n < 64, n < 256, the rest); each lane has its own stream and runs
+// rounds of `intervals_per_round` sweep + exchange steps over the tiles of all
+// its resident requests, so the short launches of small instances are not held
+// back by the long ones of large instances and the lanes' kernels overlap on
+// the device.  Requests that arrive while others run are admitted at their
+// lane's next round boundary as slots and memory free up.  Finished requests
+// get the FP64 polish / certified side on a stream of their own and then the
+// exact certificate on a host thread pool.  A request's arithmetic depends only
+// on the request (its seed, n, parameters), never on what else is resident.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,11 +84,13 @@ struct Req {
     uint64_t ticket = 0;
     InstSpec spec;
     Plan plan;
+    int lane = 0;
+    std::vector<float4> init;   // initial replica state, built by the submitting thread
     double t_submit = 0, t_admit = -1, t_done = -1, t_cert = -1;
     int slot = -1;
     size_t sq_off = 0, rep_off = 0, arch_off = 0;
     size_t bytes = 0;
-    bool done = false;       // result (and certificate, if asked) available
+    bool done = false;          // result (and certificate, if asked) available
     sis_result res{};
     std::vector<double> q;
     std::string L50, cert_reason;
@@ -98,14 +103,51 @@ size_t request_bytes(int n, int R) {
            (size_t)R * (sizeof(RepState) + sizeof(SlotState) + sizeof(int));
 }
 
+constexpr int kLanes = 3;
+int lane_of(int n) { return n < 64 ? 0 : (n < 256 ? 1 : 2); }
+
+// One size class of the loop (loop thread only).
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done_ev = nullptr;
+    bool in_flight = false;
+    bool dirty = true;
+    std::vector<Req*> active;         // resident, in list order
+    std::vector<Tile> tiles;          // four-wavefront tiles first (n_big of them)
+    std::vector<int> list;            // descriptor slots of `active`
+    int n_big = 0;
+    size_t lds_big = 0, lds_small = 0;
+    Tile* d_tiles = nullptr;
+    int* d_list = nullptr;
+    InstDesc* d_desc = nullptr;       // descriptors of `active` after the round
+    InstDesc* h_desc = nullptr;       // pinned mirror
+};
+
+// A finalize launch: up to kFinJobs finished requests, device buffers and
+// pinned mirrors allocated once.  Its requests keep their arena ranges and
+// slots until it completes.
+constexpr int kFinJobs = 32;
+constexpr int kFinBlocks = 8;
+struct FinBlock {
+    FinalizeJob* d_jobs = nullptr;
+    FinalizeOut* d_fo = nullptr;
+    double* d_q = nullptr;
+    FinalizeJob* h_jobs = nullptr;
+    FinalizeOut* h_fo = nullptr;
+    double* h_q = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<Req*> reqs;
+    std::vector<InstDesc> desc;       // descriptors at the end of each request
+    bool busy = false;
+};
+
 }  // namespace
 
 struct Server {
     sis_server_cfg cfg{};
     std::mutex mu;
     std::condition_variable cv, cert_cv;
-    std::deque<Req*> queue;
-    std::vector<Req*> active;
+    std::deque<Req*> queue[kLanes];   // submitted, not yet admitted (under mu)
     std::unordered_map<uint64_t, Req*> all;
     std::deque<Req*> cert_queue;
     uint64_t next_ticket = 1;
@@ -113,31 +155,35 @@ struct Server {
     bool loop_done = false;  // the device loop has exited (set under mu)
     std::thread loop;
     std::vector<std::thread> certifiers;
-    std::string error;
-    size_t resident_bytes = 0;
+    std::string error;       // under mu once the loop runs
 
-    // device arena
+    // loop thread only
+    std::string loop_err;
+    size_t resident_bytes = 0;
     size_t cap_sq = 0, cap_rep = 0;
     int cap_inst = 0;
     RangeAlloc sq_alloc, rep_alloc, arch_alloc;
     std::vector<int> free_slots;
-    hipStream_t stream = nullptr;
+    std::vector<InstDesc> h_inst;
+    Lane lanes[kLanes];
+    FinBlock fins[kFinBlocks];
+    hipStream_t fin_stream = nullptr;
     InstDesc* d_inst = nullptr;
     RepState* d_reps = nullptr;
     SlotState* d_slots = nullptr;
     int* d_ros = nullptr;
     float4* d_state = nullptr;
     float4* d_arch = nullptr;
-    Tile* d_tiles = nullptr;
-    size_t tiles_cap = 0;
-    int* d_list = nullptr;
-    std::vector<InstDesc> h_inst;
 
     int init();
     void run();
     void certify_worker();
-    int admit_one(Req* r);
-    int finish(std::vector<Req*>& done);
+    int admit_one(Req* r, Lane& ln);
+    int launch_round(Lane& ln);
+    int round_done(Lane& ln);
+    int finalize_start(std::vector<Req*>& done);
+    int finalize_done(FinBlock& fb);
+    bool work_left();
     void release_dev();
 };
 
@@ -145,7 +191,7 @@ struct Server {
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \
         if (e_ != hipSuccess) {                                                            \
-            error = std::string(#x) + ": " + hipGetErrorString(e_);                      \
+            loop_err = std::string(#x) + ": " + hipGetErrorString(e_);                   \
             return SIS_EDEVICE;                                                            \
         }                                                                                  \
     } while (0)
@@ -162,27 +208,59 @@ int Server::init() {
     arch_alloc.reset((size_t)cap_inst * kMaxN);
     for (int s = cap_inst - 1; s >= 0; --s) free_slots.push_back(s);
     h_inst.assign(cap_inst, InstDesc{});
-    SRV_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     SRV_CHECK(hipMalloc(&d_inst, sizeof(InstDesc) * cap_inst));
     SRV_CHECK(hipMalloc(&d_reps, sizeof(RepState) * cap_rep));
     SRV_CHECK(hipMalloc(&d_slots, sizeof(SlotState) * cap_rep));
     SRV_CHECK(hipMalloc(&d_ros, sizeof(int) * cap_rep));
     SRV_CHECK(hipMalloc(&d_state, sizeof(float4) * cap_sq));
     SRV_CHECK(hipMalloc(&d_arch, sizeof(float4) * (size_t)cap_inst * kMaxN));
-    SRV_CHECK(hipMalloc(&d_list, sizeof(int) * cap_inst));
+    for (Lane& ln : lanes) {
+        SRV_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+        SRV_CHECK(hipEventCreateWithFlags(&ln.done_ev, hipEventDisableTiming));
+        // a lane never has more tiles than replicas resident
+        SRV_CHECK(hipMalloc(&ln.d_tiles, sizeof(Tile) * cap_rep));
+        SRV_CHECK(hipMalloc(&ln.d_list, sizeof(int) * cap_inst));
+        SRV_CHECK(hipMalloc(&ln.d_desc, sizeof(InstDesc) * cap_inst));
+        SRV_CHECK(hipHostMalloc(&ln.h_desc, sizeof(InstDesc) * cap_inst));
+    }
+    SRV_CHECK(hipStreamCreateWithFlags(&fin_stream, hipStreamNonBlocking));
+    for (FinBlock& fb : fins) {
+        SRV_CHECK(hipMalloc(&fb.d_jobs, sizeof(FinalizeJob) * kFinJobs));
+        SRV_CHECK(hipMalloc(&fb.d_fo, sizeof(FinalizeOut) * kFinJobs));
+        SRV_CHECK(hipMalloc(&fb.d_q, sizeof(double) * 3 * kMaxN * kFinJobs));
+        SRV_CHECK(hipHostMalloc(&fb.h_jobs, sizeof(FinalizeJob) * kFinJobs));
+        SRV_CHECK(hipHostMalloc(&fb.h_fo, sizeof(FinalizeOut) * kFinJobs));
+        SRV_CHECK(hipHostMalloc(&fb.h_q, sizeof(double) * 3 * kMaxN * kFinJobs));
+        SRV_CHECK(hipEventCreateWithFlags(&fb.ev, hipEventDisableTiming));
+    }
     return SIS_OK;
 }
 
 void Server::release_dev() {
     for (void* p : {(void*)d_inst, (void*)d_reps, (void*)d_slots, (void*)d_ros, (void*)d_state,
-                    (void*)d_arch, (void*)d_tiles, (void*)d_list})
+                    (void*)d_arch})
         if (p) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
+    for (Lane& ln : lanes) {
+        if (ln.d_tiles) (void)hipFree(ln.d_tiles);
+        if (ln.d_list) (void)hipFree(ln.d_list);
+        if (ln.d_desc) (void)hipFree(ln.d_desc);
+        if (ln.h_desc) (void)hipHostFree(ln.h_desc);
+        if (ln.done_ev) (void)hipEventDestroy(ln.done_ev);
+        if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    }
+    for (FinBlock& fb : fins) {
+        for (void* p : {(void*)fb.d_jobs, (void*)fb.d_fo, (void*)fb.d_q})
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)fb.h_jobs, (void*)fb.h_fo, (void*)fb.h_q})
+            if (p) (void)hipHostFree(p);
+        if (fb.ev) (void)hipEventDestroy(fb.ev);
+    }
+    if (fin_stream) (void)hipStreamDestroy(fin_stream);
 }
 
-// Device state of a newly admitted request (called with the lock released,
-// only from the loop thread).
-int Server::admit_one(Req* r) {
+// Device state of a newly admitted request, on its lane's stream (the lane is
+// idle: admission happens at round boundaries).
+int Server::admit_one(Req* r, Lane& ln) {
     const InstSpec& s = r->spec;
     const int n = s.n, R = s.p.replicas;
     InstDesc D{};
@@ -193,8 +271,6 @@ int Server::admit_one(Req* r) {
     D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
     D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
     D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
-    std::vector<float4> st;
-    initial_state(s, st);
     std::vector<RepState> reps(R);
     std::vector<SlotState> slots(R);
     std::vector<int> ros(R);
@@ -208,210 +284,277 @@ int Server::admit_one(Req* r) {
         ros[k] = k;
     }
     h_inst[r->slot] = D;
-    SRV_CHECK(hipMemcpyAsync(d_state + r->sq_off, st.data(), sizeof(float4) * st.size(), hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipMemcpyAsync(d_arch + r->arch_off, st.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipMemcpyAsync(d_reps + r->rep_off, reps.data(), sizeof(RepState) * R, hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipMemcpyAsync(d_slots + r->rep_off, slots.data(), sizeof(SlotState) * R, hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipMemcpyAsync(d_ros + r->rep_off, ros.data(), sizeof(int) * R, hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipMemcpyAsync(d_inst + r->slot, &h_inst[r->slot], sizeof(InstDesc), hipMemcpyHostToDevice, stream));
-    SRV_CHECK(hipStreamSynchronize(stream));
+    const std::vector<float4>& st = r->init;
+    hipStream_t sm = ln.stream;
+    SRV_CHECK(hipMemcpyAsync(d_state + r->sq_off, st.data(), sizeof(float4) * st.size(), hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipMemcpyAsync(d_arch + r->arch_off, st.data(), sizeof(float4) * n, hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipMemcpyAsync(d_reps + r->rep_off, reps.data(), sizeof(RepState) * R, hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipMemcpyAsync(d_slots + r->rep_off, slots.data(), sizeof(SlotState) * R, hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipMemcpyAsync(d_ros + r->rep_off, ros.data(), sizeof(int) * R, hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipMemcpyAsync(d_inst + r->slot, &h_inst[r->slot], sizeof(InstDesc), hipMemcpyHostToDevice, sm));
+    SRV_CHECK(hipStreamSynchronize(sm));  // the host sources are released next
+    std::vector<float4>().swap(r->init);
     return SIS_OK;
 }
 
-// FP64 finalisation of finished requests; frees their arena ranges.
-int Server::finish(std::vector<Req*>& done) {
-    const int k = (int)done.size();
-    std::vector<FinalizeJob> jobs(k);
-    long long qoff = 0;
-    std::vector<double> e32(k, 0.0);
-    for (int j = 0; j < k; ++j) {
-        Req* r = done[j];
-        const InstDesc& D = h_inst[r->slot];
-        FinalizeJob& job = jobs[j];
-        job.n = D.n;
-        job.iters = r->spec.p.polish_iters;
-        job.src64 = nullptr;
-        job.out_off = qoff;
-        qoff += 3ll * D.n;
-        if (D.arch_valid) {
-            job.src32 = d_arch + D.arch_off;
-            job.L = D.L_arch;
-        } else {
-            std::vector<RepState> reps(D.R);
-            SRV_CHECK(hipMemcpyAsync(reps.data(), d_reps + D.rep_off, sizeof(RepState) * D.R, hipMemcpyDeviceToHost, stream));
-            SRV_CHECK(hipStreamSynchronize(stream));
-            int best = 0;
-            for (int q = 1; q < D.R; ++q)
-                if (reps[q].E < reps[best].E) best = q;
-            job.src32 = d_state + D.state_off + (size_t)best * D.n;
-            job.L = D.L;
-            e32[j] = reps[best].E;
+// One round of a lane: intervals of sweep + exchange over its tiles, then the
+// descriptors of its requests into pinned memory, then the lane's event.
+int Server::launch_round(Lane& ln) {
+    if (ln.dirty) {
+        ln.tiles.clear();
+        ln.list.clear();
+        ln.lds_big = ln.lds_small = 0;
+        long long total_reps = 0;
+        for (Req* r : ln.active) total_reps += r->spec.p.replicas;
+        for (Req* r : ln.active) {
+            Plan pl;
+            if (plan_instance(r->spec, (int)std::min<long long>(total_reps, 1 << 30), pl)) {
+                loop_err = "tile planning failed";
+                return SIS_EDEVICE;
+            }
+            r->plan.G = pl.G;  // grouping never changes a replica's arithmetic
+            size_t& lds = r->spec.n >= kBigN ? ln.lds_big : ln.lds_small;
+            lds = std::max(lds, pl.lds);
+            for (int r0 = 0; r0 < r->spec.p.replicas; r0 += pl.G)
+                ln.tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), 0});
+            ln.list.push_back(r->slot);
         }
+        ln.n_big = order_tiles(ln.tiles, [&](int slot) { return h_inst[slot].n; });
+        SRV_CHECK(hipMemcpyAsync(ln.d_tiles, ln.tiles.data(), sizeof(Tile) * ln.tiles.size(),
+                                 hipMemcpyHostToDevice, ln.stream));
+        SRV_CHECK(hipMemcpyAsync(ln.d_list, ln.list.data(), sizeof(int) * ln.list.size(),
+                                 hipMemcpyHostToDevice, ln.stream));
+        // the host vectors stay unchanged until the lane's next round boundary
+        ln.dirty = false;
     }
-    DevBuf<FinalizeJob> d_jobs;
-    DevBuf<FinalizeOut> d_fo;
-    DevBuf<double> d_q;
-    SRV_CHECK(d_jobs.alloc(k));
-    SRV_CHECK(d_fo.alloc(k));
-    SRV_CHECK(d_q.alloc(std::max<long long>(1, qoff)));
-    SRV_CHECK(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(FinalizeJob) * k, hipMemcpyHostToDevice, stream));
-    SRV_CHECK(launch_finalize(stream, k, d_jobs, d_fo, d_q));
-    std::vector<FinalizeOut> fo(k);
-    std::vector<double> q(std::max<long long>(1, qoff));
-    SRV_CHECK(hipMemcpyAsync(fo.data(), d_fo, sizeof(FinalizeOut) * k, hipMemcpyDeviceToHost, stream));
-    SRV_CHECK(hipMemcpyAsync(q.data(), d_q, sizeof(double) * q.size(), hipMemcpyDeviceToHost, stream));
-    SRV_CHECK(hipStreamSynchronize(stream));
-    const double t = now_s();
-    for (int j = 0; j < k; ++j) {
-        Req* r = done[j];
+    const int ipr = std::max(1, cfg.intervals_per_round);
+    const int nt = (int)ln.tiles.size();
+    for (int k = 0; k < ipr; ++k) {
+        if (ln.n_big > 0)
+            SRV_CHECK(launch_sweep(ln.stream, ln.n_big, ln.lds_big, ln.d_tiles, d_inst, d_reps, d_slots, d_state,
+                                   kSweepThreadsBig));
+        if (nt > ln.n_big)
+            SRV_CHECK(launch_sweep(ln.stream, nt - ln.n_big, ln.lds_small, ln.d_tiles + ln.n_big, d_inst, d_reps,
+                                   d_slots, d_state, kSweepThreads));
+        SRV_CHECK(launch_ladder(ln.stream, (int)ln.list.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
+                                ln.d_list));
+    }
+    SRV_CHECK(launch_gather_desc(ln.stream, (int)ln.list.size(), d_inst, ln.d_list, ln.d_desc));
+    SRV_CHECK(hipMemcpyAsync(ln.h_desc, ln.d_desc, sizeof(InstDesc) * ln.list.size(), hipMemcpyDeviceToHost,
+                             ln.stream));
+    SRV_CHECK(hipEventRecord(ln.done_ev, ln.stream));
+    ln.in_flight = true;
+    return SIS_OK;
+}
+
+// A lane's round has completed: pick up its descriptors and hand finished
+// requests to the finalize stream.
+int Server::round_done(Lane& ln) {
+    ln.in_flight = false;
+    std::vector<Req*> done, keep;
+    for (size_t k = 0; k < ln.active.size(); ++k) {
+        Req* r = ln.active[k];
+        h_inst[r->slot] = ln.h_desc[k];
         const InstDesc& D = h_inst[r->slot];
+        (D.sweeps_done >= D.sweeps_total ? done : keep).push_back(r);
+    }
+    if (done.empty()) return SIS_OK;
+    ln.active.swap(keep);
+    ln.dirty = true;
+    return finalize_start(done);
+}
+
+int Server::finalize_start(std::vector<Req*>& done) {
+    for (size_t first = 0; first < done.size(); first += kFinJobs) {
+        FinBlock* fb = nullptr;
+        for (;;) {
+            for (FinBlock& f : fins)
+                if (!f.busy) { fb = &f; break; }
+            if (fb) break;
+            // every block in flight: wait for the oldest to finish
+            for (FinBlock& f : fins)
+                if (f.busy) {
+                    SRV_CHECK(hipEventSynchronize(f.ev));
+                    if (int rc = finalize_done(f)) return rc;
+                    break;
+                }
+        }
+        const int k = (int)std::min<size_t>(kFinJobs, done.size() - first);
+        fb->reqs.assign(done.begin() + first, done.begin() + first + k);
+        fb->desc.resize(k);
+        for (int j = 0; j < k; ++j) {
+            Req* r = fb->reqs[j];
+            const InstDesc& D = h_inst[r->slot];
+            fb->desc[j] = D;
+            FinalizeJob& job = fb->h_jobs[j];
+            job = FinalizeJob{};
+            job.n = D.n;
+            job.iters = r->spec.p.polish_iters;
+            job.out_off = 3ll * kMaxN * j;
+            if (D.arch_valid) {
+                job.src32 = d_arch + D.arch_off;
+                job.L = D.L_arch;
+            } else {
+                // the best replica at the current box side, chosen on the device
+                job.src32 = d_state + D.state_off;
+                job.reps = d_reps + D.rep_off;
+                job.R = D.R;
+                job.L = D.L;
+            }
+        }
+        SRV_CHECK(hipMemcpyAsync(fb->d_jobs, fb->h_jobs, sizeof(FinalizeJob) * k, hipMemcpyHostToDevice, fin_stream));
+        SRV_CHECK(launch_finalize(fin_stream, k, fb->d_jobs, fb->d_fo, fb->d_q));
+        SRV_CHECK(hipMemcpyAsync(fb->h_fo, fb->d_fo, sizeof(FinalizeOut) * k, hipMemcpyDeviceToHost, fin_stream));
+        SRV_CHECK(hipMemcpyAsync(fb->h_q, fb->d_q, sizeof(double) * 3 * kMaxN * k, hipMemcpyDeviceToHost, fin_stream));
+        SRV_CHECK(hipEventRecord(fb->ev, fin_stream));
+        fb->busy = true;
+    }
+    return SIS_OK;
+}
+
+// A finalize launch has completed: results out, arena ranges and slots back.
+int Server::finalize_done(FinBlock& fb) {
+    const double t = now_s();
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t j = 0; j < fb.reqs.size(); ++j) {
+        Req* r = fb.reqs[j];
+        const InstDesc& D = fb.desc[j];
+        const FinalizeOut& fo = fb.h_fo[j];
         FinalRecord rec;
-        rec.q.assign(q.begin() + jobs[j].out_off, q.begin() + jobs[j].out_off + 3ll * D.n);
-        rec.target_L = jobs[j].L;
-        rec.L_cert = fo[j].L_cert;
-        rec.lambda = fo[j].lambda;
-        rec.e32 = e32[j];
-        rec.e64_before = fo[j].e64_before;
-        rec.e64_after = fo[j].e64_after;
+        rec.q.assign(fb.h_q + 3 * kMaxN * j, fb.h_q + 3 * kMaxN * j + 3ll * D.n);
+        rec.target_L = D.arch_valid ? D.L_arch : D.L;
+        rec.L_cert = fo.L_cert;
+        rec.lambda = fo.lambda;
+        rec.e32 = D.arch_valid ? 0.0 : fo.e32;
+        rec.e64_before = fo.e64_before;
+        rec.e64_after = fo.e64_after;
         rec.desc = D;
         r->q = rec.q;
         sis_result res{};
         res.q_out = nullptr;
-        Batch dummy;
-        fill_result(rec, dummy, 0, t - r->t_admit, &res);
+        Batch none;
+        fill_result(rec, none, 0, t - r->t_admit, &res);
         res.device_seconds = 0;
         r->res = res;
         r->t_done = t;
+        const int n = r->spec.n, R = r->spec.p.replicas;
+        sq_alloc.release(r->sq_off, (size_t)n * R);
+        rep_alloc.release(r->rep_off, R);
+        arch_alloc.release(r->arch_off, n);
+        free_slots.push_back(r->slot);
+        resident_bytes -= r->bytes;
+        if (cfg.certify)
+            cert_queue.push_back(r);
+        else
+            r->done = true;
     }
+    fb.reqs.clear();
+    fb.busy = false;
+    cert_cv.notify_all();
     return SIS_OK;
 }
 
+bool Server::work_left() {
+    for (const Lane& ln : lanes)
+        if (ln.in_flight || !ln.active.empty()) return true;
+    for (const FinBlock& fb : fins)
+        if (fb.busy) return true;
+    return false;
+}
+
 void Server::run() {
-    if (hipSetDevice(device_info().id) != hipSuccess) {
+    auto die = [&](const std::string& why) {
         std::lock_guard<std::mutex> lk(mu);
-        error = "hipSetDevice failed in the server loop";
+        error = why.empty() ? std::string("device loop failed") : why;
         stop = true;
+    };
+    if (hipSetDevice(device_info().id) != hipSuccess) {
+        die("hipSetDevice failed in the server loop");
         return;
     }
-    std::vector<Tile> tiles;
-    std::vector<int> list;
-    bool dirty = true;
-    size_t max_lds = 0;
     for (;;) {
-        std::vector<Req*> admitted;
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            if (active.empty() && queue.empty()) {
-                if (stop) break;
-                cv.wait_for(lk, std::chrono::milliseconds(2));
-                continue;
-            }
-            // admission: FIFO while a slot, the HBM budget and arena ranges allow
-            while (!queue.empty() && !free_slots.empty()) {
-                Req* r = queue.front();
-                const double budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : 1e300;
-                if (resident_bytes > 0 && resident_bytes + r->bytes > budget) break;
-                size_t so, ro, ao;
-                const int n = r->spec.n, R = r->spec.p.replicas;
-                if (!sq_alloc.alloc((size_t)n * R, so)) break;
-                if (!rep_alloc.alloc(R, ro)) { sq_alloc.release(so, (size_t)n * R); break; }
-                if (!arch_alloc.alloc(n, ao)) { sq_alloc.release(so, (size_t)n * R); rep_alloc.release(ro, R); break; }
-                queue.pop_front();
-                r->sq_off = so; r->rep_off = ro; r->arch_off = ao;
-                r->slot = free_slots.back();
-                free_slots.pop_back();
-                r->t_admit = now_s();
-                resident_bytes += r->bytes;
-                admitted.push_back(r);
-            }
-        }
-        for (Req* r : admitted) {
-            if (admit_one(r) != SIS_OK) {
-                std::lock_guard<std::mutex> lk(mu);
-                stop = true;
+        bool progressed = false;
+        // finished finalize launches
+        for (FinBlock& fb : fins) {
+            if (!fb.busy) continue;
+            const hipError_t q = hipEventQuery(fb.ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                die(std::string("finalize failed: ") + hipGetErrorString(q));
                 return;
             }
-            active.push_back(r);
-            dirty = true;
-        }
-        if (active.empty()) continue;
-        if (dirty) {
-            tiles.clear();
-            list.clear();
-            max_lds = 0;
-            long long total_reps = 0;
-            for (Req* r : active) total_reps += r->spec.p.replicas;
-            for (Req* r : active) {
-                Plan pl;
-                plan_instance(r->spec, (int)std::min<long long>(total_reps, 1 << 30), pl);
-                r->plan.G = pl.G;  // grouping never changes a replica's arithmetic
-                h_inst[r->slot].G = pl.G;
-                max_lds = std::max(max_lds, pl.lds);
-                for (int r0 = 0; r0 < r->spec.p.replicas; r0 += pl.G)
-                    tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), 0});
-                list.push_back(r->slot);
+            if (finalize_done(fb) != SIS_OK) {
+                die(loop_err);
+                return;
             }
-            std::stable_sort(tiles.begin(), tiles.end(), [&](const Tile& a, const Tile& b) {
-                return (long long)a.g * h_inst[a.inst].n > (long long)b.g * h_inst[b.inst].n;
-            });
-            if (tiles.size() > tiles_cap) {
-                if (d_tiles) (void)hipFree(d_tiles);
-                tiles_cap = tiles.size() * 2;
-                if (hipMalloc(&d_tiles, sizeof(Tile) * tiles_cap) != hipSuccess) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    error = "tile allocation failed";
-                    stop = true;
+            progressed = true;
+        }
+        for (int li = 0; li < kLanes; ++li) {
+            Lane& ln = lanes[li];
+            if (ln.in_flight) {
+                const hipError_t q = hipEventQuery(ln.done_ev);
+                if (q == hipErrorNotReady) continue;
+                if (q != hipSuccess) {
+                    die(std::string("device round failed: ") + hipGetErrorString(q));
                     return;
                 }
+                if (round_done(ln) != SIS_OK) {
+                    die(loop_err);
+                    return;
+                }
+                progressed = true;
             }
-            (void)hipMemcpyAsync(d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice, stream);
-            (void)hipMemcpyAsync(d_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, stream);
-            dirty = false;
-        }
-        const int ipr = std::max(1, cfg.intervals_per_round);
-        bool ok = true;
-        for (int k = 0; k < ipr && ok; ++k) {
-            ok = launch_sweep(stream, (int)tiles.size(), max_lds, d_tiles, d_inst, d_reps, d_slots, d_state) == hipSuccess;
-            ok = ok && launch_ladder(stream, (int)list.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
-                                     d_list) == hipSuccess;
-        }
-        ok = ok && hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * cap_inst, hipMemcpyDeviceToHost,
-                                  stream) == hipSuccess;
-        ok = ok && hipStreamSynchronize(stream) == hipSuccess;
-        if (!ok) {
-            std::lock_guard<std::mutex> lk(mu);
-            error = std::string("device round failed: ") + hipGetErrorString(hipGetLastError());
-            stop = true;
-            return;
-        }
-        std::vector<Req*> done, keep;
-        for (Req* r : active) {
-            const InstDesc& D = h_inst[r->slot];
-            (D.sweeps_done >= D.sweeps_total ? done : keep).push_back(r);
-        }
-        if (!done.empty()) {
-            if (finish(done) != SIS_OK) {
+            // admission into this lane: FIFO while a slot, the HBM budget and
+            // arena ranges allow
+            std::vector<Req*> admitted;
+            {
                 std::lock_guard<std::mutex> lk(mu);
-                stop = true;
-                return;
-            }
-            std::lock_guard<std::mutex> lk(mu);
-            for (Req* r : done) {
-                const int n = r->spec.n, R = r->spec.p.replicas;
-                sq_alloc.release(r->sq_off, (size_t)n * R);
-                rep_alloc.release(r->rep_off, R);
-                arch_alloc.release(r->arch_off, n);
-                free_slots.push_back(r->slot);
-                resident_bytes -= r->bytes;
-                if (cfg.certify) {
-                    cert_queue.push_back(r);
-                } else {
-                    r->done = true;
+                std::deque<Req*>& qu = queue[li];
+                while (!qu.empty() && !free_slots.empty()) {
+                    Req* r = qu.front();
+                    const double budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : 1e300;
+                    if (resident_bytes > 0 && resident_bytes + r->bytes > budget) break;
+                    size_t so, ro, ao;
+                    const int n = r->spec.n, R = r->spec.p.replicas;
+                    if (!sq_alloc.alloc((size_t)n * R, so)) break;
+                    if (!rep_alloc.alloc(R, ro)) { sq_alloc.release(so, (size_t)n * R); break; }
+                    if (!arch_alloc.alloc(n, ao)) { sq_alloc.release(so, (size_t)n * R); rep_alloc.release(ro, R); break; }
+                    qu.pop_front();
+                    r->sq_off = so; r->rep_off = ro; r->arch_off = ao;
+                    r->slot = free_slots.back();
+                    free_slots.pop_back();
+                    r->t_admit = now_s();
+                    resident_bytes += r->bytes;
+                    admitted.push_back(r);
                 }
             }
-            cert_cv.notify_all();
-            active.swap(keep);
-            dirty = true;
+            for (Req* r : admitted) {
+                if (admit_one(r, ln) != SIS_OK) {
+                    die(loop_err);
+                    return;
+                }
+                ln.active.push_back(r);
+                ln.dirty = true;
+            }
+            if (!ln.active.empty()) {
+                if (launch_round(ln) != SIS_OK) {
+                    die(loop_err);
+                    return;
+                }
+                progressed = true;
+            }
+        }
+        if (progressed) continue;
+        std::unique_lock<std::mutex> lk(mu);
+        bool queued = false;
+        for (const auto& qu : queue) queued = queued || !qu.empty();
+        if (!work_left()) {
+            if (queued) continue;  // admission is waiting for memory that nothing will free
+            if (stop) break;
+            cv.wait_for(lk, std::chrono::milliseconds(2));
+        } else {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
         }
     }
 }
@@ -458,7 +601,7 @@ sis_server* sis_server_create(const sis_server_cfg* cfg) {
     auto* h = new sis_server;
     h->s.cfg = *cfg;
     if (h->s.init() != SIS_OK) {
-        fail(SIS_EDEVICE, h->s.error.empty() ? "server device set-up failed" : h->s.error);
+        fail(SIS_EDEVICE, h->s.loop_err.empty() ? "server device set-up failed" : h->s.loop_err);
         h->s.release_dev();
         delete h;
         return nullptr;
@@ -479,7 +622,13 @@ int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params
         delete r;
         return rc;
     }
+    if ((size_t)inst->n * params->replicas > h->s.cap_sq || (size_t)params->replicas > h->s.cap_rep) {
+        delete r;
+        return fail(SIS_EINVAL, "request larger than the server arena");
+    }
+    r->lane = lane_of(inst->n);
     r->bytes = request_bytes(inst->n, params->replicas);
+    initial_state(r->spec, r->init);  // in the caller's thread, off the device loop
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> lk(h->s.mu);
@@ -487,13 +636,9 @@ int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params
             delete r;
             return fail(SIS_EDEVICE, h->s.error.empty() ? "server stopped" : h->s.error);
         }
-        if ((size_t)inst->n * params->replicas > h->s.cap_sq || (size_t)params->replicas > h->s.cap_rep) {
-            delete r;
-            return fail(SIS_EINVAL, "request larger than the server arena");
-        }
         r->ticket = h->s.next_ticket++;
         h->s.all[r->ticket] = r;
-        h->s.queue.push_back(r);
+        h->s.queue[r->lane].push_back(r);
         *ticket = r->ticket;
     }
     h->s.cv.notify_all();

@@ -102,10 +102,13 @@ def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
     return st, acc, en, cnt
 
 
-@pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (120, 12.5, 3, 3, 2)])
+@pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (120, 12.5, 3, 3, 2),
+                                                (300, 17.5, 2, 3, 1), (300, 17.5, 4, 2, 2)])
 def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     """k_sweep == the CPU model of the same colour-checkerboard sweep: same
-    states, accept counts and canonical energies, bit for bit."""
+    states, accept counts and canonical energies, bit for bit.  n = 300 runs
+    the four-wavefront tiles, with the move variates drawn ahead into LDS
+    (300 squares per tile) and inline (600)."""
     rng = np.random.default_rng(n)
     st = _seeded_state(rng, R, n, L)
     T = np.geomspace(1e-3, 1e-1, R)
@@ -225,9 +228,10 @@ def test_solve_n11_certifies(sis, oracle, ref_certifier):
 def test_batch_equals_alone(sis):
     """Alone vs in the mix: each instance's result is bit-identical whether it
     runs alone (plain) or inside a batch with others of different n."""
-    insts = [(13, 5.0, None), (37, 8.0, None), (60, 9.5, None)]
+    # n = 270 runs on four-wavefront tiles, launched beside the others
+    insts = [(13, 5.0, None), (37, 8.0, None), (60, 9.5, None), (270, 17.0, None)]
     ps = [sis.default_params(replicas=r, sweeps=s, rng_seed=seed, init_mode=1, tmin=1e-4, tmax=3e-2)
-          for r, s, seed in ((16, 300, 1), (32, 200, 2), (8, 250, 3))]
+          for r, s, seed in ((16, 300, 1), (32, 200, 2), (8, 250, 3), (6, 40, 4))]
     say("batch")
     mix = sis.solve_batch(insts, ps)
     for (n, L, q), p, m in zip(insts, ps, mix):

@@ -31,10 +31,13 @@ def _wait(srv, tickets, timeout=120):
 
 
 def test_server_results_equal_alone(sis):
+    # every size class of the loop (n < 64, n < 256, larger); the last request
+    # stays infeasible, so its export is the best replica picked on the device
     reqs = [(15, 5.5, dict(replicas=32, sweeps=300, rng_seed=11)),
             (40, 8.0, dict(replicas=16, sweeps=500, rng_seed=12)),
             (90, 11.0, dict(replicas=8, sweeps=200, rng_seed=13)),
-            (25, 6.0, dict(replicas=24, sweeps=150, rng_seed=14))]
+            (25, 6.0, dict(replicas=24, sweeps=150, rng_seed=14)),
+            (300, 17.4, dict(replicas=8, sweeps=60, rng_seed=15))]
     say("server mix")
     srv = sis.Server(max_instances=8, intervals_per_round=3, certifier_threads=2, certify=True)
     tickets = []
@@ -45,12 +48,14 @@ def test_server_results_equal_alone(sis):
     res = _wait(srv, tickets)
     times = [srv.times(t) for t in tickets]
     srv.close()
+    assert not res[tickets[-1]].feasible
     for (n, L, kw), t, (tq, td, tc, L50) in zip(reqs, tickets, times):
         say("alone", n)
         alone = sis.solve(n, L, None, init_mode=1, tmin=1e-4, tmax=3e-2, **kw)
         m = res[t]
         assert np.array_equal(alone.q, m.q)
-        for k in ("proposals", "accepted", "shrinks", "exchanges_accepted", "L_cert", "target_L", "e64_after"):
+        for k in ("proposals", "accepted", "shrinks", "exchanges_accepted", "L_cert", "target_L", "e64_after",
+                  "e32", "e64_before"):
             assert alone.fields[k] == m.fields[k], k
         assert 0 <= tq <= td <= tc
         if m.feasible:
