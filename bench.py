@@ -63,6 +63,83 @@ def cpu_baseline(target_s: float = 12.0):
                       f"PT-SA in {r['seconds']:.2f} s on {threads} threads"}
 
 
+def traffic_per_launch():
+    """HBM bytes per sweep launch of the plain n=50 workload, from the
+    FETCH_SIZE and WRITE_SIZE counter runs summarised in
+    profiles/k_sweep_traffic.json (each scaled by the same counter's reading
+    for a copy of known size), or None when no summary is committed."""
+    p = ROOT / "profiles" / "k_sweep_traffic.json"
+    if not p.exists():
+        return None
+    return json.loads(p.read_text()).get("traffic_bytes_per_launch")
+
+
+def cpu_c1(sis):
+    """The CPU configuration run exactly: n=11, L0=5, centres U[0.5, 4.5]^2,
+    angles U[0, pi/2), 8 replicas on a geometric ladder 1e-4..1e-1, 2e5 sweeps,
+    exchange every 10 sweeps, L -= 1e-4 on a zero-overlap configuration, seed
+    42; one thread per replica and one thread alone; then the exact certificate
+    of the best configuration (dilation ladder)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+
+    o = oracle_lib.load()
+    runs = {}
+    for threads in (8, 1):
+        runs[threads] = o.ptsa(n=11, L0=5.0, replicas=8, tmin=1e-4, tmax=1e-1, sweeps=200000, swap_every=10,
+                               shrink=1e-4, seed=42, threads=threads)
+    r = runs[8]
+    t0 = time.perf_counter()
+    try:
+        dil, L50 = sis.export_certify(r["best_q"])
+    except Exception as e:  # an uncertifiable export is reported, not hidden
+        dil, L50 = None, f"not certified: {e}"
+    cert_s = time.perf_counter() - t0
+    return {
+        "workload": "n=11, L0=5, 8 replicas, T 1e-4..1e-1, 2e5 sweeps (1.76e7 proposals), exchange every 10, "
+                    "shrink 1e-4, seed 42",
+        "threads_8": {"seconds": runs[8]["seconds"], "proposals_per_s": runs[8]["proposals"] / runs[8]["seconds"]},
+        "threads_1": {"seconds": runs[1]["seconds"], "proposals_per_s": runs[1]["proposals"] / runs[1]["seconds"]},
+        "L_feasible": r["L_feasible"], "shrinks": r["shrinks"], "certified_L": L50, "dilation": dil,
+        "certify_seconds": cert_s,
+    }
+
+
+def precision_leg(sis, sweeps: int = 10000):
+    """n = 500 and 1000 started from the grid of side ceil(sqrt(n)) with
+    jitter U(+-0.02) on the centres and U(+-0.01 rad) on the angles, 128
+    replicas, FP32 sweeps, 200 FP64 polish iterations, FP64 L-shrink test and
+    certified side.  Reports the FP32-vs-FP64 energy gap of the export, the
+    FP32 zero-overlap configurations the FP64 test rejected, the certified
+    slack and the exact certificate."""
+    import numpy as np
+
+    out = []
+    for n in (500, 1000):
+        side = math.isqrt(n - 1) + 1
+        rng = np.random.default_rng(n)
+        i = np.arange(n)
+        q = np.zeros((n, 3))
+        q[:, 0] = i % side + 0.5 + rng.uniform(-0.02, 0.02, n)
+        q[:, 1] = i // side + 0.5 + rng.uniform(-0.02, 0.02, n)
+        q[:, 2] = rng.uniform(-0.01, 0.01, n)
+        t0 = time.perf_counter()
+        r = sis.solve(n, float(side), q, replicas=128, sweeps=sweeps, polish_iters=200, init_mode=0,
+                      init_jitter=0.0, tmin=1e-6, tmax=1e-2, shrink=1e-3, rng_seed=n)
+        wall = time.perf_counter() - t0
+        try:
+            dil, L50 = sis.export_certify(r.q)
+        except Exception as e:
+            dil, L50 = None, f"not certified: {e}"
+        out.append({"n": n, "L0": side, "sweeps": sweeps, "replicas": 128, "target_L": r.target_L,
+                    "L_cert": r.L_cert, "certified_slack": r.L_cert - r.target_L, "lambda": r.fields["lambda"],
+                    "e32": r.e32, "e64_before": r.e64_before, "e32_vs_e64_gap": r.e_gap,
+                    "fp64_rejects": r.fp64_rejects, "feasible": r.feasible, "shrinks": r.shrinks,
+                    "certified_L": L50, "dilation": dil, "proposals_per_s": r.proposals / wall,
+                    "seconds": wall})
+    return out
+
+
 def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool = True):
     """Mixed-n stream (512 instances, n ~ U{20..200} with seed 3, 64 replicas
     each): all instances in one HIP-graph schedule vs the plain form (one
@@ -236,8 +313,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="one 256 MiB device copy (profiling)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-sweep-scale", type=float, default=0.05)
+    ap.add_argument("--serve-sweep-scale", type=float, default=0.1)
     ap.add_argument("--plain-seconds", type=float, default=8.0)
+    ap.add_argument("--no-precision", action="store_true", help="skip the n=500/1000 precision leg")
+    ap.add_argument("--precision-sweeps", type=int, default=10000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -308,10 +387,12 @@ def main():
             "device": dev, "compute_units": cus,
         },
         "roofline": {
-            "bound": "valu", "kernel": "k_sweep",
+            "bound": "mfma", "kernel": "k_sweep<64>",
+            "roof": "FP32 compute peak; the sweep runs on the FP32 vector ALUs (no dense contraction on this "
+                    "path, no matrix-core work); HBM traffic is far below its roof",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
-            "traffic": None,
+            "traffic": traffic_per_launch(),
             "flop_per_proposal": flop_per_prop, "C_per_proposal": c_per, "P_per_proposal": p_per,
             "sweep_kernel_ms_avg": sweep_ms / max(1, launches), "sweep_launches": launches,
         },
@@ -326,8 +407,11 @@ def main():
         line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
     if not a.no_served:
         line["served"] = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale, a.plain_seconds)
+    if not a.no_precision:
+        line["precision"] = precision_leg(sis, a.precision_sweeps)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        line["cpu_baseline"]["c1_exact"] = cpu_c1(sis)
     print(json.dumps(line), flush=True)
 
 

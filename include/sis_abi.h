@@ -80,6 +80,8 @@ typedef struct sis_result {
     int status;                 /* SIS_OK or an error code */
     double seconds;             /* host wall time of the solve */
     double device_seconds;      /* time of the sweep + exchange kernels (HIP events) */
+    int fp64_rejects;           /* zero-overlap configurations in FP32 that the FP64 test rejected */
+    int reserved;
 } sis_result;
 
 /* Server configuration for the continuous-batching request loop. */
@@ -131,10 +133,11 @@ int sis_batch_destroy(sis_batch* b);
 /* Autotuning (no reference counterpart: the reference picks chains from an
  * occupancy query, cuda/packing.cu:788-799, and its autotune.py runs a fixed
  * launch matrix, cuda/autotune.py:8-20).  Runs `probe_sweeps` sweeps of the
- * given batch for every candidate of replicas per workgroup (1, 2, 4, 16, 32,
- * 64), sweeps per exchange (5, 10, 20) and cell width (0.7075, 0.8) and
- * reports the fastest by HIP-event time.  Replicas per workgroup never change
- * results; the other two change the trajectory. */
+ * given batch for every candidate of replicas per workgroup (0 = the automatic
+ * size of each instance by its n and the load, 1, 2, 4, 8), sweeps per
+ * exchange (5, 10, 20) and cell width (0.7075, 0.8) and reports the fastest by
+ * HIP-event time.  Replicas per workgroup never change results; the other two
+ * change the trajectory. */
 typedef struct sis_tuning {
     int replicas_per_group;
     int sweeps_per_exchange;

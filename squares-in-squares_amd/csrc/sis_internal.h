@@ -40,6 +40,8 @@ struct InstDesc {
     int arch_valid;
     int shrinks;
     int feasible;
+    int fp64_rejects;           // FP32-zero configurations the FP64 test found overlapping
+    int pad;
     unsigned long long proposals, accepted, cand, near, ex_tried, ex_acc;
 };
 

@@ -839,6 +839,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         }
         __syncthreads();
         const double e64 = total64<kLadderThreads>(x, y, cc, ss, n, L0, tmp);
+        if (e64 != 0.0 && tid == 0) D->fp64_rejects += 1;
         if (e64 == 0.0) {
             archive = !had_arch || L0 < L_arch;
             if (archive)

@@ -176,17 +176,28 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
     pl.spe = s.p.sweeps_per_exchange;
     const size_t hard = dev.lds_per_block ? dev.lds_per_block : 65536;
     int G = s.p.replicas_per_group;
-    // Automatic tiles aim at 20 KB of LDS so that eight tiles (eight waves)
-    // share a 160 KB compute unit; an explicit group size may use all the
-    // LDS of a workgroup.
-    const size_t budget = G > 0 ? hard : std::min<size_t>(hard, 20 * 1024);
+    const bool big = s.n >= kBigN;
+    // Automatic tiles aim at 20 KB of LDS per wavefront-sized tile (eight
+    // tiles share a 160 KB compute unit) and 40 KB per four-wavefront tile;
+    // an explicit group size may use all the LDS of a workgroup.
+    const size_t budget = G > 0 ? hard : std::min<size_t>(hard, big ? 40 * 1024 : 20 * 1024);
     if (G <= 0) {
-        // The sweep is bound by LDS latency, not by lane fill: many small tiles
-        // (one or two replicas per wavefront) keep more waves in flight than
-        // fewer, fuller ones (measured on a 512-instance mixed-n batch:
-        // 2 replicas/tile 1.38e10 proposals/s, 1: 1.32e10, 4: 0.97e10, 8: 0.57e10).
+        // With few replicas every replica gets a workgroup of its own, spread
+        // over the compute units.  Under load (several tiles per compute unit)
+        // the tile that keeps the most lanes busy per pass wins: measured at
+        // 2048 replicas of one instance (1e9 proposals/s), one-wavefront tiles
+        // of about 64 squares (n=10: 1 replica 1.04, 2: 1.20, 4: 1.30, 8: 1.21;
+        // n=30: 2.71, 3.07, 2.65; n=100: 7.27, 6.26, 4.14; n=200: 10.8, 8.0)
+        // and four-wavefront tiles of 500-800 squares (n=256: 9.4, 13.3;
+        // n=400: 12.5, 14.1; n=600: 13.4, 11.9; n=1000: 15.6, 9.0).
         const int cus = std::max(1, dev.cus);
-        G = (long long)batch_replicas >= 32ll * cus ? 2 : 1;
+        G = 1;
+        if ((long long)batch_replicas >= 8ll * cus) {
+            if (big)
+                G = s.n < 500 ? 2 : 1;
+            else
+                while (2 * G * s.n <= 64) G *= 2;
+        }
     }
     G = std::min(G, s.p.replicas);
     while (G > 1 && (sweep_lds_bytes(s.n, pl.M, G) > budget || G * s.n > 65535)) --G;
@@ -505,6 +516,7 @@ void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis
     out->status = SIS_OK;
     out->seconds = wall_s;
     out->device_seconds = b.device_ms * 1e-3;
+    out->fp64_rejects = D.fp64_rejects;
     (void)k;
 }
 
@@ -593,7 +605,8 @@ int sis_autotune(const sis_instance* insts, int count, const sis_params* params,
     for (int k = 0; k < count; ++k)
         if ((rc = validate(insts + k, params + k))) return rc;
     if ((rc = ensure_device(-1))) return rc;
-    const int groups[6] = {1, 2, 4, 16, 32, 64};
+    // 0: the automatic tile size of every instance (by its n and the load)
+    const int groups[5] = {0, 1, 2, 4, 8};
     const int spes[3] = {5, 10, 20};
     const double cells[2] = {0.7075, 0.8};
     double best = -1;

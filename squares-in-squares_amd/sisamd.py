@@ -61,6 +61,7 @@ class SisResult(ctypes.Structure):
         ("candidates", ctypes.c_uint64), ("near_pairs", ctypes.c_uint64),
         ("shrinks", ctypes.c_int), ("feasible", ctypes.c_int), ("replicas", ctypes.c_int),
         ("status", ctypes.c_int), ("seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
+        ("fp64_rejects", ctypes.c_int), ("reserved", ctypes.c_int),
     ]
 
 

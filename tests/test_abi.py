@@ -29,6 +29,30 @@ def test_library_exports_every_declared_symbol(sis):
     assert not missing, missing
 
 
+def test_ctypes_layouts_match_header(sis, tmp_path):
+    """The Python mirror's structs have the header's size and field offsets
+    (compiled with the host C compiler)."""
+    structs = {"sis_instance": sis.SisInstance, "sis_params": sis.SisParams, "sis_result": sis.SisResult,
+               "sis_server_cfg": sis.SisServerCfg, "sis_tuning": sis.SisTuning}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            cf = "lambda" if fname == "lambda_" else fname
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for ln in filter(None, out):
+        cname, field, val = ln.split()
+        py = structs[cname]
+        got = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert got == int(val), (cname, field, got, val)
+
+
 def test_default_params(sis):
     p = sis.default_params()
     assert p.tmin == 1e-5 and p.tmax == 5e-2
