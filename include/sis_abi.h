@@ -53,7 +53,8 @@ typedef struct sis_params {
     int sweeps_per_exchange;    /* sweeps between replica-exchange steps */
     int replicas_per_group;     /* replicas per workgroup tile; 0 = autotuned */
     int polish_iters;           /* FP64 conjugate-gradient polish iterations at the end */
-    int init_mode;              /* 0: seed configuration in every replica; 1: uniform random */
+    int init_mode;              /* 0: seed configuration (+ init_jitter) in every replica;
+                                   1: uniform random; 2: portfolio (sis_portfolio) */
     int flags;                  /* reserved, 0 */
     uint64_t rng_seed;          /* Philox key */
 } sis_params;
@@ -105,6 +106,18 @@ int sis_device_info(int* compute_units, char* name, int name_len);
 
 /* Defaults of cuda/packing.cu:777-786 mapped to this engine. */
 void sis_default_params(sis_params* p);
+
+/* Starting configurations of init_mode 2 (cuda/packing.cu:304-452, the portfolio
+ * built on the host before the first epoch, cuda/packing.cu:806, and the
+ * structured jitter of cuda/packing.cu:454-475): replica r at temperature
+ * position r % ladder of ladder r / ladder gets, by that position scaled to
+ * eight classes, the seed or a Goebel-strip topology (classes 0-1, the seed
+ * on the first two ladders), a Goebel-strip topology (2-4), a wall-hugging grid
+ * with tilted rows (5-6) or corner squares with a tilted grid (7); then,
+ * when `jitter` is 1, class 0 stays exact, class 1 moves by 1e-4 x 100^u and
+ * the rest by 0.03.  out: replicas*n*3 (x, y, angle in radians).  Host only:
+ * needs no device. */
+int sis_portfolio(const sis_instance* inst, const sis_params* params, int jitter, double* out);
 
 /* Plain form: one instance per launch sequence with a host synchronisation
  * every exchange interval, as the reference's epoch loop

@@ -102,7 +102,7 @@ int main(int argc, char** argv) {
     p.sweeps_per_exchange = std::max(1, std::min(steps, 10));
     p.polish_iters = 200;
     p.rng_seed = rng_seed;
-    p.init_mode = 0;
+    p.init_mode = 2;  // portfolio starts + structured jitter, as cuda/packing.cu:806,821
     sis_instance inst{n, L, seed.data()};
     std::vector<double> q(3 * (size_t)n);
     sis_result r{};

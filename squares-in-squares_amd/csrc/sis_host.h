@@ -105,6 +105,10 @@ const DeviceInfo& device_info();
 void default_params(sis_params* p);
 int validate(const sis_instance* in, const sis_params* p);
 void initial_state(const InstSpec& s, std::vector<float4>& out);
+void portfolio_state(const InstSpec& s, bool jitter, std::vector<double>& q);
+// sis_portfolio.cpp
+void portfolio_layouts(int n, int R, int ladder, double seed_L, const double* seed, std::vector<double>& out);
+double portfolio_jitter(int cls, double u);
 int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
 InstSpec make_spec(const sis_instance* in, const sis_params* p);
 void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);

@@ -188,7 +188,8 @@ struct TileLds {
     float4* pos;
     float4* rnd;      // per square: move variates fx, fy, ftheta in [-1, 1) and the accept
                       // uniform; null when the tile draws them inline
-    float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k]
+    float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k];
+                      // null when the tile reads positions through `sorted`
     uint16_t* A;      // per replica grid_words(M) cell starts
     uint16_t* sorted;
     uint16_t* cellk;  // kx | ky << 8
@@ -218,8 +219,12 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
         t.rnd = next;
         next += gn;
     }
-    t.spos = next;
-    t.A = reinterpret_cast<uint16_t*>(t.spos + gn);
+    t.spos = nullptr;
+    if (sweep_spos_in_lds(gn)) {
+        t.spos = next;
+        next += gn;
+    }
+    t.A = reinterpret_cast<uint16_t*>(next);
     t.sorted = t.A + (size_t)G * grid_words(M);
     t.cellk = t.sorted + gn;
     t.rank = t.cellk + gn;
@@ -291,7 +296,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
         const int c = (ck >> 8) * M + (ck & 255);
         const int k = t.A[g * gw + c] + t.rank[q];
         t.sorted[k] = (uint16_t)i;
-        t.spos[k] = t.pos[q];
+        if (t.spos) t.spos[k] = t.pos[q];
         t.slotq[q] = (uint16_t)k;
     }
     tile_sync<NT>();
@@ -386,13 +391,21 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         return other ? en - eo : 0.0f;
     };
     // Four candidates at once: their positions are read from the cell-list
-    // ordered copy in one go, one LDS round trip for four candidates.
+    // ordered copy in one go (or through their square indices, four and
+    // four), so four candidates cost one or two LDS round trips.
+    const float4* pg = t.pos + (size_t)g * n;
     auto quad = [&](int f0, int f1, int f2, int f3, float& d0, float& d1, float& d2, float& d3) {
         const int k0 = sc.at(min(f0, last));
         const int k1 = sc.at(min(f1, last));
         const int k2 = sc.at(min(f2, last));
         const int k3 = sc.at(min(f3, last));
-        const float4 v0 = t.spos[k0], v1 = t.spos[k1], v2 = t.spos[k2], v3 = t.spos[k3];
+        float4 v0, v1, v2, v3;
+        if (t.spos) {
+            v0 = t.spos[k0]; v1 = t.spos[k1]; v2 = t.spos[k2]; v3 = t.spos[k3];
+        } else {
+            const int j0 = t.sorted[k0], j1 = t.sorted[k1], j2 = t.sorted[k2], j3 = t.sorted[k3];
+            v0 = pg[j0]; v1 = pg[j1]; v2 = pg[j2]; v3 = pg[j3];
+        }
         d0 = delta(v0, k0, f0 <= last);
         d1 = delta(v1, k1, f1 <= last);
         d2 = delta(v2, k2, f2 <= last);
@@ -441,7 +454,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     if (sub == 0 && (dE <= 0.0f || r.w < exp2f(-dE * t.ik[g]))) {
         const float4 np = make_float4(nx, ny, nc, ns);
         t.pos[q] = np;
-        t.spos[self] = np;
+        if (t.spos) t.spos[self] = np;
         atomicAdd(&t.acc[g], 1);
     }
 }

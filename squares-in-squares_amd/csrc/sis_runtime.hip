@@ -112,6 +112,7 @@ int validate(const sis_instance* in, const sis_params* p) {
         return fail(SIS_EINVAL, "replicas_per_group out of range");
     if (p->polish_iters < 0 || p->polish_iters > 100000) return fail(SIS_EINVAL, "polish_iters out of range");
     if (!(p->init_jitter >= 0 && p->init_jitter < 1)) return fail(SIS_EINVAL, "init_jitter in [0,1)");
+    if (p->init_mode < 0 || p->init_mode > 2) return fail(SIS_EINVAL, "init_mode must be 0, 1 or 2");
     const float w = (float)p->cell;
     const int M = grid_cells((float)in->L, w) + 4;
     if (M > 255) return fail(SIS_EINVAL, "cell grid wider than 255 cells: raise the cell width");
@@ -137,11 +138,62 @@ static inline double u01d(uint32_t hi, uint32_t lo) {
     return ((double)k + 0.5) * (1.0 / 4503599627370496.0);
 }
 
+// Portfolio starts (init_mode 2): the structured families of every replica by
+// its ladder position, then, when `jitter`, the per-class jitter of
+// cuda/packing.cu:454-475 drawn from Philox (amplitude: counter (0, r, 2, 3);
+// square i: (i, r, 0, 3) for x and y, (i, r, 1, 3) for the angle), centres
+// clamped to [1/2, L - 1/2].  Without a seed configuration the seed is a
+// uniform random one (replica 0's draws of init_mode 1).
+void portfolio_state(const InstSpec& s, bool jitter, std::vector<double>& q) {
+    const int n = s.n, R = s.p.replicas, lad = s.p.ladder > 0 ? s.p.ladder : R;
+    const uint32_t k0 = (uint32_t)s.p.rng_seed, k1 = (uint32_t)(s.p.rng_seed >> 32);
+    const double L = s.L;
+    std::vector<double> seed(s.q);
+    if (seed.empty()) {
+        seed.resize(3 * (size_t)n);
+        const double lo = std::min(0.5, 0.5 * L), span = std::max(0.0, L - 2 * lo);
+        for (int i = 0; i < n; ++i) {
+            uint32_t o[4], o2[4];
+            philox_host((uint32_t)i, 0u, 0u, 3u, k0, k1, o);
+            philox_host((uint32_t)i, 0u, 1u, 3u, k0, k1, o2);
+            seed[3 * i] = lo + span * u01d(o[0], o[1]);
+            seed[3 * i + 1] = lo + span * u01d(o[2], o[3]);
+            seed[3 * i + 2] = 1.5707963267948966 * u01d(o2[0], o2[1]);
+        }
+    }
+    portfolio_layouts(n, R, lad, L, seed.data(), q);
+    if (!jitter) return;
+    const double lo = std::min(0.5, 0.5 * L), hi = std::max(L - 0.5, 0.5 * L);
+    for (int r = 0; r < R; ++r) {
+        uint32_t o[4], o2[4];
+        philox_host(0u, (uint32_t)r, 2u, 3u, k0, k1, o);
+        const double j = portfolio_jitter((r % lad) * 8 / lad, u01d(o[0], o[1]));
+        if (j == 0.0) continue;
+        for (int i = 0; i < n; ++i) {
+            philox_host((uint32_t)i, (uint32_t)r, 0u, 3u, k0, k1, o);
+            philox_host((uint32_t)i, (uint32_t)r, 1u, 3u, k0, k1, o2);
+            double* p = &q[((size_t)r * n + i) * 3];
+            p[0] = std::min(hi, std::max(lo, p[0] + j * (2 * u01d(o[0], o[1]) - 1)));
+            p[1] = std::min(hi, std::max(lo, p[1] + j * (2 * u01d(o[2], o[3]) - 1)));
+            p[2] += j * (2 * u01d(o2[0], o2[1]) - 1);
+        }
+    }
+}
+
 void initial_state(const InstSpec& s, std::vector<float4>& out) {
     const int n = s.n, R = s.p.replicas;
     const uint32_t k0 = (uint32_t)s.p.rng_seed, k1 = (uint32_t)(s.p.rng_seed >> 32);
     out.resize((size_t)n * R);
     const double L = s.L;
+    if (s.p.init_mode == 2) {
+        std::vector<double> q;
+        portfolio_state(s, true, q);
+        for (size_t k = 0; k < (size_t)n * R; ++k) {
+            const double x = std::min(std::max(q[3 * k], 0.0), L), y = std::min(std::max(q[3 * k + 1], 0.0), L);
+            out[k] = make_float4((float)x, (float)y, (float)std::cos(q[3 * k + 2]), (float)std::sin(q[3 * k + 2]));
+        }
+        return;
+    }
     for (int r = 0; r < R; ++r)
         for (int i = 0; i < n; ++i) {
             double x, y, a;
@@ -556,6 +608,16 @@ int sis_device_info(int* compute_units, char* name, int name_len) {
 
 void sis_default_params(sis_params* p) {
     if (p) default_params(p);
+}
+
+int sis_portfolio(const sis_instance* inst, const sis_params* params, int jitter, double* out) {
+    if (!out) return fail(SIS_EINVAL, "null output");
+    int rc = validate(inst, params);
+    if (rc) return rc;
+    std::vector<double> q;
+    portfolio_state(make_spec(inst, params), jitter != 0, q);
+    std::copy(q.begin(), q.end(), out);
+    return SIS_OK;
 }
 
 int sis_solve(const sis_instance* inst, const sis_params* params, sis_result* out) {

@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
     L.sis_device_info.argtypes = [ip, ctypes.c_char_p, ctypes.c_int]
     L.sis_default_params.argtypes = [ctypes.POINTER(SisParams)]
     L.sis_default_params.restype = None
+    L.sis_portfolio.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.c_int, dp]
     L.sis_solve.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.POINTER(SisResult)]
     L.sis_solve_batch.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams),
                                   ctypes.POINTER(SisResult)]
@@ -177,6 +178,17 @@ def _instance(n: int, L: float, q) -> tuple[SisInstance, np.ndarray | None]:
         return SisInstance(n, L, None), None
     arr = np.ascontiguousarray(np.asarray(q, dtype=np.float64).reshape(n, 3))
     return SisInstance(n, L, _ptr(arr, ctypes.c_double)), arr
+
+
+def portfolio(n: int, L: float, q=None, params: SisParams | None = None, jitter: bool = True, **kw) -> np.ndarray:
+    """Starting configurations of init_mode 2 (host only): (replicas, n, 3)."""
+    p = params if params is not None else default_params(**kw)
+    inst, _keep = _instance(n, L, q)
+    out = np.zeros((p.replicas, n, 3), dtype=np.float64)
+    rc = lib().sis_portfolio(ctypes.byref(inst), ctypes.byref(p), int(jitter), _ptr(out, ctypes.c_double))
+    if rc:
+        _err(rc)
+    return out
 
 
 def solve(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Result:

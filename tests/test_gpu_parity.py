@@ -225,6 +225,27 @@ def test_solve_n11_certifies(sis, oracle, ref_certifier):
     assert float(L50) <= r.L_cert * dil + 1e-9
 
 
+def test_portfolio_start_reaches_goebel_side(sis, oracle, ref_certifier):
+    """init_mode 2 (portfolio starts, as the reference's `packing` seeds its
+    chains): n = 10 from the 4 x 4 grid seed reaches Goebel's side 3 + 1/sqrt2
+    (one of the starting topologies) within the box-shrink budget, and the
+    export certifies."""
+    from refcert import certify_q
+    n = 10
+    i = np.arange(n)
+    q = np.stack([i % 4 + 0.5, i // 4 + 0.5, np.zeros(n)], axis=1)
+    say("portfolio n=10")
+    r = sis.solve(n, 4.0, q, replicas=64, ladder=8, sweeps=3000, shrink=1e-2, rng_seed=9, init_mode=2,
+                  tmin=1e-6, tmax=1e-2)
+    assert r.feasible == 1
+    assert r.L_cert <= 3 + 1 / math.sqrt(2) + 0.02
+    for a in range(n):
+        for b in range(a + 1, n):
+            assert oracle.lib.o_sat_oracle64(ptr(r.q[a].copy(), f64), ptr(r.q[b].copy(), f64)) == 0.0
+    dil, L50 = certify_q(ref_certifier, r.q)
+    assert dil is not None
+
+
 def test_batch_equals_alone(sis):
     """Alone vs in the mix: each instance's result is bit-identical whether it
     runs alone (plain) or inside a batch with others of different n."""
