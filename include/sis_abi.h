@@ -57,6 +57,11 @@ typedef struct sis_params {
                                    1: uniform random; 2: portfolio (sis_portfolio) */
     int flags;                  /* reserved, 0 */
     uint64_t rng_seed;          /* Philox key */
+    int collective_moves;       /* collective proposals per replica after each interval's sweeps
+                                   (row / column / cluster / orientation / diagonal / tilted-split
+                                   moves of cuda/packing.cu:564-616), 0..64 */
+    int kick_every;             /* intervals between kicks (worst-square relocation + jitter,
+                                   cuda/packing.cu:696-745); 0 = none */
 } sis_params;
 
 /* Result of one instance.  Replaces raw{n}.json (cuda/packing.cu:859-865). */
@@ -82,7 +87,8 @@ typedef struct sis_result {
     double seconds;             /* host wall time of the solve */
     double device_seconds;      /* time of the sweep + exchange kernels (HIP events) */
     int fp64_rejects;           /* zero-overlap configurations in FP32 that the FP64 test rejected */
-    int reserved;
+    int kicks;                  /* replica kicks applied */
+    uint64_t collective_tried, collective_accepted;
 } sis_result;
 
 /* Server configuration for the continuous-batching request loop. */
@@ -204,6 +210,15 @@ int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* in
                    const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
                    uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,
                    uint64_t counts[3]);
+/* One sweep launch as above followed by `collective` collective proposals per
+ * replica (cuda/packing.cu:564-616) and, when (interval + 1) is a multiple of
+ * kick_every > 0, the kick (cuda/packing.cu:696-745; relocating on odd kick
+ * numbers).  counts: proposals, candidates, near pairs, collective proposals,
+ * collective accepts, kicked replicas. */
+int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
+                  const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
+                  uint32_t sweep0, int sweeps, int collective, int kick_every, uint32_t interval,
+                  float* xycs, int* accepted, float* energy, uint64_t counts[6]);
 /* FP64 polish + certified side of one configuration (cuda/packing.cu:624-697):
  * q in/out n*3; writes the exported (translated, dilated) configuration. */
 int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,

@@ -821,6 +821,158 @@ float o_energy32_model(int n, float L, float w, const float* x, const float* y, 
     return n > 0 ? e[0] : 0.0f;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Collective moves and the kick of the product (CPU model): the FP32       */
+/* restatement of what the device applies after a launch's sweeps, for      */
+/* cuda/packing.cu:564-616 (modes 1-6) and :696-745 (worst-square void      */
+/* relocation + jitter).  Same draws, same operation order.                 */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+struct Coll {
+    float cx, cy, cc, cs, mv, rt, tc, ts, th;
+    int mode;
+};
+
+bool coll_take(const Coll& w, float x, float y, float c, float s) {
+    const float dx = x - w.cx, dy = y - w.cy;
+    switch (w.mode) {
+        case 1: return fabsf(dy) < 0.55f;
+        case 2: return fabsf(dx) < 0.55f;
+        case 3: return fmaf(dx, dx, dy * dy) < 2.5f;
+        case 4: {
+            const float cd = fmaf(c, w.cc, s * w.cs), sd = fmaf(s, w.cc, -(c * w.cs));
+            return fmaxf(fabsf(cd), fabsf(sd)) > 0.99820054f;
+        }
+        case 5: return fabsf(dx + dy) < 0.85f || fabsf(dx - dy) < 0.85f;
+        default: return fmaxf(fabsf(c), fabsf(s)) < 0.99920011f;
+    }
+}
+
+}  // namespace
+
+int o_collective32_model(const o_sweep_cfg* cfg, int moves, float* x, float* y, float* c, float* s,
+                         float* energy) {
+    const int n = cfg->n;
+    const float L = cfg->L;
+    const uint32_t key[2] = {cfg->key0, cfg->key1};
+    float e_old = o_energy32_model(n, L, cfg->w, x, y, c, s);
+    std::vector<float> bx(n), by(n), bc(n), bs(n);
+    int accepted = 0;
+    for (int k = 0; k < moves; ++k) {
+        uint32_t o1[4], o2[4];
+        {
+            uint32_t ctr[4] = {(uint32_t)k, cfg->replica, cfg->sweep, 5u};
+            o_philox(ctr, key, o1);
+            uint32_t ctr2[4] = {(uint32_t)k, cfg->replica, cfg->sweep, 6u};
+            o_philox(ctr2, key, o2);
+        }
+        Coll w;
+        const int anchor = std::min(n - 1, (int)(o_u01f(o1[0]) * (float)n));
+        w.mode = 1 + std::min(5, (int)(o_u01f(o1[1]) * 6.0f));
+        w.cx = x[anchor]; w.cy = y[anchor]; w.cc = c[anchor]; w.cs = s[anchor];
+        w.mv = cfg->step_xy * fmaf(2.0f, o_u01f(o1[2]), -1.0f);
+        w.rt = cfg->step_xy * fmaf(2.0f, o_u01f(o1[3]), -1.0f);
+        w.th = cfg->step_th * fmaf(2.0f, o_u01f(o2[0]), -1.0f);
+        o_rot32(1.0f, 0.0f, w.th, &w.tc, &w.ts);
+        const float uacc = o_u01f(o2[1]);
+        for (int i = 0; i < n; ++i) { bx[i] = x[i]; by[i] = y[i]; bc[i] = c[i]; bs[i] = s[i]; }
+        for (int i = 0; i < n; ++i) {
+            if (!coll_take(w, bx[i], by[i], bc[i], bs[i])) continue;
+            const float dx = bx[i] - w.cx, dy = by[i] - w.cy;
+            float nx, ny;
+            if (w.mode == 3 || w.mode == 4) {
+                nx = (w.cx + fmaf(w.tc, dx, -(w.ts * dy))) + w.mv;
+                ny = (w.cy + fmaf(w.ts, dx, w.tc * dy)) + w.rt;
+            } else if (w.mode == 6) {
+                const float sg = (i & 1) ? 1.0f : -1.0f;
+                nx = bx[i] + sg * w.mv;
+                ny = by[i] + sg * w.rt;
+            } else {
+                nx = bx[i] + w.mv;
+                ny = by[i] + w.rt;
+            }
+            o_rot32(bc[i], bs[i], w.th, &c[i], &s[i]);
+            x[i] = fminf(L - 0.45f, fmaxf(0.45f, nx));
+            y[i] = fminf(L - 0.45f, fmaxf(0.45f, ny));
+        }
+        const float e_new = o_energy32_model(n, L, cfg->w, x, y, c, s);
+        const float dE = e_new - e_old;
+        if (dE <= 0.0f || uacc < exp2f(-dE * cfg->inv_t_log2e)) {
+            e_old = e_new;
+            accepted++;
+        } else {
+            for (int i = 0; i < n; ++i) { x[i] = bx[i]; y[i] = by[i]; c[i] = bc[i]; s[i] = bs[i]; }
+        }
+    }
+    if (energy) *energy = e_old;
+    return accepted;
+}
+
+int o_kick32_model(const o_sweep_cfg* cfg, uint32_t interval, int relocate, float* x, float* y, float* c,
+                   float* s, float* energy) {
+    const int n = cfg->n;
+    const float L = cfg->L;
+    const uint32_t key[2] = {cfg->key0, cfg->key1};
+    const float e0 = o_energy32_model(n, L, cfg->w, x, y, c, s);
+    if (energy) *energy = e0;
+    if (!(e0 > 1e-12f)) return 0;
+    Grid g;
+    build_grid(g, n, L, cfg->w, 0.0f, 0.0f, x, y);
+    std::vector<float> sq(n);
+    int worst = 0;
+    for (int i = 0; i < n; ++i) {
+        float ei = o_wall32_cs(x[i], y[i], c[i], s[i], L);
+        for (int oy = -2; oy <= 2; ++oy) {
+            const int row = (g.ky[i] + oy) * g.M;
+            const int a = g.start[row + g.kx[i] - 2], b = g.start[row + g.kx[i] + 3];
+            for (int k = a; k < b; ++k) {
+                const int j = g.sorted[k];
+                if (j == i) continue;
+                ei += o_depth2_32_cs(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]);
+            }
+        }
+        sq[i] = ei;
+        if (ei > sq[worst]) worst = i;  /* first maximum */
+    }
+    if (relocate && sq[worst] > 1e-5f) {
+        int best = 0;
+        float best_d2 = -1.0f;
+        for (int p = 0; p < 64; ++p) {
+            const float vx = fmaf(L - 1.1f, ((p & 7) + 0.5f) * 0.125f, 0.55f);
+            const float vy = fmaf(L - 1.1f, ((p >> 3) + 0.5f) * 0.125f, 0.55f);
+            float d2 = 1e9f;
+            for (int j = 0; j < n; ++j) {
+                if (j == worst) continue;
+                const float dx = vx - x[j], dy = vy - y[j];
+                d2 = fminf(d2, fmaf(dx, dx, dy * dy));
+            }
+            if (d2 > best_d2) { best_d2 = d2; best = p; }  /* first maximum */
+        }
+        const float vx = fmaf(L - 1.1f, ((best & 7) + 0.5f) * 0.125f, 0.55f);
+        const float vy = fmaf(L - 1.1f, ((best >> 3) + 0.5f) * 0.125f, 0.55f);
+        uint32_t o[4];
+        uint32_t ctr[4] = {0u, cfg->replica, interval, 7u};
+        o_philox(ctr, key, o);
+        x[worst] = fminf(L - 0.5f, fmaxf(0.5f, vx + 0.05f * fmaf(2.0f, o_u01f(o[0]), -1.0f)));
+        y[worst] = fminf(L - 0.5f, fmaxf(0.5f, vy + 0.05f * fmaf(2.0f, o_u01f(o[1]), -1.0f)));
+        o_rot32(0.70710678f, 0.70710678f, o_u01f(o[2]) * 1.57079633f - 0.78539816f, &c[worst], &s[worst]);
+    }
+    for (int i = 0; i < n; ++i) {
+        const bool hot = sq[i] > 1e-6f;
+        const float axy = hot ? 0.08f : 0.008f, aa = hot ? 0.18f : 0.015f;
+        uint32_t o[4];
+        uint32_t ctr[4] = {(uint32_t)i, cfg->replica, interval, 8u};
+        o_philox(ctr, key, o);
+        x[i] = fminf(L - 0.48f, fmaxf(0.48f, x[i] + axy * fmaf(2.0f, o_u01f(o[0]), -1.0f)));
+        y[i] = fminf(L - 0.48f, fmaxf(0.48f, y[i] + axy * fmaf(2.0f, o_u01f(o[1]), -1.0f)));
+        const float c0 = c[i], s0 = s[i];
+        o_rot32(c0, s0, aa * fmaf(2.0f, o_u01f(o[2]), -1.0f), &c[i], &s[i]);
+    }
+    if (energy) *energy = o_energy32_model(n, L, cfg->w, x, y, c, s);
+    return 1;
+}
+
 /* src/squares/core.py:27-35: half-integer grid with integer pitch. */
 int o_grid_packing(int n, double* q) {
     if (n < 1) return 2;

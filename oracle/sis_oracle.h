@@ -97,6 +97,16 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
 /* Canonical FP32 replica energy, same summation order as the device. */
 float o_energy32_model(int n, float L, float w, const float* x, const float* y,
                        const float* c, const float* s);
+/* `moves` collective proposals after a launch whose last sweep counter is
+ * cfg->sweep - 1 (draws use cfg->sweep = the sweep count after the launch);
+ * step_xy / step_th are the slot's half-widths.  Returns accepts; *energy =
+ * canonical energy at the end. */
+int o_collective32_model(const o_sweep_cfg* cfg, int moves, float* x, float* y, float* c, float* s,
+                         float* energy);
+/* Kick after interval `interval` (relocating when `relocate`).  Returns 1 when
+ * the replica had energy (and was kicked); *energy = canonical energy after. */
+int o_kick32_model(const o_sweep_cfg* cfg, uint32_t interval, int relocate, float* x, float* y, float* c,
+                   float* s, float* energy);
 
 /* ---- analytic fixtures (cuda/scheduler.py:25-74, src/squares/core.py:27-35) ---- */
 int o_grid_packing(int n, double* q);

@@ -100,6 +100,10 @@ int main(int argc, char** argv) {
     p.ladder = 8;
     p.sweeps = (int64_t)eff_epochs * steps;
     p.sweeps_per_exchange = std::max(1, std::min(steps, 10));
+    // the reference's epoch mixes collective moves into its steps
+    // (cuda/packing.cu:564-616) and kicks once per epoch (:696-745)
+    p.collective_moves = 4;
+    p.kick_every = std::max(1, steps / p.sweeps_per_exchange);
     p.polish_iters = 200;
     p.rng_seed = rng_seed;
     p.init_mode = 2;  // portfolio starts + structured jitter, as cuda/packing.cu:806,821

@@ -41,8 +41,19 @@ struct InstDesc {
     int shrinks;
     int feasible;
     int fp64_rejects;           // FP32-zero configurations the FP64 test found overlapping
-    int pad;
+    int collective;             // collective proposals per replica after each launch's sweeps
+    int kick_every;             // intervals between kicks (0: none)
+    int kicks;                  // kicks applied (counted once per replica)
     unsigned long long proposals, accepted, cand, near, ex_tried, ex_acc;
+    unsigned long long coll_tried, coll_acc;
+};
+
+// Per-replica scratch of the collective moves and the kick (LDS).
+struct CollWork {
+    float cx, cy, cc, cs;       // anchor square
+    float mv, rt, tc, ts;       // translation, rotation cos/sin
+    float th, uacc, eold, enew;
+    int mode, anchor, accept, worst;
 };
 
 struct Tile {
@@ -88,7 +99,8 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot
     b = (b + 15) & ~(size_t)15;
     b += (size_t)G * 8;                         // per-replica occupied-group mask
-    b += (size_t)G * 8 * 4;                     // shift x/y (this and next sweep), steps, 1/T, accepts
+    b += (size_t)G * sizeof(CollWork);          // collective-move / kick scratch
+    b += (size_t)G * 9 * 4;                     // shift x/y (this and next sweep), steps, 1/T, accepts, energy
     b += 2 * (kGroups + 2) * 4;                 // level counts / cursors, level starts
     return b;
 }

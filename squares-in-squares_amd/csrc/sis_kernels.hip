@@ -203,6 +203,8 @@ struct TileLds {
     float* sxy;
     float* sth;
     float* ik;
+    float* eout;      // canonical energy per replica at the end of the launch
+    CollWork* cw;     // per replica: collective-move / kick scratch
     int* acc;
     int* gcnt;        // kGroups + 2: level counts, then scatter cursors
     int* gstart;      // kGroups + 2: level starts (read by every wavefront of a large tile)
@@ -234,6 +236,8 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     off = (off + 15) & ~(size_t)15;
     t.gmask = reinterpret_cast<unsigned long long*>(smem + off);
     off += 8 * (size_t)G;
+    t.cw = reinterpret_cast<CollWork*>(smem + off);
+    off += sizeof(CollWork) * (size_t)G;
     float* f = reinterpret_cast<float*>(smem + off);
     t.shx = f;
     t.shy = f + G;
@@ -242,7 +246,8 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     t.sxy = f + 4 * G;
     t.sth = f + 5 * G;
     t.ik = f + 6 * G;
-    t.acc = reinterpret_cast<int*>(f + 7 * G);
+    t.eout = f + 7 * G;
+    t.acc = reinterpret_cast<int*>(f + 8 * G);
     t.gcnt = t.acc + G;
     t.gstart = t.gcnt + (kGroups + 2);
     return t;
@@ -500,6 +505,199 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
 }
 
 // ---------------------------------------------------------------------------
+// Collective moves (cuda/packing.cu:564-616) and the kick (:696-745), applied
+// to a tile after its sweeps.  Every draw is Philox on (index, replica,
+// sweep or interval, purpose 5..8); the energies are the canonical FP32 ones.
+// ---------------------------------------------------------------------------
+// (cos a, sin a) for a in [0, pi/2) without a transcendental: the polynomial
+// rotation of (cos pi/4, sin pi/4) by a - pi/4.
+__device__ __forceinline__ void angle_cs(float a, float& c, float& s) {
+    rot32(0.70710678f, 0.70710678f, a - 0.78539816f, c, s);
+}
+
+// Does square z join the group of a collective move anchored at w?  Modes:
+// 1 row, 2 column, 3 cluster, 4 same orientation (within 0.06 rad of the
+// anchor's modulo pi/2), 5 the two diagonals, 6 tilted squares (more than
+// 0.04 rad from axis-aligned).
+__device__ __forceinline__ bool coll_take(const CollWork& w, float4 z) {
+    const float dx = z.x - w.cx, dy = z.y - w.cy;
+    switch (w.mode) {
+        case 1: return fabsf(dy) < 0.55f;
+        case 2: return fabsf(dx) < 0.55f;
+        case 3: return __builtin_fmaf(dx, dx, dy * dy) < 2.5f;
+        case 4: {
+            const float cd = __builtin_fmaf(z.z, w.cc, z.w * w.cs);
+            const float sd = __builtin_fmaf(z.w, w.cc, -(z.z * w.cs));
+            return fmaxf(fabsf(cd), fabsf(sd)) > 0.99820054f;  // cos 0.06
+        }
+        case 5: return fabsf(dx + dy) < 0.85f || fabsf(dx - dy) < 0.85f;
+        default: return fmaxf(fabsf(z.z), fabsf(z.w)) < 0.99920011f;  // cos 0.04
+    }
+}
+
+// One collective proposal for every replica of the tile.  t.eout holds the
+// canonical energy on entry and on exit.  The configuration before the move
+// is kept in the replica's own HBM slot, written and read back by the same
+// thread.
+template <int NT>
+__device__ void collective_step(const TileLds& t, int n, int G, int M, int m, float inv_w, float L,
+                                int r0, uint32_t sweep_end, uint32_t c, uint32_t k0, uint32_t k1,
+                                float4* gstate, unsigned& tried, unsigned& accd) {
+    const int tid = threadIdx.x;
+    for (int g = tid; g < G; g += NT) {
+        CollWork& w = t.cw[g];
+        const uint4 o1 = philox(c, (uint32_t)(r0 + g), sweep_end, 5u, k0, k1);
+        const uint4 o2 = philox(c, (uint32_t)(r0 + g), sweep_end, 6u, k0, k1);
+        w.anchor = min(n - 1, (int)(u01(o1.x) * (float)n));
+        w.mode = 1 + min(5, (int)(u01(o1.y) * 6.0f));
+        const float4 a = t.pos[g * n + w.anchor];
+        w.cx = a.x; w.cy = a.y; w.cc = a.z; w.cs = a.w;
+        w.mv = t.sxy[g] * __builtin_fmaf(2.0f, u01(o1.z), -1.0f);
+        w.rt = t.sxy[g] * __builtin_fmaf(2.0f, u01(o1.w), -1.0f);
+        w.th = t.sth[g] * __builtin_fmaf(2.0f, u01(o2.x), -1.0f);
+        rot32(1.0f, 0.0f, w.th, w.tc, w.ts);
+        w.uacc = u01(o2.y);
+        w.eold = t.eout[g];
+    }
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
+        const int g = q / n, i = q - g * n;
+        const CollWork& w = t.cw[g];
+        const float4 z = t.pos[q];
+        gstate[q] = z;
+        if (!coll_take(w, z)) continue;
+        const float dx = z.x - w.cx, dy = z.y - w.cy;
+        float nx, ny;
+        if (w.mode == 3 || w.mode == 4) {  // rotate the group about the anchor, then shift
+            nx = (w.cx + __builtin_fmaf(w.tc, dx, -(w.ts * dy))) + w.mv;
+            ny = (w.cy + __builtin_fmaf(w.ts, dx, w.tc * dy)) + w.rt;
+        } else if (w.mode == 6) {          // split: alternate squares move apart
+            const float sg = (i & 1) ? 1.0f : -1.0f;
+            nx = z.x + sg * w.mv;
+            ny = z.y + sg * w.rt;
+        } else {
+            nx = z.x + w.mv;
+            ny = z.y + w.rt;
+        }
+        float nc, ns;
+        rot32(z.z, z.w, w.th, nc, ns);
+        t.pos[q] = make_float4(fminf(L - 0.45f, fmaxf(0.45f, nx)), fminf(L - 0.45f, fmaxf(0.45f, ny)), nc, ns);
+    }
+    tile_sync<NT>();
+    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+    for (int g = tid; g < G; g += NT) {
+        CollWork& w = t.cw[g];
+        const float en = t.eout[g], dE = en - w.eold;
+        w.accept = (dE <= 0.0f || w.uacc < exp2f(-dE * t.ik[g])) ? 1 : 0;
+        t.eout[g] = w.accept ? en : w.eold;
+        tried += 1;
+        accd += (unsigned)w.accept;
+    }
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT)
+        if (!t.cw[q / n].accept) t.pos[q] = gstate[q];
+    tile_sync<NT>();
+}
+
+// Kick of every replica with energy (cuda/packing.cu:696-745): per-square
+// energies (wall + every stencil pair), the worst square (first on ties); on
+// a relocating kick, when its energy exceeds 1e-5, it moves to the probe of
+// an 8 x 8 grid farthest from every other centre (first on ties) with a
+// +-0.05 jitter and a uniform angle; then every square is jittered, squares
+// with energy above 1e-6 by +-0.08 / +-0.18 rad, the rest by +-0.008 /
+// +-0.015 rad, centres clamped to [0.48, L - 0.48].  t.eout holds the
+// canonical energy on entry and on exit.
+template <int NT>
+__device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, float L, int r0,
+                     uint32_t interval, bool relocate, uint32_t k0, uint32_t k1, unsigned& kicked) {
+    const int tid = threadIdx.x, gw = grid_words(M);
+    float* sq = reinterpret_cast<float*>(t.rank);  // rank + items: 4 bytes per square
+    for (int g = tid; g < G; g += NT) {
+        t.gmask[g] = 0ull;
+        t.shx[g] = 0.0f;
+        t.shy[g] = 0.0f;
+    }
+    tile_sync<NT>();
+    bin_tile<NT>(t, n, G, M, m, inv_w);  // cell lists of the current positions
+    for (int q = tid; q < G * n; q += NT) {
+        const int g = q / n, i = q - g * n;
+        if (!(t.eout[g] > 1e-12f)) continue;
+        const float4 p = t.pos[q];
+        const int ck = t.cellk[q];
+        const Stencil sc(t.A + g * gw, M, ck & 255, ck >> 8);
+        const float4* pg = t.pos + (size_t)g * n;
+        float ei = wall32(p.x, p.y, p.z, p.w, L);
+        for (int f = 0; f < sc.total; ++f) {
+            const int j = t.sorted[sc.at(f)];
+            if (j == i) continue;
+            const float4 v = pg[j];
+            ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
+        }
+        sq[q] = ei;
+        atomicMax(&t.gmask[g], ((unsigned long long)__float_as_uint(ei) << 32) | (0xFFFFFFFFu - (unsigned)i));
+    }
+    tile_sync<NT>();
+    for (int g = tid; g < G; g += NT) {
+        CollWork& w = t.cw[g];
+        const unsigned long long k = t.gmask[g];
+        w.worst = (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFull));
+        w.eold = __uint_as_float((unsigned)(k >> 32));
+        w.accept = relocate && t.eout[g] > 1e-12f && w.eold > 1e-5f;  // relocate this replica
+        t.gmask[g] = 0ull;
+        if (t.eout[g] > 1e-12f) kicked += 1;
+    }
+    tile_sync<NT>();
+    if (relocate) {
+        for (int k = tid; k < G * 64; k += NT) {
+            const int g = k >> 6, pidx = k & 63;
+            const CollWork& w = t.cw[g];
+            if (!w.accept) continue;
+            const float vx = __builtin_fmaf(L - 1.1f, ((pidx & 7) + 0.5f) * 0.125f, 0.55f);
+            const float vy = __builtin_fmaf(L - 1.1f, ((pidx >> 3) + 0.5f) * 0.125f, 0.55f);
+            const float4* pg = t.pos + (size_t)g * n;
+            float d2 = 1e9f;
+            for (int j = 0; j < n; ++j) {
+                if (j == w.worst) continue;
+                const float4 v = pg[j];
+                const float dx = vx - v.x, dy = vy - v.y;
+                d2 = fminf(d2, __builtin_fmaf(dx, dx, dy * dy));
+            }
+            atomicMax(&t.gmask[g], ((unsigned long long)__float_as_uint(d2) << 32) | (0xFFFFFFFFu - (unsigned)pidx));
+        }
+        tile_sync<NT>();
+        for (int g = tid; g < G; g += NT) {
+            const CollWork& w = t.cw[g];
+            if (!w.accept) continue;
+            const int pidx = (int)(0xFFFFFFFFu - (unsigned)(t.gmask[g] & 0xFFFFFFFFull));
+            const float vx = __builtin_fmaf(L - 1.1f, ((pidx & 7) + 0.5f) * 0.125f, 0.55f);
+            const float vy = __builtin_fmaf(L - 1.1f, ((pidx >> 3) + 0.5f) * 0.125f, 0.55f);
+            const uint4 o = philox(0u, (uint32_t)(r0 + g), interval, 7u, k0, k1);
+            const float x = fminf(L - 0.5f, fmaxf(0.5f, vx + 0.05f * __builtin_fmaf(2.0f, u01(o.x), -1.0f)));
+            const float y = fminf(L - 0.5f, fmaxf(0.5f, vy + 0.05f * __builtin_fmaf(2.0f, u01(o.y), -1.0f)));
+            float c, s;
+            angle_cs(u01(o.z) * 1.57079633f, c, s);
+            t.pos[g * n + w.worst] = make_float4(x, y, c, s);
+        }
+        tile_sync<NT>();
+    }
+    for (int q = tid; q < G * n; q += NT) {
+        const int g = q / n, i = q - g * n;
+        if (!(t.eout[g] > 1e-12f)) continue;
+        const bool hot = sq[q] > 1e-6f;
+        const float axy = hot ? 0.08f : 0.008f, aa = hot ? 0.18f : 0.015f;
+        const uint4 o = philox((uint32_t)i, (uint32_t)(r0 + g), interval, 8u, k0, k1);
+        const float4 p = t.pos[q];
+        const float x = fminf(L - 0.48f, fmaxf(0.48f, p.x + axy * __builtin_fmaf(2.0f, u01(o.x), -1.0f)));
+        const float y = fminf(L - 0.48f, fmaxf(0.48f, p.y + axy * __builtin_fmaf(2.0f, u01(o.y), -1.0f)));
+        float c, s;
+        rot32(p.z, p.w, aa * __builtin_fmaf(2.0f, u01(o.z), -1.0f), c, s);
+        t.pos[q] = make_float4(x, y, c, s);
+    }
+    tile_sync<NT>();
+    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+}
+
+// ---------------------------------------------------------------------------
 // k_sweep: `spe` sweeps (or what is left of the budget) of every replica of
 // one tile.  One sweep = every square proposes exactly once.  NT threads: one
 // wavefront, or four for large instances (the same arithmetic either way).
@@ -522,6 +720,8 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
     const int m = grid_cells(L, w);
     const uint32_t k0 = D->key0, k1 = D->key1;
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
+    const int ncoll = D->collective, kick_every = D->kick_every;
+    const uint32_t interval = D->interval;
     TileLds t = carve(smem, n, M, G);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
@@ -637,13 +837,19 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         t.shy_next = tmp;
     }
 
-    float* Eout = t.sth;  // reuse: steps are no longer needed
-    tile_energy<NT>(t, n, G, M, m, inv_w, L, Eout);
+    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+    unsigned ctried = 0, caccd = 0, kicked = 0;
+    for (int c = 0; c < ncoll; ++c)
+        collective_step<NT>(t, n, G, M, m, inv_w, L, tl.r0, sweep0 + (uint32_t)nsw, (uint32_t)c, k0, k1,
+                            gstate, ctried, caccd);
+    if (kick_every > 0 && (interval + 1u) % (uint32_t)kick_every == 0u)
+        kick<NT>(t, n, G, M, m, inv_w, L, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u,
+                 k0, k1, kicked);
     for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
     unsigned long long accsum = 0;
     for (int g = tid; g < G; g += NT) {
         RepState* rs = reps + D->rep_off + tl.r0 + g;
-        rs->E = Eout[g];
+        rs->E = t.eout[g];
         rs->acc = (unsigned)t.acc[g];
         accsum += (unsigned)t.acc[g];
     }
@@ -652,11 +858,19 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         cC += __shfl_xor(cC, off, 64);
         cP += __shfl_xor(cP, off, 64);
         accsum += __shfl_xor(accsum, off, 64);
+        ctried += __shfl_xor(ctried, off, 64);
+        caccd += __shfl_xor(caccd, off, 64);
+        kicked += __shfl_xor(kicked, off, 64);
     }
     if (lane == 0) {  // one set of counter atomics per wavefront
         atomicAdd(&D->cand, cC);
         atomicAdd(&D->near, cP);
         atomicAdd(&D->accepted, accsum);
+        if (ctried) {
+            atomicAdd(&D->coll_tried, (unsigned long long)ctried);
+            atomicAdd(&D->coll_acc, (unsigned long long)caccd);
+        }
+        if (kicked) atomicAdd(&D->kicks, (int)kicked);
         if (tid == 0) atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
     }
 }

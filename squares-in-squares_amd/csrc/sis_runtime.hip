@@ -113,6 +113,8 @@ int validate(const sis_instance* in, const sis_params* p) {
     if (p->polish_iters < 0 || p->polish_iters > 100000) return fail(SIS_EINVAL, "polish_iters out of range");
     if (!(p->init_jitter >= 0 && p->init_jitter < 1)) return fail(SIS_EINVAL, "init_jitter in [0,1)");
     if (p->init_mode < 0 || p->init_mode > 2) return fail(SIS_EINVAL, "init_mode must be 0, 1 or 2");
+    if (p->collective_moves < 0 || p->collective_moves > 64) return fail(SIS_EINVAL, "collective_moves in 0..64");
+    if (p->kick_every < 0 || p->kick_every > 1000000) return fail(SIS_EINVAL, "kick_every in 0..1e6");
     const float w = (float)p->cell;
     const int M = grid_cells((float)in->L, w) + 4;
     if (M > 255) return fail(SIS_EINVAL, "cell grid wider than 255 cells: raise the cell width");
@@ -308,6 +310,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
         D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
         D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
+        D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
         nsq += (size_t)s.n * s.p.replicas;
         nrep += s.p.replicas;
         narch += s.n;
@@ -569,6 +572,9 @@ void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis
     out->seconds = wall_s;
     out->device_seconds = b.device_ms * 1e-3;
     out->fp64_rejects = D.fp64_rejects;
+    out->kicks = D.kicks;
+    out->collective_tried = D.coll_tried;
+    out->collective_accepted = D.coll_acc;
     (void)k;
 }
 

@@ -271,6 +271,7 @@ int Server::admit_one(Req* r, Lane& ln) {
     D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
     D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
     D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
+    D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
     std::vector<RepState> reps(R);
     std::vector<SlotState> slots(R);
     std::vector<int> ros(R);

@@ -46,7 +46,7 @@ class SisParams(ctypes.Structure):
         ("sweeps", ctypes.c_int64), ("replicas", ctypes.c_int), ("ladder", ctypes.c_int),
         ("sweeps_per_exchange", ctypes.c_int), ("replicas_per_group", ctypes.c_int),
         ("polish_iters", ctypes.c_int), ("init_mode", ctypes.c_int), ("flags", ctypes.c_int),
-        ("rng_seed", ctypes.c_uint64),
+        ("rng_seed", ctypes.c_uint64), ("collective_moves", ctypes.c_int), ("kick_every", ctypes.c_int),
     ]
 
 
@@ -61,7 +61,8 @@ class SisResult(ctypes.Structure):
         ("candidates", ctypes.c_uint64), ("near_pairs", ctypes.c_uint64),
         ("shrinks", ctypes.c_int), ("feasible", ctypes.c_int), ("replicas", ctypes.c_int),
         ("status", ctypes.c_int), ("seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
-        ("fp64_rejects", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("fp64_rejects", ctypes.c_int), ("kicks", ctypes.c_int),
+        ("collective_tried", ctypes.c_uint64), ("collective_accepted", ctypes.c_uint64),
     ]
 
 
@@ -119,6 +120,9 @@ def lib() -> ctypes.CDLL:
     L.sis_dev_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                  fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                  fp, ip, fp, u64p]
+    L.sis_dev_epoch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_uint32, fp, ip, fp, u64p]
     L.sis_dev_polish.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
     _lib = L
     return L
@@ -392,8 +396,11 @@ def dev_cellcoord(x: np.ndarray, shift: float, w: float, lo: int, hi: int) -> np
 
 
 def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, step_th, key0: int,
-               key1: int, sweep0: int, sweeps: int, group: int = 1):
-    """xycs: (R, n, 4) float32 (x, y, cos, sin); returns (state, accepted, energy, counts)."""
+               key1: int, sweep0: int, sweeps: int, group: int = 1, collective: int = 0, kick_every: int = 0,
+               interval: int = 0):
+    """xycs: (R, n, 4) float32 (x, y, cos, sin); returns (state, accepted, energy, counts).  With
+    collective / kick_every the launch ends with collective moves and a kick (counts then has six
+    entries: proposals, candidates, near pairs, collective proposals, collective accepts, kicks)."""
     R, n, _ = xycs.shape
     st = np.ascontiguousarray(xycs, dtype=np.float32).copy()
     ik = np.ascontiguousarray(np.broadcast_to(np.asarray(inv_t_log2e, np.float32), (R,)))
@@ -401,11 +408,18 @@ def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, st
     sth = np.ascontiguousarray(np.broadcast_to(np.asarray(step_th, np.float32), (R,)))
     acc = np.zeros(R, dtype=np.int32)
     en = np.zeros(R, dtype=np.float32)
-    counts = np.zeros(3, dtype=np.uint64)
-    rc = lib().sis_dev_sweeps(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
-                              _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps,
-                              _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int), _ptr(en, ctypes.c_float),
-                              _ptr(counts, ctypes.c_uint64))
+    if collective or kick_every:
+        counts = np.zeros(6, dtype=np.uint64)
+        rc = lib().sis_dev_epoch(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
+                                 _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps, collective, kick_every,
+                                 interval, _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int),
+                                 _ptr(en, ctypes.c_float), _ptr(counts, ctypes.c_uint64))
+    else:
+        counts = np.zeros(3, dtype=np.uint64)
+        rc = lib().sis_dev_sweeps(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
+                                  _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps,
+                                  _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int), _ptr(en, ctypes.c_float),
+                                  _ptr(counts, ctypes.c_uint64))
     if rc:
         _err(rc)
     return st, acc, en, counts

@@ -78,6 +78,8 @@ class Oracle:
         L.o_sweep32_model.argtypes = [P(SweepCfg), P(f32), P(f32), P(f32), P(f32), P(u64)]
         L.o_energy32_model.argtypes = [i32, f32, f32, P(f32), P(f32), P(f32), P(f32)]
         L.o_energy32_model.restype = f32
+        L.o_collective32_model.argtypes = [P(SweepCfg), i32, P(f32), P(f32), P(f32), P(f32), P(f32)]
+        L.o_kick32_model.argtypes = [P(SweepCfg), u32, i32, P(f32), P(f32), P(f32), P(f32), P(f32)]
         L.o_grid_packing.argtypes = [i32, P(f64)]
 
     # -- thin wrappers -------------------------------------------------------
@@ -120,6 +122,21 @@ class Oracle:
 
     def energy32(self, x, y, c, s, L, w):
         return self.lib.o_energy32_model(len(x), L, w, ptr(x, f32), ptr(y, f32), ptr(c, f32), ptr(s, f32))
+
+    def collective_model(self, x, y, c, s, L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep_end,
+                         moves):
+        cfg = SweepCfg(len(x), L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep_end)
+        e = ctypes.c_float()
+        acc = self.lib.o_collective32_model(ctypes.byref(cfg), moves, ptr(x, f32), ptr(y, f32), ptr(c, f32),
+                                            ptr(s, f32), ctypes.byref(e))
+        return acc, e.value
+
+    def kick_model(self, x, y, c, s, L, w, key0, key1, replica, interval, relocate):
+        cfg = SweepCfg(len(x), L, w, 0.0, 0.0, 0.0, key0, key1, replica, 0)
+        e = ctypes.c_float()
+        k = self.lib.o_kick32_model(ctypes.byref(cfg), interval, int(relocate), ptr(x, f32), ptr(y, f32),
+                                    ptr(c, f32), ptr(s, f32), ctypes.byref(e))
+        return k, e.value
 
     def ptsa(self, n, L0, replicas, tmin, tmax, sweeps, swap_every=10, shrink=1e-4, seed=42, threads=1,
              ladder=0, move_sigma=0.05, rot_sigma=0.05, target_accept=0.3, init=None):

@@ -124,6 +124,45 @@ def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     assert dcnt[0] == R * n * sweeps and np.array_equal(dcnt, mcnt)
 
 
+@pytest.mark.parametrize("n,L,R,sweeps,group,moves,kick_every,interval", [
+    (40, 8.0, 6, 3, 2, 4, 1, 0),      # kick 1: relocating
+    (40, 8.0, 4, 2, 1, 2, 1, 1),      # kick 2: jitter only
+    (300, 17.5, 2, 2, 1, 3, 2, 1),    # four-wavefront tiles, relocating kick
+])
+def test_collective_moves_and_kick_match_cpu_model(sis, oracle, n, L, R, sweeps, group, moves, kick_every,
+                                                   interval):
+    """Sweeps, then collective moves (cuda/packing.cu:564-616), then the kick
+    (:696-745) on the device == the oracle's model, bit for bit: states,
+    energies and the collective / kick counters."""
+    rng = np.random.default_rng(n + moves)
+    st = _seeded_state(rng, R, n, L)
+    T = np.geomspace(1e-3, 1e-1, R)
+    ik = (LOG2E / T).astype(np.float32)
+    sxy = np.full(R, 0.12, np.float32)
+    sth = np.full(R, 0.1, np.float32)
+    say("collective+kick", n, R, moves, kick_every, interval)
+    dst, dacc, den, dcnt = sis.dev_sweeps(st, L, 0.7075, ik, sxy, sth, 7, 0, 11, sweeps, group=group,
+                                          collective=moves, kick_every=kick_every, interval=interval)
+    mst, macc, _, _ = _model_run(oracle, st, L, 0.7075, ik, sxy, sth, 7, 0, 11, sweeps)
+    coll_acc, kicked = 0, 0
+    men = np.zeros(R, np.float32)
+    relocate = ((interval + 1) // kick_every) % 2 == 1
+    for r in range(R):
+        x, y, c, s = (np.ascontiguousarray(mst[r, :, k]) for k in range(4))
+        a, e = oracle.collective_model(x, y, c, s, L, 0.7075, sxy[r], sth[r], ik[r], 7, 0, r, 11 + sweeps, moves)
+        coll_acc += a
+        if (interval + 1) % kick_every == 0:
+            k, e = oracle.kick_model(x, y, c, s, L, 0.7075, 7, 0, r, interval, relocate)
+            kicked += k
+        mst[r, :, 0], mst[r, :, 1], mst[r, :, 2], mst[r, :, 3] = x, y, c, s
+        men[r] = e
+    assert np.array_equal(dacc, macc)
+    assert np.array_equal(dst, mst)
+    assert np.array_equal(den, men)
+    assert dcnt[3] == R * moves and dcnt[4] == coll_acc and dcnt[5] == kicked
+    assert kicked > 0
+
+
 def test_sweep_energy_vs_fp64_recompute(sis, oracle):
     """Energy after device sweeps equals the FP64 all-pairs recompute of the
     same state within 1e-5 relative (cuda/tests/globaltest.cu:49)."""
