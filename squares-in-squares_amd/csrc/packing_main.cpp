@@ -1,11 +1,15 @@
-// `packing seed.txt epochs steps chains rng_seed out.json` on MI355X.
+// `packing seed.txt epochs steps chains rng_seed out.json` on MI355X, and,
+// built with SIS_SEEDED, `seeded` with the same arguments.
 //
-// Keeps the reference's process contract (cuda/packing.cu:768-868): the argv
-// ranges and exit code 2, the seed text `n L` + n lines `x y angle`, the
-// PACK_* environment knobs with their ranges and "invalid NAME" message, and
-// the raw JSON fields at precision 17.  The search itself runs through the C
-// ABI of libsisamd (colour-checkerboard FP32 sweeps, replica exchange, FP64
-// polish and certified side on the GPU).
+// Keeps the reference's process contracts: the argv ranges and exit code 2,
+// the seed text `n L` + n lines `x y angle`, the environment knobs with their
+// ranges and "invalid NAME" message, and the raw JSON fields at precision 17
+// (`packing`: cuda/packing.cu:768-868, PACK_* knobs; `seeded`:
+// cuda/seeded.cu:832-969, SEED_* knobs, a fixed coldest temperature 1e-11,
+// switchable collective moves, polish and structured starts).  The search
+// itself runs through the C ABI of libsisamd (colour-checkerboard FP32
+// sweeps, collective moves and kicks, replica exchange, FP64 polish and
+// certified side on the GPU).
 //
 // Mapping of the reference arguments onto this engine:
 //   epochs  exchange rounds of the search (0: 50/40/30 by n; >120 capped to 80)
@@ -43,7 +47,11 @@ static void json_escape(std::ostream& o, const char* s) {
 
 int main(int argc, char** argv) {
     if (argc != 7) {
+#ifdef SIS_SEEDED
+        std::cerr << "seeded seed.txt epochs steps chains seed output.json\n";
+#else
         std::cerr << "packing seed.txt epochs steps chains random_seed output.json\n";
+#endif
         return 2;
     }
     int epochs = std::atoi(argv[2]), steps = std::atoi(argv[3]), chains = std::atoi(argv[4]);
@@ -62,6 +70,18 @@ int main(int argc, char** argv) {
             !std::isfinite(seed[3 * i + 2]))
             return 2;
     }
+#ifdef SIS_SEEDED
+    // cuda/seeded.cu:846-853
+    const double shrink = envf("SEED_SHRINK", .001, 1e-7, .1);
+    const double tmax = envf("SEED_TMAX", .0001, 1e-11, .1);
+    const double move = envf("SEED_MOVE", .02, 1e-6, .3);
+    const int collective = (int)envf("SEED_COLLECTIVE", 1, 0, 1);
+    const int polish = (int)envf("SEED_POLISH", 200, 0, 4096);
+    const int structured = (int)envf("SEED_STRUCTURED", 1, 0, 1);
+    const int autotune = (int)envf("SEED_AUTOTUNE", 1, 0, 1);
+    const int max_chains = (int)envf("SEED_MAX_CHAINS", 768, 8, 4096);
+    const double tmin = 1e-11, rot = .12;  // cuda/seeded.cu:494: ladder from 1e-11 to tmax
+#else
     const double tmin = envf("PACK_TMIN", 1e-11, 1e-12, 1.0);
     const double tmax = envf("PACK_TMAX", 1e-4, 1e-12, 1.0);
     const double move = envf("PACK_MOVE", .02, .000001, .3);
@@ -70,6 +90,8 @@ int main(int argc, char** argv) {
     const double shrink = envf("PACK_SHRINK", .001, .000001, .1);
     const int autotune = (int)envf("PACK_AUTOTUNE", 1, 0, 1);
     const int max_chains = (int)envf("PACK_MAX_CHAINS", 768, 8, 4096);
+    const int collective = 1, polish = 200, structured = 1;
+#endif
     const unsigned long long rng_seed = std::strtoull(argv[5], nullptr, 10);
 
     if (sis_init(0) != SIS_OK) {
@@ -101,12 +123,15 @@ int main(int argc, char** argv) {
     p.sweeps = (int64_t)eff_epochs * steps;
     p.sweeps_per_exchange = std::max(1, std::min(steps, 10));
     // the reference's epoch mixes collective moves into its steps
-    // (cuda/packing.cu:564-616) and kicks once per epoch (:696-745)
-    p.collective_moves = 4;
-    p.kick_every = std::max(1, steps / p.sweeps_per_exchange);
-    p.polish_iters = 200;
+    // (cuda/packing.cu:564-616) and, when it polishes, kicks once per epoch
+    // (:696-745)
+    p.collective_moves = collective ? 4 : 0;
+    p.kick_every = polish > 0 ? std::max(1, steps / p.sweeps_per_exchange) : 0;
+    p.polish_iters = polish;
     p.rng_seed = rng_seed;
-    p.init_mode = 2;  // portfolio starts + structured jitter, as cuda/packing.cu:806,821
+    // portfolio starts + structured jitter (cuda/packing.cu:806,821), or the
+    // seed in every chain
+    p.init_mode = structured ? 2 : 0;
     sis_instance inst{n, L, seed.data()};
     std::vector<double> q(3 * (size_t)n);
     sis_result r{};
@@ -123,6 +148,19 @@ int main(int argc, char** argv) {
     std::ofstream out(argv[6]);
     if (!out) return 2;
     out.precision(17);
+#ifdef SIS_SEEDED
+    // cuda/seeded.cu:950-969
+    out << "{\"n\":" << n << ",\"target_L\":" << bestL << ",\"seconds\":" << r.seconds
+        << ",\"shrinks\":" << (improved ? r.shrinks : 0) << ",\"epochs\":" << eff_epochs << ",\"steps\":" << steps
+        << ",\"chains\":" << chains << ",\"device\":\"";
+    json_escape(out, name);
+    out << "\",\"sm_count\":" << cus << ",\"blocks_per_sm\":" << 2 << ",\"wave_chains\":" << wave_chains
+        << ",\"collective\":" << collective << ",\"polish\":" << polish << ",\"structured\":" << structured
+        << ",\"proposals\":" << r.proposals + r.collective_tried
+        << ",\"accepted\":" << r.accepted + r.collective_accepted
+        << ",\"exchange_attempts\":" << r.exchanges_tried << ",\"exchange_accepts\":" << r.exchanges_accepted
+        << ",\"q\":[";
+#else
     out << "{\"n\":" << n << ",\"target_L\":" << bestL << ",\"seconds\":" << r.seconds
         << ",\"chain_steps\":" << (long long)chains * eff_epochs * steps << ",\"proposals\":" << r.proposals
         << ",\"feasible_checkpoints\":" << std::max(1, r.feasible ? r.shrinks + 1 : 0)
@@ -131,6 +169,7 @@ int main(int argc, char** argv) {
         << ",\"device\":\"";
     json_escape(out, name);
     out << "\",\"q\":[";
+#endif
     for (int i = 0; i < n; ++i) {
         if (i) out << ",";
         out << "[" << best[3 * i] << "," << best[3 * i + 1] << "," << best[3 * i + 2] << "]";

@@ -148,6 +148,42 @@ def test_cli_tmax_below_tmin(tmp_path):
     assert r.returncode == 2
 
 
+# -- the `seeded` CLI keeps cuda/seeded.cu:832-969 ------------------------------
+SEEDED = PKG / "bin" / "seeded"
+
+
+def run_seeded(args, env=None):
+    e = {k: v for k, v in os.environ.items() if not k.startswith("SEED_")}
+    if env:
+        e.update(env)
+    return subprocess.run([str(SEEDED), *args], capture_output=True, text=True, env=e, timeout=120)
+
+
+def test_seeded_usage_exit_2():
+    r = run_seeded(["x"])
+    assert r.returncode == 2 and "seeded seed.txt" in r.stderr
+
+
+@pytest.mark.parametrize("name,value", [("SEED_SHRINK", "0"), ("SEED_TMAX", "0.5"), ("SEED_MOVE", "1"),
+                                        ("SEED_COLLECTIVE", "2"), ("SEED_POLISH", "5000"),
+                                        ("SEED_STRUCTURED", "-1"), ("SEED_AUTOTUNE", "x"),
+                                        ("SEED_MAX_CHAINS", "7")])
+def test_seeded_env_knobs_range_checked(tmp_path, name, value):
+    seed = tmp_path / "seed.txt"
+    seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
+    r = run_seeded([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")], env={name: value})
+    assert r.returncode == 2
+    assert f"invalid {name}" in r.stderr
+
+
+def test_seeded_argument_and_seed_validation(tmp_path):
+    seed = tmp_path / "seed.txt"
+    seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
+    assert run_seeded([str(seed), "0", "0", "12", "42", str(tmp_path / "o.json")]).returncode == 2
+    seed.write_text("2 3\n0.5 0.5 0\n")
+    assert run_seeded([str(seed), "0", "0", "0", "42", str(tmp_path / "o.json")]).returncode == 2
+
+
 def test_server_without_device_fails_loudly(sis):
     if _has_device(sis):
         pytest.skip("a HIP device is present; covered by the GPU tests")

@@ -340,3 +340,38 @@ def test_packing_cli_end_to_end(tmp_path, ref_certifier):
     assert raw["target_L"] <= w * f + 1e-12
     dil, L50 = certify_q(ref_certifier, np.asarray(raw["q"]))
     assert dil is not None
+
+
+def test_seeded_cli_end_to_end(tmp_path, ref_certifier):
+    """`seeded seed.txt epochs steps chains seed out.json` keeps the JSON
+    fields of cuda/seeded.cu:950-969 and its output certifies; with
+    SEED_STRUCTURED=0 and SEED_COLLECTIVE=0 it still runs and certifies."""
+    import json
+    import os
+    import subprocess
+
+    from conftest import PKG
+    from refcert import certify_q
+    n = 10
+    f = 1.00002
+    rows = [((i % 4 + 0.5) * f, (i // 4 + 0.5) * f, 0.0) for i in range(n)]
+    seed = tmp_path / "seed10.txt"
+    seed.write_text(f"{n} {4 * f:.12g}\n" + "".join(f"{x:.12g} {y:.12g} {a:.12g}\n" for x, y, a in rows))
+    for env in ({}, {"SEED_STRUCTURED": "0", "SEED_COLLECTIVE": "0", "SEED_POLISH": "0"}):
+        out = tmp_path / "raw10.json"
+        say("seeded cli", env)
+        e = {k: v for k, v in os.environ.items() if not k.startswith("SEED_")}
+        e.update(env)
+        r = subprocess.run([str(PKG / "bin" / "seeded"), str(seed), "12", "32", "64", "7", str(out)],
+                           capture_output=True, text=True, timeout=300, env=e)
+        assert r.returncode == 0, r.stderr
+        raw = json.loads(out.read_text())
+        for k in ("n", "target_L", "seconds", "shrinks", "epochs", "steps", "chains", "device", "sm_count",
+                  "blocks_per_sm", "wave_chains", "collective", "polish", "structured", "proposals", "accepted",
+                  "exchange_attempts", "exchange_accepts", "q"):
+            assert k in raw, k
+        assert raw["collective"] == int(env.get("SEED_COLLECTIVE", 1))
+        assert raw["structured"] == int(env.get("SEED_STRUCTURED", 1))
+        assert raw["target_L"] <= 4 * f + 1e-12 and len(raw["q"]) == n
+        dil, L50 = certify_q(ref_certifier, np.asarray(raw["q"]))
+        assert dil is not None
