@@ -137,6 +137,25 @@ __device__ __forceinline__ float quad_xor1(float v) {
 __device__ __forceinline__ float quad_xor2(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
+// Inclusive prefix sum over the 64 lanes of a wavefront in DPP moves (row
+// shifts by 1, 2, 4, 8 inside each row of 16 lanes, then the row totals
+// broadcast to the following rows), no LDS round trip.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// Value of the lane N places up inside a row of 16 lanes (DPP row shift).
+template <int N>
+__device__ __forceinline__ float row_up(float v) {
+    static_assert(N >= 1 && N <= 15, "row shift of 1..15 lanes");
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x100 + N, 0xF, 0xF, true));
+}
 
 // 16-bit counters packed two per 32-bit LDS word (counts never exceed 1000).
 __device__ __forceinline__ unsigned add_u16(uint16_t* base, int idx, unsigned v) {
@@ -162,12 +181,7 @@ __device__ void wave_scan_u16(uint16_t* a, int K) {
         sum += (w.x & 0xFFFFu) + (w.x >> 16) + (w.y & 0xFFFFu) + (w.y >> 16) + (w.z & 0xFFFFu) +
                (w.z >> 16) + (w.w & 0xFFFFu) + (w.w >> 16);
     }
-    unsigned incl = sum;
-#pragma unroll
-    for (int off = 1; off < kSweepThreads; off <<= 1) {
-        const unsigned t = __shfl_up(incl, off, kSweepThreads);
-        if (lane >= off) incl += t;
-    }
+    const unsigned incl = (unsigned)wave_incl_scan((int)sum);
     unsigned run = incl - sum;
     for (int k = b; k < e; ++k) {
         uint4 w = v[k];
@@ -209,11 +223,20 @@ struct TileLds {
     int* gcnt;        // kGroups + 2: level counts, then scatter cursors
     int* gstart;      // kGroups + 2: level starts (read by every wavefront of a large tile)
     unsigned long long* gmask;  // per replica: which colour x rank groups are occupied
+    unsigned divn;    // ceil(2^26 / n): replica of a tile square by a multiply
 };
+
+// Replica of tile square q, q / n, as a multiply by m = ceil(2^26 / n): exact
+// while q (m n - 2^26) < 2^26, which q < 2^16 (G n <= 65535) and n < 2^10
+// guarantee.
+__device__ __forceinline__ int rep_of(const TileLds& t, int q) {
+    return (int)__umulhi((unsigned)q << 6, t.divn);
+}
 
 __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     TileLds t;
     const int gn = G * n;
+    t.divn = (67108864u + (unsigned)n - 1u) / (unsigned)n;
     t.pos = reinterpret_cast<float4*>(smem);
     float4* next = t.pos + gn;
     t.rnd = nullptr;
@@ -266,7 +289,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
     for (int k = tid; k < (G * gw) / 8; k += NT) A128[k] = make_uint4(0u, 0u, 0u, 0u);
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n;
+        const int g = rep_of(t, q);
         const float4 p = t.pos[q];
         const int kx = cellcoord(p.x, t.shx[g], inv_w, 0, m - 1) + 2;
         const int ky = cellcoord(p.y, t.shy[g], inv_w, 0, m - 1) + 2;
@@ -277,7 +300,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
     if (tid < 64) wave_scan_u16(t.A, G * gw);
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
         const unsigned slot = add_u16(t.A + g * gw, c + 1, 1u);
@@ -285,7 +308,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
     }
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
         const uint16_t* Ag = t.A + g * gw;
@@ -296,7 +319,7 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
     }
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         const int ck = t.cellk[q];
         const int c = (ck >> 8) * M + (ck & 255);
         const int k = t.A[g * gw + c] + t.rank[q];
@@ -350,8 +373,9 @@ struct Stencil {
 };
 
 // One Metropolis proposal for square i of tile replica g, computed by `lpi`
-// consecutive lanes (1, 2 or 4; this lane is `sub`).  The move variates come
-// from t.rnd (drawn for the whole sweep before the first level).  The stencil
+// consecutive lanes (1, 2, 4, 8 or 16; this lane is `sub`).  The move
+// variates come from t.rnd (drawn for the whole sweep before the first level)
+// or are drawn here.  The stencil
 // energy change is four partial sums over the flat candidate index f
 // (f mod 4), each in increasing f, combined as (s0 + s1) + (s2 + s3): the
 // same bits whatever the number of lanes sharing the work.
@@ -359,7 +383,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
                                         float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
                                         int lpi, int sub, unsigned long long& cC,
                                         unsigned long long& cP) {
-    const int g = q / n;
+    const int g = rep_of(t, q);
     const int gw = grid_words(M);
     const float4 p = t.pos[q];
     const float4 r = t.rnd ? t.rnd[q] : move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
@@ -440,7 +464,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         }
         const float h = sa + sb;
         sum = h + quad_xor1(h);
-    } else {
+    } else if (lpi == 4) {
         // sub k owns partial k
         float sa = 0.0f;
         for (int f = sub; f <= last; f += 16) {
@@ -449,6 +473,28 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
             sa += d1;
             sa += d2;
             sa += d3;
+        }
+        const float h = sa + quad_xor1(sa);
+        sum = h + quad_xor2(h);
+    } else {
+        // 8 or 16 lanes, one candidate each per round of lpi candidates: lane
+        // sub holds f = round + sub, and partial k (f = k mod 4) folds the
+        // lanes k, k + 4, (k + 8, k + 12) of each round in that (increasing f)
+        // order into lane k; the lanes of a mover are aligned inside one row
+        // of 16 lanes.
+        float sa = 0.0f;
+        for (int base = 0; base <= last; base += lpi) {
+            const int f = base + sub;
+            const int k = sc.at(min(f, last));
+            const float4 v = t.spos ? t.spos[k] : pg[t.sorted[k]];
+            const float d = delta(v, k, f <= last);
+            const float d4 = row_up<4>(d);
+            if (lpi == 8) {
+                sa = (sa + d) + d4;
+            } else {
+                const float d8 = row_up<8>(d), d12 = row_up<12>(d);
+                sa = (((sa + d) + d4) + d8) + d12;
+            }
         }
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
@@ -477,7 +523,7 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
     float* e = reinterpret_cast<float*>(t.rank);   // rank + items: 4 bytes per square
     const int gw = grid_words(M);
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
         const int kx = ck & 255, ky = ck >> 8;
@@ -495,7 +541,7 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
     tile_sync<NT>();
     for (int s = 1; s < n; s <<= 1) {
         for (int q = tid; q < G * n; q += NT) {
-            const int g = q / n, i = q - g * n;
+            const int g = rep_of(t, q), i = q - g * n;
             if ((i & (2 * s - 1)) == 0 && i + s < n) e[q] += e[q + s];
         }
         tile_sync<NT>();
@@ -561,7 +607,7 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
     }
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         const CollWork& w = t.cw[g];
         const float4 z = t.pos[q];
         gstate[q] = z;
@@ -595,7 +641,7 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
     }
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT)
-        if (!t.cw[q / n].accept) t.pos[q] = gstate[q];
+        if (!t.cw[rep_of(t, q)].accept) t.pos[q] = gstate[q];
     tile_sync<NT>();
 }
 
@@ -620,7 +666,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
     tile_sync<NT>();
     bin_tile<NT>(t, n, G, M, m, inv_w);  // cell lists of the current positions
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         if (!(t.eout[g] > 1e-12f)) continue;
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
@@ -681,7 +727,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         tile_sync<NT>();
     }
     for (int q = tid; q < G * n; q += NT) {
-        const int g = q / n, i = q - g * n;
+        const int g = rep_of(t, q), i = q - g * n;
         if (!(t.eout[g] > 1e-12f)) continue;
         const bool hot = sq[q] > 1e-6f;
         const float axy = hot ? 0.08f : 0.008f, aa = hot ? 0.18f : 0.015f;
@@ -756,7 +802,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         // Small tiles draw the sweep's move variates here, one Philox block
         // per square, while the LDS atomics of the key pass are in flight.
         for (int q = tid; q < G * n; q += NT) {
-            const int g = q / n, i = q - g * n;
+            const int g = rep_of(t, q), i = q - g * n;
             const int ck = t.cellk[q];
             const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
             const int r = t.rank[q];
@@ -778,7 +824,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         for (int q = tid; q < G * n; q += NT) {
             const int key = t.rank[q];
             const int lvl =
-                key < kGroups ? __popcll(t.gmask[q / n] & ((1ull << key) - 1ull)) : kGroups;
+                key < kGroups ? __popcll(t.gmask[rep_of(t, q)] & ((1ull << key) - 1ull)) : kGroups;
             t.rank[q] = (uint16_t)lvl;
             atomicAdd(&t.gcnt[lvl], 1);
         }
@@ -791,13 +837,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         int gs = 0;
         if (tid < 64) {
             const int c = lane <= kGroups ? t.gcnt[lane] : 0;
-            int incl = c;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int up = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += up;
-            }
-            gs = incl - c;
+            gs = wave_incl_scan(c) - c;
             if (lane <= kGroups) t.gcnt[lane] = gs;
             if (NT > 64 && lane <= kGroups + 1) t.gstart[lane] = gs;
         }
@@ -812,8 +852,10 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             const int a = __builtin_amdgcn_readlane(gs, lv);
             const int b = __builtin_amdgcn_readlane(gs, lv + 1);
             if (a == b) break;  // levels are dense: the first empty one is the last
-            // sparse levels give each mover 2 or 4 lanes (of one wavefront)
-            const int lpi = b - a > NT / 2 ? 1 : (b - a > NT / 4 ? 2 : 4);
+            // sparse levels give each mover 2, 4, 8 or 16 lanes (aligned
+            // inside one row of 16 lanes of one wavefront)
+            const int cnt = b - a;
+            const int lpi = cnt > NT / 2 ? 1 : cnt > NT / 4 ? 2 : cnt > NT / 8 ? 4 : cnt > NT / 16 ? 8 : 16;
             const int per = NT / lpi;
             const int slot = tid / lpi, sub = tid - slot * lpi;
             for (int base = a; base < b; base += per)
