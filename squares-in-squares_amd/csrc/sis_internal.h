@@ -77,12 +77,13 @@ struct SlotState {              // per global temperature slot
 // LDS of one sweep tile (G replicas of n squares, M x M cells).
 // Cell starts per replica: M*M + 2 u16, padded to 16 bytes.
 __host__ __device__ inline int grid_words(int M) { return (M * M + 2 + 7) & ~7; }
-// Small tiles draw the sweep's move variates ahead into LDS (their levels are
-// sparse, so several lanes share a move and would each draw them); large
-// tiles move one square per lane and draw inline, keeping their LDS, and with
-// it the tiles resident per compute unit, smaller.
-constexpr int kRndLdsMaxSquares = 512;
-__host__ __device__ inline bool sweep_rnd_in_lds(int gn) { return gn <= kRndLdsMaxSquares; }
+// Small tiles stage every square's proposal of the sweep (variates, proposed
+// position and orientation, wall-energy change) into LDS in one pass over all
+// lanes before the first level, so the sparse levels only evaluate the
+// stencil; large tiles move one square per lane and propose inline, keeping
+// their LDS, and with it the tiles resident per compute unit, smaller.
+constexpr int kPropLdsMaxSquares = 512;
+__host__ __device__ inline bool sweep_prop_in_lds(int gn) { return gn <= kPropLdsMaxSquares; }
 // Tiles up to this size also keep their positions in cell-list order (one
 // LDS read per candidate instead of an index read and a position read).
 // Larger tiles skip the copy so that more of them fit a compute unit
@@ -93,7 +94,7 @@ __host__ __device__ inline bool sweep_spos_in_lds(int gn) { return gn <= kSposLd
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     size_t gn = (size_t)G * n;
     size_t b = 16 * gn;                         // float4 x, y, cos, sin
-    if (sweep_rnd_in_lds((int)gn)) b += 16 * gn;  // float4 move variates of the sweep
+    if (sweep_prop_in_lds((int)gn)) b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
     if (sweep_spos_in_lds((int)gn)) b += 16 * gn;  // float4 positions in cell-list order
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
     b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot

@@ -200,8 +200,9 @@ __device__ void wave_scan_u16(uint16_t* a, int K) {
 
 struct TileLds {
     float4* pos;
-    float4* rnd;      // per square: move variates fx, fy, ftheta in [-1, 1) and the accept
-                      // uniform; null when the tile draws them inline
+    float4* prop;     // per square: proposed x, y, cos, sin of the sweep (cos = kNoMove:
+                      // rejected before the stencil); null when the tile proposes inline
+    float2* pw;       // per square: wall-energy change of the proposal, accept uniform
     float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k];
                       // null when the tile reads positions through `sorted`
     uint16_t* A;      // per replica grid_words(M) cell starts
@@ -239,15 +240,20 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     t.divn = (67108864u + (unsigned)n - 1u) / (unsigned)n;
     t.pos = reinterpret_cast<float4*>(smem);
     float4* next = t.pos + gn;
-    t.rnd = nullptr;
-    if (sweep_rnd_in_lds(gn)) {
-        t.rnd = next;
+    t.prop = nullptr;
+    t.pw = nullptr;
+    if (sweep_prop_in_lds(gn)) {
+        t.prop = next;
         next += gn;
     }
     t.spos = nullptr;
     if (sweep_spos_in_lds(gn)) {
         t.spos = next;
         next += gn;
+    }
+    if (t.prop) {
+        t.pw = reinterpret_cast<float2*>(next);
+        next += (gn + 1) / 2;  // keeps the cell starts 16-byte aligned
     }
     t.A = reinterpret_cast<uint16_t*>(next);
     t.sorted = t.A + (size_t)G * grid_words(M);
@@ -372,10 +378,31 @@ struct Stencil {
     }
 };
 
+// Proposal of tile square q (replica g) at state p with variates r: the new
+// centre and orientation and the wall-energy change, or false when the move
+// leaves the box or its cell (rejected before any neighbour is examined).
+struct Move {
+    float nx, ny, nc, ns, wdiff;
+};
+constexpr float kNoMove = 8.0f;  // cos field of a staged proposal rejected early
+__device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4& p, const float4& r, int kx,
+                                          int ky, int m, float inv_w, float L, Move& mv) {
+    const float sxy = t.sxy[g];
+    mv.nx = p.x + sxy * r.x;
+    mv.ny = p.y + sxy * r.y;
+    if (!(mv.nx >= 0.0f && mv.nx <= L && mv.ny >= 0.0f && mv.ny <= L)) return false;
+    if (cellcoord(mv.nx, t.shx[g], inv_w, 0, m - 1) + 2 != kx ||
+        cellcoord(mv.ny, t.shy[g], inv_w, 0, m - 1) + 2 != ky)
+        return false;
+    rot32(p.z, p.w, t.sth[g] * r.z, mv.nc, mv.ns);
+    mv.wdiff = wall32(mv.nx, mv.ny, mv.nc, mv.ns, L) - wall32(p.x, p.y, p.z, p.w, L);
+    return true;
+}
+
 // One Metropolis proposal for square i of tile replica g, computed by `lpi`
-// consecutive lanes (1, 2, 4, 8 or 16; this lane is `sub`).  The move
-// variates come from t.rnd (drawn for the whole sweep before the first level)
-// or are drawn here.  The stencil
+// consecutive lanes (1, 2, 4, 8 or 16; this lane is `sub`).  The move comes
+// from t.prop (staged for the whole sweep before the first level) or is
+// drawn here.  The stencil
 // energy change is four partial sums over the flat candidate index f
 // (f mod 4), each in increasing f, combined as (s0 + s1) + (s2 + s3): the
 // same bits whatever the number of lanes sharing the work.
@@ -386,22 +413,25 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     const int g = rep_of(t, q);
     const int gw = grid_words(M);
     const float4 p = t.pos[q];
-    const float4 r = t.rnd ? t.rnd[q] : move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
     const int self = t.slotq[q];
     const int ck = t.cellk[q];
     const int kx = ck & 255, ky = ck >> 8;
     // the stencil depends only on the current cell: its loads go out first
     const Stencil sc(t.A + g * gw, M, kx, ky);
-    const float sxy = t.sxy[g];
-    const float nx = p.x + sxy * r.x;
-    const float ny = p.y + sxy * r.y;
-    if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) return;
-    if (cellcoord(nx, t.shx[g], inv_w, 0, m - 1) + 2 != kx ||
-        cellcoord(ny, t.shy[g], inv_w, 0, m - 1) + 2 != ky)
-        return;
-    float nc, ns;
-    rot32(p.z, p.w, t.sth[g] * r.z, nc, ns);
-    const float wdiff = wall32(nx, ny, nc, ns, L) - wall32(p.x, p.y, p.z, p.w, L);
+    Move mv;
+    float u;
+    if (t.prop) {
+        const float4 a = t.prop[q];
+        if (a.z == kNoMove) return;
+        const float2 b = t.pw[q];
+        mv = Move{a.x, a.y, a.z, a.w, b.x};
+        u = b.y;
+    } else {
+        const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
+        if (!make_move(t, g, p, r, kx, ky, m, inv_w, L, mv)) return;
+        u = r.w;
+    }
+    const float nx = mv.nx, ny = mv.ny, nc = mv.nc, ns = mv.ns, wdiff = mv.wdiff;
     const int last = sc.total - 1;  // >= 0: the mover is in its own cell
     unsigned cnt = 0, near = 0;
     // Energy change against the square at cell-list position k.  The mover
@@ -502,7 +532,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     cC += cnt;
     cP += near;
     const float dE = wdiff + sum;
-    if (sub == 0 && (dE <= 0.0f || r.w < exp2f(-dE * t.ik[g]))) {
+    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * t.ik[g]))) {
         const float4 np = make_float4(nx, ny, nc, ns);
         t.pos[q] = np;
         if (t.spos) t.spos[self] = np;
@@ -799,8 +829,9 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         tile_sync<NT>();
         bin_tile<NT>(t, n, G, M, m, inv_w);
         // group key: occupancy rank x colour; ranks >= kRankCap go serial.
-        // Small tiles draw the sweep's move variates here, one Philox block
-        // per square, while the LDS atomics of the key pass are in flight.
+        // Small tiles stage every square's proposal here (one Philox block
+        // per square; a square's own state does not change before its level
+        // runs), while the LDS atomics of the key pass are in flight.
         for (int q = tid; q < G * n; q += NT) {
             const int g = rep_of(t, q), i = q - g * n;
             const int ck = t.cellk[q];
@@ -809,7 +840,13 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             const int key = r < kRankCap ? r * 9 + colour : kGroups;
             t.rank[q] = (uint16_t)key;
             atomicOr(&t.gmask[g], 1ull << key);
-            if (t.rnd) t.rnd[q] = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
+            if (t.prop) {
+                const float4 r = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
+                Move mv;
+                const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
+                t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
+                t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
+            }
         }
         if (sw + 1 < nsw)
             for (int g = tid; g < G; g += NT) {
