@@ -219,6 +219,12 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
                   uint32_t sweep0, int sweeps, int collective, int kick_every, uint32_t interval,
                   float* xycs, int* accepted, float* energy, uint64_t counts[6]);
+/* The FP64 zero-overlap test that gates archiving and shrinking after each
+ * exchange step (cuda/packing.cu:625-632, where every pair is summed): the
+ * grid test used by the exchange kernel (zero = 1 when the configuration has
+ * no overlap) beside the all-pairs FP64 energy of the same configuration.
+ * xycs: n float4 (x, y, cos, sin). */
+int sis_dev_zero64(int n, double L, const float* xycs, int* zero, double* e64);
 /* FP64 polish + certified side of one configuration (cuda/packing.cu:624-697):
  * q in/out n*3; writes the exported (translated, dilated) configuration. */
 int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,

@@ -54,7 +54,7 @@ public:
     std::vector<InstDesc> h_inst;
     std::vector<Tile> h_tiles;  // tiles of large instances (four wavefronts) first
     int n_big = 0;              // number of those
-    size_t lds_big = 0, lds_small = 0;
+    size_t lds_big = 0, lds_small = 0, lds_ladder = 0;
     size_t n_sq = 0, n_rep = 0, n_arch = 0;
     long long max_intervals = 0, intervals_done = 0;
     double device_ms = 0, last_ms = 0, sweep_ms = 0;

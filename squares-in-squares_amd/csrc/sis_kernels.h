@@ -29,8 +29,12 @@ struct FinalizeOut {
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
                         InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
                         int threads);
+// Probe: the exchange kernel's zero-overlap test and the all-pairs FP64
+// energy of one configuration.
+hipError_t launch_zero64(int n, double L, const float4* q, int* zero, double* e64);
 // list: ninst instance indices into insts, or null for insts[0..ninst).
-hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
+// lds: the largest ladder_lds_bytes (n, initial L) over those instances.
+hipError_t launch_ladder(hipStream_t st, int ninst, size_t lds, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
                          const int* list = nullptr);
 hipError_t launch_finalize(hipStream_t st, int njobs, const FinalizeJob* jobs, FinalizeOut* outs,

@@ -1068,6 +1068,116 @@ __device__ double total64(const double* x, const double* y, const double* c, con
     return block_sum_d<NT>(e, tmp);
 }
 
+// Exclusive prefix sum of one int per thread over the workgroup (NT a
+// multiple of 64, at most 4096): wavefront 0 scans the NT values, NT / 64 per
+// lane.
+template <int NT>
+__device__ int block_excl_scan(int v, int* part) {
+    static_assert(NT % 64 == 0 && NT <= 4096, "NT / 64 values per lane of one wavefront");
+    part[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        constexpr int K = NT / 64;
+        const int l = threadIdx.x;
+        int loc[K], sum = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            loc[k] = part[l * K + k];
+            sum += loc[k];
+        }
+        int run = wave_incl_scan(sum) - sum;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            part[l * K + k] = run;
+            run += loc[k];
+        }
+    }
+    __syncthreads();
+    const int r = part[threadIdx.x];
+    __syncthreads();
+    return r;
+}
+
+// Zero-overlap test in FP64 of the configuration held in LDS: true when every
+// wall term and every pair term that total64 adds is exactly zero, i.e. when
+// total64 returns 0.  Terms are never negative, so the sum is zero exactly
+// when each term is.  A pair term is zero unless d^2 < 2.000001, so only pairs
+// in the same or adjacent cells of a uniform grid of width 1.5 can be
+// nonzero; each such pair i < j is evaluated with the arguments total64
+// passes.  O(n) instead of the O(n^2) of total64.
+struct ZeroGrid {
+    int* start;       // zero_grid_side(L)^2 + 1
+    int* part;        // kLadderThreads
+    int* flag;
+    uint16_t* idx;    // n
+};
+// The exchange kernel's LDS (ladder_lds_bytes) for n squares at side L.
+struct LadderLds {
+    double *x, *y, *c, *s;
+    ZeroGrid zg;
+};
+__device__ inline LadderLds carve_ladder(unsigned char* smem, int n, double L) {
+    LadderLds t;
+    const int mc = zero_grid_side(L);
+    t.x = reinterpret_cast<double*>(smem);
+    t.y = t.x + n;
+    t.c = t.y + n;
+    t.s = t.c + n;
+    t.zg.start = reinterpret_cast<int*>(t.s + n);
+    t.zg.part = t.zg.start + (mc * mc + 1);
+    t.zg.flag = t.zg.part + kLadderThreads;
+    t.zg.idx = reinterpret_cast<uint16_t*>(t.zg.flag + 1);
+    return t;
+}
+template <int NT>
+__device__ bool overlap_free64(const double* x, const double* y, const double* c, const double* s, int n,
+                               double L, const ZeroGrid& zg) {
+    const int tid = threadIdx.x;
+    const int mc = zero_grid_side(L);
+    const int ncell = mc * mc;
+    auto cell = [&](double v) {
+        const int k = (int)floor(v / kZeroCell);
+        return k < 0 ? 0 : (k > mc - 1 ? mc - 1 : k);
+    };
+    for (int k = tid; k <= ncell; k += NT) zg.start[k] = 0;
+    if (tid == 0) *zg.flag = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) atomicAdd(&zg.start[cell(y[i]) * mc + cell(x[i]) + 1], 1);
+    __syncthreads();
+    // start[c] = squares in cells before c (counts were placed one cell up)
+    const int per = (ncell + 1 + NT - 1) / NT;
+    const int b = min(ncell + 1, tid * per), e = min(ncell + 1, b + per);
+    int sum = 0;
+    for (int k = b; k < e; ++k) sum += zg.start[k];
+    int run = block_excl_scan<NT>(sum, zg.part);
+    for (int k = b; k < e; ++k) {
+        run += zg.start[k];
+        zg.start[k] = run;
+    }
+    __syncthreads();
+    // scatter; start[c] then holds the end of cell c (the start of c + 1)
+    for (int i = tid; i < n; i += NT) zg.idx[atomicAdd(&zg.start[cell(y[i]) * mc + cell(x[i])], 1)] = (uint16_t)i;
+    __syncthreads();
+    bool bad = false;
+    for (int i = tid; i < n; i += NT) {
+        bad |= wall64(x[i], y[i], c[i], s[i], L) > 0.0;
+        const int cx = cell(x[i]), cy = cell(y[i]);
+        for (int yy = max(cy - 1, 0); yy <= min(cy + 1, mc - 1); ++yy)
+            for (int xx = max(cx - 1, 0); xx <= min(cx + 1, mc - 1); ++xx) {
+                const int cc = yy * mc + xx;
+                for (int k = cc ? zg.start[cc - 1] : 0; k < zg.start[cc]; ++k) {
+                    const int j = zg.idx[k];
+                    if (j > i) bad |= pair64(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]) > 0.0;
+                }
+            }
+    }
+    if (bad) *zg.flag = 1;
+    __syncthreads();
+    const bool ok = *zg.flag == 0;
+    __syncthreads();
+    return ok;
+}
+
 // ---------------------------------------------------------------------------
 // k_ladder: one workgroup per instance (insts[list[b]], or insts[b] without a
 // list), after every sweep launch.
@@ -1079,8 +1189,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                                                            int* __restrict__ rep_of_slot,
                                                            float4* __restrict__ state,
                                                            float4* __restrict__ arch) {
-    __shared__ double x[kMaxN], y[kMaxN], cc[kMaxN], ss[kMaxN];
-    __shared__ double tmp[kLadderThreads];
+    extern __shared__ __align__(16) unsigned char lsm[];
     __shared__ int zero_slot;
     InstDesc* D = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
     const long long left = D->sweeps_total - D->sweeps_done;
@@ -1138,15 +1247,16 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     if (zero_slot < R) {
         const int r = rep_of_slot[base + zero_slot];
         const float4* src = state + D->state_off + (size_t)r * n;
+        const LadderLds t = carve_ladder(lsm, n, L0);
         for (int i = tid; i < n; i += kLadderThreads) {
             const float4 v = src[i];
             const double c = v.z, s = v.w, h = 1.0 / sqrt(c * c + s * s);
-            x[i] = v.x; y[i] = v.y; cc[i] = c * h; ss[i] = s * h;
+            t.x[i] = v.x; t.y[i] = v.y; t.c[i] = c * h; t.s[i] = s * h;
         }
         __syncthreads();
-        const double e64 = total64<kLadderThreads>(x, y, cc, ss, n, L0, tmp);
-        if (e64 != 0.0 && tid == 0) D->fp64_rejects += 1;
-        if (e64 == 0.0) {
+        const bool zero64 = overlap_free64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L0, t.zg);
+        if (!zero64 && tid == 0) D->fp64_rejects += 1;
+        if (zero64) {
             archive = !had_arch || L0 < L_arch;
             if (archive)
                 for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
@@ -1365,6 +1475,33 @@ __global__ __launch_bounds__(kPolishThreads) void k_finalize(const FinalizeJob* 
 // ---------------------------------------------------------------------------
 // Parity probes.
 // ---------------------------------------------------------------------------
+// Probe: the ladder's grid zero-overlap test beside the all-pairs FP64 energy
+// of one configuration (x, y, cos, sin as float4, normalised as k_ladder does).
+__global__ __launch_bounds__(kLadderThreads) void k_zero64(int n, double L, const float4* __restrict__ q,
+                                                           int* __restrict__ zero, double* __restrict__ e64) {
+    extern __shared__ __align__(16) unsigned char lsm[];
+    __shared__ double tmp[kLadderThreads];
+    const LadderLds t = carve_ladder(lsm, n, L);
+    for (int i = threadIdx.x; i < n; i += kLadderThreads) {
+        const float4 v = q[i];
+        const double c = v.z, s = v.w, h = 1.0 / sqrt(c * c + s * s);
+        t.x[i] = v.x; t.y[i] = v.y; t.c[i] = c * h; t.s[i] = s * h;
+    }
+    __syncthreads();
+    const bool z = overlap_free64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L, t.zg);
+    const double e = total64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L, tmp);
+    if (threadIdx.x == 0) {
+        *zero = z ? 1 : 0;
+        *e64 = e;
+    }
+}
+
+hipError_t launch_zero64(int n, double L, const float4* q, int* zero, double* e64) {
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kLadderThreads), ladder_lds_bytes(n, L), nullptr, n, L, q, zero,
+                       e64);
+    return hipGetLastError();
+}
+
 __global__ void k_depth32(int npairs, const float* __restrict__ in, float* __restrict__ depth,
                           float* __restrict__ d2out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1400,10 +1537,10 @@ hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tile
     return hipGetLastError();
 }
 
-hipError_t launch_ladder(hipStream_t st, int ninst, InstDesc* insts, RepState* reps,
+hipError_t launch_ladder(hipStream_t st, int ninst, size_t lds, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
                          const int* list) {
-    hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), 0, st, insts, list, reps, slots,
+    hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), lds, st, insts, list, reps, slots,
                        rep_of_slot, state, arch);
     return hipGetLastError();
 }

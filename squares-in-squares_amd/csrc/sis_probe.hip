@@ -125,6 +125,24 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
     return SIS_OK;
 }
 
+int sis_dev_zero64(int n, double L, const float* xycs, int* zero, double* e64) {
+    if (n < 1 || n > kMaxN || !(L > 0 && L <= 64) || !xycs || !zero || !e64) return fail(SIS_EINVAL, "bad arguments");
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    DevBuf<float4> d_q;
+    DevBuf<int> d_z;
+    DevBuf<double> d_e;
+    SIS_HIPCHECK(d_q.alloc(n));
+    SIS_HIPCHECK(d_z.alloc(1));
+    SIS_HIPCHECK(d_e.alloc(1));
+    SIS_HIPCHECK(hipMemcpy(d_q, xycs, sizeof(float4) * n, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_zero64(n, L, d_q, d_z, d_e));
+    SIS_HIPCHECK(hipDeviceSynchronize());
+    SIS_HIPCHECK(hipMemcpy(zero, d_z, sizeof(int), hipMemcpyDeviceToHost));
+    SIS_HIPCHECK(hipMemcpy(e64, d_e, sizeof(double), hipMemcpyDeviceToHost));
+    return SIS_OK;
+}
+
 int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,
                    double* energy_after, double* lam, double* L_cert, double* q_export) {
     if (n < 1 || n > kMaxN || !(L > 0 && L <= 64) || iters < 0 || !q) return fail(SIS_EINVAL, "bad arguments");

@@ -241,14 +241,16 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
         // the tile that keeps the most lanes busy per pass wins: measured at
         // 2048 replicas of one instance (1e9 proposals/s), one-wavefront tiles
         // of about 64 squares (n=10: 1 replica 1.04, 2: 1.20, 4: 1.30, 8: 1.21;
-        // n=30: 2.71, 3.07, 2.65; n=100: 7.27, 6.26, 4.14; n=200: 10.8, 8.0)
-        // and four-wavefront tiles of 500-800 squares (n=256: 9.4, 13.3;
-        // n=400: 12.5, 14.1; n=600: 13.4, 11.9; n=1000: 15.6, 9.0).
+        // n=30: 2.71, 3.07, 2.65; n=64: 5.83, 5.54) and four-wavefront tiles
+        // of 400-1000 squares (replicas 1, 2, 3, 4: n=100: 4.36, 6.70, 7.20,
+        // 7.53; n=150: 6.00, 8.93, 9.99, 8.61; n=200: 7.69, 11.53, 9.77, 9.89;
+        // n=300: 9.90, 12.38, 12.09, 11.26; n=500: 14.86, 15.85, 13.63, 14.80;
+        // n=700: 14.71, 13.29; n=1000: 17.23, 15.43).
         const int cus = std::max(1, dev.cus);
         G = 1;
         if ((long long)batch_replicas >= 8ll * cus) {
             if (big)
-                G = s.n < 500 ? 2 : 1;
+                G = s.n < 600 ? std::min(4, std::max(2, 512 / s.n)) : 1;
             else
                 while (2 * G * s.n <= 64) G *= 2;
         }
@@ -295,7 +297,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
     h_inst.assign(ni, InstDesc{});
     h_tiles.clear();
     size_t nsq = 0, nrep = 0, narch = 0;
-    lds_big = lds_small = 0;
+    lds_big = lds_small = lds_ladder = 0;
     max_intervals = 0;
     for (int k = 0; k < ni; ++k) {
         const InstSpec& s = specs[k];
@@ -316,6 +318,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         narch += s.n;
         size_t& lds = s.n >= kBigN ? lds_big : lds_small;
         lds = std::max(lds, pl.lds);
+        lds_ladder = std::max(lds_ladder, ladder_lds_bytes(s.n, s.L));
         const long long iv = (s.p.sweeps + pl.spe - 1) / pl.spe;
         max_intervals = std::max(max_intervals, iv);
         for (int r0 = 0; r0 < s.p.replicas; r0 += pl.G)
@@ -393,7 +396,7 @@ int Batch::launch_tiles() {
 
 int Batch::enqueue_interval() {
     if (int rc = launch_tiles()) return rc;
-    HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
+    HIPCHECK(launch_ladder(stream, (int)specs.size(), lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
     return SIS_OK;
 }
 
@@ -407,7 +410,7 @@ int Batch::run_plain(long long intervals) {
         HIPCHECK(hipEventRecord(evs0, stream));
         if (int rc = launch_tiles()) return rc;
         HIPCHECK(hipEventRecord(evs1, stream));
-        HIPCHECK(launch_ladder(stream, (int)specs.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
+        HIPCHECK(launch_ladder(stream, (int)specs.size(), lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
         HIPCHECK(hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * specs.size(),
                                 hipMemcpyDeviceToHost, stream));
         HIPCHECK(hipStreamSynchronize(stream));

@@ -116,7 +116,7 @@ struct Lane {
     std::vector<Tile> tiles;          // four-wavefront tiles first (n_big of them)
     std::vector<int> list;            // descriptor slots of `active`
     int n_big = 0;
-    size_t lds_big = 0, lds_small = 0;
+    size_t lds_big = 0, lds_small = 0, lds_ladder = 0;
     Tile* d_tiles = nullptr;
     int* d_list = nullptr;
     InstDesc* d_desc = nullptr;       // descriptors of `active` after the round
@@ -304,7 +304,7 @@ int Server::launch_round(Lane& ln) {
     if (ln.dirty) {
         ln.tiles.clear();
         ln.list.clear();
-        ln.lds_big = ln.lds_small = 0;
+        ln.lds_big = ln.lds_small = ln.lds_ladder = 0;
         long long total_reps = 0;
         for (Req* r : ln.active) total_reps += r->spec.p.replicas;
         for (Req* r : ln.active) {
@@ -316,6 +316,7 @@ int Server::launch_round(Lane& ln) {
             r->plan.G = pl.G;  // grouping never changes a replica's arithmetic
             size_t& lds = r->spec.n >= kBigN ? ln.lds_big : ln.lds_small;
             lds = std::max(lds, pl.lds);
+            ln.lds_ladder = std::max(ln.lds_ladder, ladder_lds_bytes(r->spec.n, r->spec.L));
             for (int r0 = 0; r0 < r->spec.p.replicas; r0 += pl.G)
                 ln.tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), 0});
             ln.list.push_back(r->slot);
@@ -337,7 +338,7 @@ int Server::launch_round(Lane& ln) {
         if (nt > ln.n_big)
             SRV_CHECK(launch_sweep(ln.stream, nt - ln.n_big, ln.lds_small, ln.d_tiles + ln.n_big, d_inst, d_reps,
                                    d_slots, d_state, kSweepThreads));
-        SRV_CHECK(launch_ladder(ln.stream, (int)ln.list.size(), d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
+        SRV_CHECK(launch_ladder(ln.stream, (int)ln.list.size(), ln.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
                                 ln.d_list));
     }
     SRV_CHECK(launch_gather_desc(ln.stream, (int)ln.list.size(), d_inst, ln.d_list, ln.d_desc));

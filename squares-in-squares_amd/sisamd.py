@@ -124,6 +124,7 @@ def lib() -> ctypes.CDLL:
                                 fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_uint32, fp, ip, fp, u64p]
     L.sis_dev_polish.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+    L.sis_dev_zero64.argtypes = [ctypes.c_int, ctypes.c_double, fp, ip, dp]
     _lib = L
     return L
 
@@ -423,6 +424,18 @@ def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, st
     if rc:
         _err(rc)
     return st, acc, en, counts
+
+
+def dev_zero64(xycs: np.ndarray, L: float):
+    """The exchange kernel's FP64 zero-overlap test of one configuration ((n, 4) float32 x, y,
+    cos, sin) and the all-pairs FP64 energy of the same configuration: (zero, e64)."""
+    q = np.ascontiguousarray(xycs, dtype=np.float32)
+    n = q.shape[0]
+    z, e = ctypes.c_int(0), ctypes.c_double(0.0)
+    rc = lib().sis_dev_zero64(n, L, _ptr(q, ctypes.c_float), ctypes.byref(z), ctypes.byref(e))
+    if rc:
+        _err(rc)
+    return bool(z.value), e.value
 
 
 def dev_polish(q: np.ndarray, L: float, iters: int):

@@ -102,13 +102,14 @@ def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
     return st, acc, en, cnt
 
 
-@pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (120, 12.5, 3, 3, 2),
+@pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (90, 11.0, 3, 3, 1),
+                                                (100, 11.0, 4, 3, 4), (120, 12.5, 3, 3, 2),
                                                 (300, 17.5, 2, 3, 1), (300, 17.5, 4, 2, 2)])
 def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     """k_sweep == the CPU model of the same colour-checkerboard sweep: same
-    states, accept counts and canonical energies, bit for bit.  n = 300 runs
-    the four-wavefront tiles, with the move variates drawn ahead into LDS
-    (300 squares per tile) and inline (600)."""
+    states, accept counts and canonical energies, bit for bit.  n >= 96 runs
+    the four-wavefront tiles, with the proposals staged in LDS (up to 512
+    squares per tile) and drawn inline (600)."""
     rng = np.random.default_rng(n)
     st = _seeded_state(rng, R, n, L)
     T = np.geomspace(1e-3, 1e-1, R)
@@ -181,6 +182,60 @@ def test_sweep_energy_vs_fp64_recompute(sis, oracle):
             c, s = np.ascontiguousarray(c * h), np.ascontiguousarray(s * h)
             e64 = oracle.lib.o_total64_cs(n, ptr(x, f64), ptr(y, f64), ptr(c, f64), ptr(s, f64), L)
             assert abs(en[r] - e64) <= 1e-5 * max(1.0, e64), (r, en[r], e64)
+
+
+def _xycs(x, y, a):
+    return np.ascontiguousarray(np.stack([x, y, np.cos(a), np.sin(a)], axis=-1).astype(np.float32))
+
+
+def test_zero_overlap_grid_test_equals_all_pairs(sis):
+    """The exchange kernel's FP64 zero-overlap test (a grid of width 1.5)
+    accepts a configuration exactly when the all-pairs FP64 energy of
+    cuda/packing.cu:625-632 is zero: touching grids, grids overlapping by
+    1e-9, one displaced square (every neighbour direction, across cell
+    boundaries), squares through a wall, the reference's own raw outputs and
+    random dense configurations."""
+    import json
+    from conftest import ROOT
+    cases = []
+    for n, pitch in ((25, 1.0), (25, 1.0 - 1e-9), (1000, 1.000001), (1000, 1.0), (400, 1.0 - 1e-7)):
+        k = int(math.isqrt(n - 1)) + 1
+        i = np.arange(n)
+        x, y = (i % k + 0.5) * pitch, (i // k + 0.5) * pitch
+        cases.append((_xycs(x, y, np.zeros(n)), k * pitch))
+    rng = np.random.default_rng(17)
+    for trial in range(60):
+        # a touching 12 x 12 grid at pitch 1.5 with one square pushed towards a neighbour
+        k, pitch = 12, 1.5
+        n = k * k
+        i = np.arange(n)
+        x, y = (i % k + 0.75) * pitch - 0.375, (i // k + 0.75) * pitch - 0.375
+        a = rng.uniform(0, 0.02, n)
+        j = int(rng.integers(0, n))
+        ang = rng.uniform(0, 2 * math.pi)
+        d = rng.choice([0.2, 0.499999, 0.5, 0.500001, 0.7])
+        x[j] += d * math.cos(ang)
+        y[j] += d * math.sin(ang)
+        cases.append((_xycs(x, y, a), k * pitch + 0.2))
+    for trial in range(20):
+        n = int(rng.integers(2, 300))
+        L = math.sqrt(n) * rng.uniform(1.0, 2.5)
+        x, y = rng.uniform(0.5, L - 0.5, n), rng.uniform(0.5, L - 0.5, n)
+        cases.append((_xycs(x, y, rng.uniform(0, math.pi / 2, n)), L))
+    x = np.array([0.2, 3.0, 5.5]); y = np.array([0.5, 3.0, 5.5])
+    cases.append((_xycs(x, y, np.zeros(3)), 6.0))  # first square through the wall
+    for nn in (5, 10, 11, 17, 50):
+        raw = json.loads((ROOT / "tests/golden/raw" / f"raw{nn}.json").read_text())
+        q = np.asarray(raw["q"], dtype=np.float64)
+        cases.append((_xycs(q[:, 0], q[:, 1], q[:, 2]), float(raw["target_L"])))
+    zeros = nonzero = 0
+    for k, (xycs, L) in enumerate(cases):
+        say("zero64", k, xycs.shape[0], L)
+        z, e = sis.dev_zero64(xycs, L)
+        assert z == (e == 0.0), (k, z, e)
+        zeros += z
+        nonzero += not z
+    assert zeros >= 3 and nonzero >= 10, (zeros, nonzero)
 
 
 def test_replicas_per_group_do_not_change_results(sis):
