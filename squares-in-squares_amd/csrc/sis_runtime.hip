@@ -280,11 +280,12 @@ void Batch::release() {
         if (p) (void)hipFree(p);
     d_inst = nullptr; d_tiles = nullptr; d_reps = nullptr; d_slots = nullptr; d_ros = nullptr;
     d_state = nullptr; d_arch = nullptr; d_jobs = nullptr; d_fout = nullptr; d_qout = nullptr;
-    for (hipEvent_t e : {ev0, ev1, evs0, evs1})
+    for (hipEvent_t e : {ev0, ev1, evs0, evs1, ev_fork, ev_join})
         if (e) (void)hipEventDestroy(e);
-    ev0 = ev1 = evs0 = evs1 = nullptr;
+    ev0 = ev1 = evs0 = evs1 = ev_fork = ev_join = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
+    if (stream2) (void)hipStreamDestroy(stream2);
+    stream = stream2 = nullptr;
 }
 
 int Batch::build(std::vector<InstSpec>&& in) {
@@ -355,6 +356,9 @@ int Batch::build(std::vector<InstSpec>&& in) {
     }
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHECK(hipEventCreate(&ev0));
     HIPCHECK(hipEventCreate(&ev1));
     HIPCHECK(hipEventCreate(&evs0));
@@ -381,16 +385,28 @@ int Batch::build(std::vector<InstSpec>&& in) {
     return SIS_OK;
 }
 
-// The sweep launches of one interval: four-wavefront tiles, then one-wavefront
-// tiles.
+// The sweep launches of one interval: four-wavefront tiles on the batch
+// stream and, when there are both kinds, one-wavefront tiles on a second
+// stream forked from it and joined back, so each launch fills the compute
+// units the other's last workgroups leave idle (the two touch disjoint
+// replicas).  Captured into the batch's graph as a fork and a join.
 int Batch::launch_tiles() {
     const int nt = (int)h_tiles.size();
+    const bool both = n_big > 0 && nt > n_big;
+    if (both) {
+        HIPCHECK(hipEventRecord(ev_fork, stream));
+        HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+    }
     if (n_big > 0)
         HIPCHECK(launch_sweep(stream, n_big, lds_big, d_tiles, d_inst, d_reps, d_slots, d_state,
                               kSweepThreadsBig));
     if (nt > n_big)
-        HIPCHECK(launch_sweep(stream, nt - n_big, lds_small, d_tiles + n_big, d_inst, d_reps, d_slots,
-                              d_state, kSweepThreads));
+        HIPCHECK(launch_sweep(both ? stream2 : stream, nt - n_big, lds_small, d_tiles + n_big, d_inst, d_reps,
+                              d_slots, d_state, kSweepThreads));
+    if (both) {
+        HIPCHECK(hipEventRecord(ev_join, stream2));
+        HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+    }
     return SIS_OK;
 }
 
