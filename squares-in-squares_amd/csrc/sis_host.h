@@ -68,6 +68,7 @@ public:
     int graph_intervals = 0;
 
     InstDesc* d_inst = nullptr;
+    InstDesc* h_pin = nullptr;  // pinned copy of the descriptors
     Tile* d_tiles = nullptr;
     RepState* d_reps = nullptr;
     SlotState* d_slots = nullptr;

@@ -278,6 +278,8 @@ void Batch::release() {
     for (void* p : {(void*)d_inst, (void*)d_tiles, (void*)d_reps, (void*)d_slots, (void*)d_ros,
                     (void*)d_state, (void*)d_arch, (void*)d_jobs, (void*)d_fout, (void*)d_qout})
         if (p) (void)hipFree(p);
+    if (h_pin) (void)hipHostFree(h_pin);
+    h_pin = nullptr;
     d_inst = nullptr; d_tiles = nullptr; d_reps = nullptr; d_slots = nullptr; d_ros = nullptr;
     d_state = nullptr; d_arch = nullptr; d_jobs = nullptr; d_fout = nullptr; d_qout = nullptr;
     for (hipEvent_t e : {ev0, ev1, evs0, evs1, ev_fork, ev_join})
@@ -357,6 +359,9 @@ int Batch::build(std::vector<InstSpec>&& in) {
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    // pinned mirror of the descriptors: the plain schedule reads them back
+    // after every interval
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(InstDesc) * ni));
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHECK(hipEventCreate(&ev0));
@@ -427,8 +432,7 @@ int Batch::run_plain(long long intervals) {
         if (int rc = launch_tiles()) return rc;
         HIPCHECK(hipEventRecord(evs1, stream));
         HIPCHECK(launch_ladder(stream, (int)specs.size(), lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
-        HIPCHECK(hipMemcpyAsync(h_inst.data(), d_inst, sizeof(InstDesc) * specs.size(),
-                                hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * specs.size(), hipMemcpyDeviceToHost, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         float sms = 0;
         HIPCHECK(hipEventElapsedTime(&sms, evs0, evs1));
@@ -437,6 +441,7 @@ int Batch::run_plain(long long intervals) {
     }
     HIPCHECK(hipEventRecord(ev1, stream));
     HIPCHECK(hipEventSynchronize(ev1));
+    std::copy(h_pin, h_pin + specs.size(), h_inst.begin());
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
     device_ms += ms;
