@@ -110,23 +110,25 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
     return b;
 }
 
-// Dynamic LDS of the exchange kernel for instances of up to n squares and
-// box side L: FP64 x, y, cos, sin of the replica under the zero-overlap test,
-// then that test's uniform grid (cells of width kZeroCell > sqrt(2.000001)):
-// cell starts, scan partials, a flag and the square indices in cell order.
+// Dynamic LDS of the exchange kernel for an instance of n squares, box side
+// L and R replicas: first the ladder by slot, then, reusing it, FP64 x, y,
+// cos, sin of the replica under the zero-overlap test and that test's uniform
+// grid (cells of width kZeroCell > sqrt(2.000001)): cell starts, scan
+// partials, a flag and the square indices in cell order.
 constexpr double kZeroCell = 1.5;
 constexpr int kZeroGridMax = 45;         // cells per side for L <= 64
 __host__ __device__ inline int zero_grid_side(double L) {
     const int m = (int)(L / kZeroCell) + 2;
     return m < kZeroGridMax ? m : kZeroGridMax;
 }
-__host__ __device__ inline size_t ladder_lds_bytes(int n, double L) {
+__host__ __device__ inline size_t ladder_lds_bytes(int n, double L, int R) {
     const size_t mc = (size_t)zero_grid_side(L);
     size_t b = 32 * (size_t)n;               // x, y, cos, sin (FP64)
     b += 4 * (mc * mc + 1);                  // cell starts
     b += 4 * (size_t)kLadderThreads + 4;     // scan partials, flag
     b += 2 * (size_t)n;                      // square indices in cell order
-    return (b + 15) & ~(size_t)15;
+    const size_t ladder = 12 * (size_t)R;    // before the test: energy, T, replica per slot
+    return ((b > ladder ? b : ladder) + 15) & ~(size_t)15;
 }
 
 __host__ __device__ inline int grid_cells(float L, float w) {

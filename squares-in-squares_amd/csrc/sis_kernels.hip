@@ -1105,6 +1105,7 @@ __device__ int block_excl_scan(int v, int* part) {
 // in the same or adjacent cells of a uniform grid of width 1.5 can be
 // nonzero; each such pair i < j is evaluated with the arguments total64
 // passes.  O(n) instead of the O(n^2) of total64.
+constexpr int kZeroAllPairsMaxN = 96;  // below this the test visits all pairs
 struct ZeroGrid {
     int* start;       // zero_grid_side(L)^2 + 1
     int* part;        // kLadderThreads
@@ -1133,6 +1134,18 @@ template <int NT>
 __device__ bool overlap_free64(const double* x, const double* y, const double* c, const double* s, int n,
                                double L, const ZeroGrid& zg) {
     const int tid = threadIdx.x;
+    if (n <= kZeroAllPairsMaxN) {
+        // few squares: every pair i < j directly, with the arguments total64
+        // uses, spread evenly over the threads (flat index over the n x n
+        // grid) and without the grid's passes and barriers
+        bool bad = false;
+        for (int i = tid; i < n; i += NT) bad |= wall64(x[i], y[i], c[i], s[i], L) > 0.0;
+        for (int p = tid; p < n * n; p += NT) {
+            const int i = p / n, j = p - i * n;
+            if (j > i) bad |= pair64(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]) > 0.0;
+        }
+        return __syncthreads_or(bad) == 0;
+    }
     const int mc = zero_grid_side(L);
     const int ncell = mc * mc;
     auto cell = [&](double v) {
@@ -1198,30 +1211,50 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     const int tid = threadIdx.x;
     const int R = D->R, n = D->n, lad = D->ladder, base = D->rep_off;
 
+    // The ladder of the instance in LDS, by temperature slot: energy and
+    // temperature of the slot, replica holding it.  One round of global loads
+    // (replica records, then each replica's slot record) instead of a chain
+    // per phase.
+    float* eS = reinterpret_cast<float*>(lsm);
+    float* tS = eS + R;
+    int* rS = reinterpret_cast<int*>(tS + R);
+    const float target = D->target_acc;
+
     // 1. step adaptation per temperature slot, from the replica that held it
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
         const float rate = (float)rs.acc / (float)((long long)n * ran);
-        const float f = expf((rate - D->target_acc) * 0.5f);
+        const float f = expf((rate - target) * 0.5f);
         SlotState* sl = slots + base + rs.slot;
-        sl->step_xy = fminf(fmaxf(sl->step_xy * f, 1.7320508e-7f), 0.8660254f);
-        sl->step_th = fminf(fmaxf(sl->step_th * f, 1.7320508e-7f), 0.7794229f);
+        const SlotState ss = *sl;
+        sl->step_xy = fminf(fmaxf(ss.step_xy * f, 1.7320508e-7f), 0.8660254f);
+        sl->step_th = fminf(fmaxf(ss.step_th * f, 1.7320508e-7f), 0.7794229f);
+        eS[rs.slot] = rs.E;
+        tS[rs.slot] = ss.T;
+        rS[rs.slot] = r;
     }
     __syncthreads();
 
-    // 2. even/odd exchange of adjacent temperatures inside each ladder
-    const int parity = (int)(D->interval & 1u);
+    // 2. even/odd exchange of adjacent temperatures inside each ladder (the
+    //    pairs are disjoint, so each thread owns the slots it swaps)
+    const uint32_t interval = D->interval, key0 = D->key0, key1 = D->key1;
+    const int parity = (int)(interval & 1u);
     unsigned long long tried = 0, accd = 0;
     for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
         const int s = 2 * p + parity;
         if ((s % lad) + 1 >= lad) continue;
-        const int ra = rep_of_slot[base + s], rb = rep_of_slot[base + s + 1];
-        const float Ta = slots[base + s].T, Tb = slots[base + s + 1].T;
-        const uint4 o = philox((uint32_t)s, 0u, D->interval, 2u, D->key0, D->key1);
+        const int ra = rS[s], rb = rS[s + 1];
+        const float Ta = tS[s], Tb = tS[s + 1];
+        const float Ea = eS[s], Eb = eS[s + 1];
+        const uint4 o = philox((uint32_t)s, 0u, interval, 2u, key0, key1);
         const float lhs = logf(u01(o.x));
-        const float rhs = (1.0f / Ta - 1.0f / Tb) * (reps[base + ra].E - reps[base + rb].E);
+        const float rhs = (1.0f / Ta - 1.0f / Tb) * (Ea - Eb);
         ++tried;
         if (lhs < rhs) {
+            rS[s] = rb;
+            rS[s + 1] = ra;
+            eS[s] = Eb;
+            eS[s + 1] = Ea;
             rep_of_slot[base + s] = rb;
             rep_of_slot[base + s + 1] = ra;
             reps[base + ra].slot = s + 1;
@@ -1240,12 +1273,13 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     const int had_arch = D->arch_valid;
     const double L_arch = D->L_arch;
     for (int s = tid; s < R; s += kLadderThreads)
-        if (reps[base + rep_of_slot[base + s]].E == 0.0f) atomicMin(&zero_slot, s);
+        if (eS[s] == 0.0f) atomicMin(&zero_slot, s);
     __syncthreads();
     double scale = 1.0, Ln = L0;
     bool archive = false;
     if (zero_slot < R) {
-        const int r = rep_of_slot[base + zero_slot];
+        const int r = rS[zero_slot];
+        __syncthreads();  // the ladder's LDS is reused below
         const float4* src = state + D->state_off + (size_t)r * n;
         const LadderLds t = carve_ladder(lsm, n, L0);
         for (int i = tid; i < n; i += kLadderThreads) {
@@ -1497,7 +1531,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_zero64(int n, double L, cons
 }
 
 hipError_t launch_zero64(int n, double L, const float4* q, int* zero, double* e64) {
-    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kLadderThreads), ladder_lds_bytes(n, L), nullptr, n, L, q, zero,
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kLadderThreads), ladder_lds_bytes(n, L, 0), nullptr, n, L, q, zero,
                        e64);
     return hipGetLastError();
 }

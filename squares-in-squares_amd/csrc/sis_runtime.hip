@@ -321,7 +321,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         narch += s.n;
         size_t& lds = s.n >= kBigN ? lds_big : lds_small;
         lds = std::max(lds, pl.lds);
-        lds_ladder = std::max(lds_ladder, ladder_lds_bytes(s.n, s.L));
+        lds_ladder = std::max(lds_ladder, ladder_lds_bytes(s.n, s.L, s.p.replicas));
         const long long iv = (s.p.sweeps + pl.spe - 1) / pl.spe;
         max_intervals = std::max(max_intervals, iv);
         for (int r0 = 0; r0 < s.p.replicas; r0 += pl.G)

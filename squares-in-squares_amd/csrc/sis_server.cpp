@@ -316,7 +316,7 @@ int Server::launch_round(Lane& ln) {
             r->plan.G = pl.G;  // grouping never changes a replica's arithmetic
             size_t& lds = r->spec.n >= kBigN ? ln.lds_big : ln.lds_small;
             lds = std::max(lds, pl.lds);
-            ln.lds_ladder = std::max(ln.lds_ladder, ladder_lds_bytes(r->spec.n, r->spec.L));
+            ln.lds_ladder = std::max(ln.lds_ladder, ladder_lds_bytes(r->spec.n, r->spec.L, r->spec.p.replicas));
             for (int r0 = 0; r0 < r->spec.p.replicas; r0 += pl.G)
                 ln.tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), 0});
             ln.list.push_back(r->slot);
