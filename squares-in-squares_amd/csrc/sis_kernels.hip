@@ -421,9 +421,12 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     Move mv;
     float u;
     if (t.prop) {
-        const float4 a = t.prop[q];
+        float4 a = t.prop[q];
+        float2 b = t.pw[q];
+        // both staged records in one round trip: without this the compiler
+        // reads the flag first and the rest only after the test
+        asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y));
         if (a.z == kNoMove) return;
-        const float2 b = t.pw[q];
         mv = Move{a.x, a.y, a.z, a.w, b.x};
         u = b.y;
     } else {
@@ -892,9 +895,10 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             // sparse levels give each mover 2, 4, 8 or 16 lanes (aligned
             // inside one row of 16 lanes of one wavefront)
             const int cnt = b - a;
-            const int lpi = cnt > NT / 2 ? 1 : cnt > NT / 4 ? 2 : cnt > NT / 8 ? 4 : cnt > NT / 16 ? 8 : 16;
-            const int per = NT / lpi;
-            const int slot = tid / lpi, sub = tid - slot * lpi;
+            const int lsh = cnt > NT / 2 ? 0 : cnt > NT / 4 ? 1 : cnt > NT / 8 ? 2 : cnt > NT / 16 ? 3 : 4;
+            const int lpi = 1 << lsh;  // a power of two: shifts, no division
+            const int per = NT >> lsh;
+            const int slot = tid >> lsh, sub = tid & (lpi - 1);
             for (int base = a; base < b; base += per)
                 if (base + slot < b)
                     propose(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1,
