@@ -381,7 +381,9 @@ def main():
         "data": "synthetic (seeded uniform random initial packings)",
         "config": {
             "workload": "plain PT-SA on GPU: n=50, 256 replicas, geometric ladder 1e-5..5e-2, sigma 0.05/0.05 "
-                        "adapted to 30% acceptance, exchange every 10 sweeps, Philox seed 7, FP64 polish at end",
+                        "adapted to 30% acceptance, exchange every 10 sweeps, Philox seed 7, FP64 polish at end; "
+                        "interaction cells of side sqrt(2) realised as 5x5 stencils of colour cells of side "
+                        "0.7075",
             "n": N, "replicas": REPLICAS, "L0": L0, "sweeps_per_step": SWEEPS_PER_STEP,
             "sweeps_timed": a.steps * SWEEPS_PER_STEP, "schedule": "plain (host sync per interval)",
             "device": dev, "compute_units": cus,

@@ -437,7 +437,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     }
     const float nx = mv.nx, ny = mv.ny, nc = mv.nc, ns = mv.ns, wdiff = mv.wdiff;
     const int last = sc.total - 1;  // >= 0: the mover is in its own cell
-    unsigned cnt = 0, near = 0;
+    unsigned near = 0;
     // Energy change against the square at cell-list position k.  The mover
     // itself and positions past the end of the stencil contribute an exact
     // +0 (no partial sum can hold -0, so this equals skipping them).
@@ -447,10 +447,10 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
         const float eo = depth2(dxo, dyo, p.z, p.w, v.z, v.w);
         const bool other = valid && k != self;
-        cnt += other;
-        near += other ? (__builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f) +
-                            (__builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f)
-                      : 0u;
+        // near pairs: two compare masks combined with `other` on the scalar
+        // unit, two conditional increments
+        near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
+        near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
         return other ? en - eo : 0.0f;
     };
     // Four candidates at once: their positions are read from the cell-list
@@ -533,7 +533,9 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
     }
-    cC += cnt;
+    // every stencil position but the mover's own is a candidate: `last` of
+    // them per proposal, counted once by the mover's first lane
+    cC += sub == 0 ? (unsigned)last : 0u;
     cP += near;
     const float dE = wdiff + sum;
     if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * t.ik[g]))) {
