@@ -319,8 +319,8 @@ def main():
     ap.add_argument("--precision-sweeps", type=int, default=10000)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(instances shard across GPUs with no inter-GPU traffic; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (requests shard across GPUs "
+              "by instance, with no inter-GPU traffic during the search)", file=sys.stderr)
         sys.exit(2)
 
     if a.copy_calibration:

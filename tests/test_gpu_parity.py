@@ -300,7 +300,7 @@ def test_polish_and_certified_side_match_oracle(sis, oracle):
 
 
 def test_solve_n11_certifies(sis, oracle, ref_certifier):
-    """End to end (BASELINE config 1 shape on the GPU): the exported packing is
+    """End to end (the n=11, 8-replica CPU workload's shape on the GPU): the exported packing is
     overlap-free in FP64 and passes the reference's interval certifier after
     the dilation ladder of cuda/demo.py:31."""
     from refcert import certify_q
