@@ -28,6 +28,7 @@ struct InstSpec {
 
 struct Plan {
     int G = 1, M = 0, spe = 1, ladder = 1, ntiles = 0;
+    int wide = 0;               // tiles stage proposals up to kMaxN squares
     size_t lds = 0;
 };
 

@@ -61,7 +61,8 @@ struct CollWork {
 };
 
 struct Tile {
-    int inst, r0, g, pad;
+    int inst, r0, g;
+    int wide;                   // stage proposals up to kMaxN squares (sweep_prop_in_lds)
 };
 
 struct RepState {               // per global replica
@@ -85,9 +86,15 @@ __host__ __device__ inline int grid_words(int M) { return (M * M + 2 + 7) & ~7; 
 // position and orientation, wall-energy change) into LDS in one pass over all
 // lanes before the first level, so the sparse levels only evaluate the
 // stencil; large tiles move one square per lane and propose inline, keeping
-// their LDS, and with it the tiles resident per compute unit, smaller.
+// their LDS, and with it the tiles resident per compute unit, smaller.  When
+// the launch leaves compute units to spare (`wide`, one replica per tile) the
+// residency does not matter and tiles up to kMaxN squares stage too (n = 1000
+// alone: 0.351 vs 0.382 ms per 10-sweep launch; under load staging them costs
+// 10-23 % of the throughput).
 constexpr int kPropLdsMaxSquares = 512;
-__host__ __device__ inline bool sweep_prop_in_lds(int gn) { return gn <= kPropLdsMaxSquares; }
+__host__ __device__ inline bool sweep_prop_in_lds(int gn, bool wide) {
+    return gn <= (wide ? kMaxN : kPropLdsMaxSquares);
+}
 // Tiles up to this size also keep their positions in cell-list order (one
 // LDS read per candidate instead of an index read and a position read).
 // Larger tiles skip the copy so that more of them fit a compute unit
@@ -95,10 +102,10 @@ __host__ __device__ inline bool sweep_prop_in_lds(int gn) { return gn <= kPropLd
 // without; n=256..600 unchanged).
 constexpr int kSposLdsMaxSquares = 768;
 __host__ __device__ inline bool sweep_spos_in_lds(int gn) { return gn <= kSposLdsMaxSquares; }
-__host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G) {
+__host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G, bool wide) {
     size_t gn = (size_t)G * n;
     size_t b = 16 * gn;                         // float4 x, y, cos, sin
-    if (sweep_prop_in_lds((int)gn)) b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
+    if (sweep_prop_in_lds((int)gn, wide)) b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
     if (sweep_spos_in_lds((int)gn)) b += 16 * gn;  // float4 positions in cell-list order
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
     b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot

@@ -235,7 +235,7 @@ __device__ __forceinline__ int rep_of(const TileLds& t, int q) {
     return (int)__umulhi((unsigned)q << 6, t.divn);
 }
 
-__device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
+__device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool wide) {
     TileLds t;
     const int gn = G * n;
     t.divn = (67108864u + (unsigned)n - 1u) / (unsigned)n;
@@ -243,7 +243,7 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G) {
     float4* next = t.pos + gn;
     t.prop = nullptr;
     t.pw = nullptr;
-    if (sweep_prop_in_lds(gn)) {
+    if (sweep_prop_in_lds(gn, wide)) {
         t.prop = next;
         next += gn;
     }
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
     const int ncoll = D->collective, kick_every = D->kick_every;
     const uint32_t interval = D->interval;
-    TileLds t = carve(smem, n, M, G);
+    TileLds t = carve(smem, n, M, G, tl.wide != 0);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
     for (int q = tid; q < G * n; q += NT) {

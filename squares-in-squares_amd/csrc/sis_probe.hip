@@ -70,8 +70,11 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
     if (M > 255) return fail(SIS_EINVAL, "cell grid too wide");
     int G = group > 0 ? std::min(group, R) : 1;
     const size_t hard = device_info().lds_per_block;
-    while (G > 1 && sweep_lds_bytes(n, M, G) > hard) --G;
-    const size_t lds = sweep_lds_bytes(n, M, G);
+    // one replica per tile stages proposals up to kMaxN squares, as a launch
+    // with compute units to spare does
+    const bool wide = G == 1;
+    while (G > 1 && sweep_lds_bytes(n, M, G, false) > hard) --G;
+    const size_t lds = sweep_lds_bytes(n, M, G, wide);
     if (lds > hard) return fail(SIS_EINVAL, "replica does not fit LDS");
     for (size_t k = 0; k < (size_t)n * R; ++k)
         if (!(xycs[4 * k] >= 0.0f && xycs[4 * k] <= L && xycs[4 * k + 1] >= 0.0f && xycs[4 * k + 1] <= L))
@@ -82,7 +85,7 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
     D.L = L; D.L_arch = L; D.Lf = L; D.scale = 1.0f; D.w = cell; D.key0 = key0; D.key1 = key1;
     D.collective = collective; D.kick_every = kick_every; D.interval = interval;
     std::vector<Tile> tiles;
-    for (int r0 = 0; r0 < R; r0 += G) tiles.push_back(Tile{0, r0, std::min(G, R - r0), 0});
+    for (int r0 = 0; r0 < R; r0 += G) tiles.push_back(Tile{0, r0, std::min(G, R - r0), wide ? 1 : 0});
     std::vector<RepState> reps(R);
     std::vector<SlotState> slots(R);
     for (int r = 0; r < R; ++r) {

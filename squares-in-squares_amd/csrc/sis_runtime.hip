@@ -248,6 +248,7 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
         // n=700: 14.71, 13.29; n=1000: 17.23, 15.43).
         const int cus = std::max(1, dev.cus);
         G = 1;
+        pl.wide = (long long)batch_replicas < 8ll * cus;
         if ((long long)batch_replicas >= 8ll * cus) {
             if (big)
                 G = s.n < 600 ? std::min(4, std::max(2, 512 / s.n)) : 1;
@@ -256,11 +257,11 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
         }
     }
     G = std::min(G, s.p.replicas);
-    while (G > 1 && (sweep_lds_bytes(s.n, pl.M, G) > budget || G * s.n > 65535)) --G;
-    if (sweep_lds_bytes(s.n, pl.M, G) > hard)
+    while (G > 1 && (sweep_lds_bytes(s.n, pl.M, G, pl.wide) > budget || G * s.n > 65535)) --G;
+    if (sweep_lds_bytes(s.n, pl.M, G, pl.wide) > hard)
         return fail(SIS_EINVAL, "one replica does not fit the LDS of a workgroup");
     pl.G = G;
-    pl.lds = sweep_lds_bytes(s.n, pl.M, G);
+    pl.lds = sweep_lds_bytes(s.n, pl.M, G, pl.wide);
     pl.ntiles = (s.p.replicas + G - 1) / G;
     return SIS_OK;
 }
@@ -325,7 +326,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         const long long iv = (s.p.sweeps + pl.spe - 1) / pl.spe;
         max_intervals = std::max(max_intervals, iv);
         for (int r0 = 0; r0 < s.p.replicas; r0 += pl.G)
-            h_tiles.push_back(Tile{k, r0, std::min(pl.G, s.p.replicas - r0), 0});
+            h_tiles.push_back(Tile{k, r0, std::min(pl.G, s.p.replicas - r0), pl.wide});
     }
     if (nsq > (size_t)INT32_MAX) return fail(SIS_EINVAL, "batch too large");
     n_big = order_tiles(h_tiles, [&](int k) { return specs[k].n; });

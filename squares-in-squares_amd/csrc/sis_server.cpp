@@ -318,7 +318,7 @@ int Server::launch_round(Lane& ln) {
             lds = std::max(lds, pl.lds);
             ln.lds_ladder = std::max(ln.lds_ladder, ladder_lds_bytes(r->spec.n, r->spec.L, r->spec.p.replicas));
             for (int r0 = 0; r0 < r->spec.p.replicas; r0 += pl.G)
-                ln.tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), 0});
+                ln.tiles.push_back(Tile{r->slot, r0, std::min(pl.G, r->spec.p.replicas - r0), pl.wide});
             ln.list.push_back(r->slot);
         }
         ln.n_big = order_tiles(ln.tiles, [&](int slot) { return h_inst[slot].n; });
