@@ -407,6 +407,7 @@ __device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4&
 // energy change is four partial sums over the flat candidate index f
 // (f mod 4), each in increasing f, combined as (s0 + s1) + (s2 + s3): the
 // same bits whatever the number of lanes sharing the work.
+template <bool kQuads>
 __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
                                         float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
                                         int lpi, int sub, unsigned long long& cC,
@@ -476,7 +477,39 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     };
     // partial k accumulates f = k (mod 4); one loop per lane count
     float sum, d0, d1, d2, d3;
-    if (lpi == 1) {
+    if (!kQuads && lpi < 8) {
+        // One-wavefront tiles: one candidate per lane and step, f = sub +
+        // lpi j, without the four-candidate batches (their registers and code
+        // buy latency hiding that a lone wavefront per SIMD does not use).
+        // Partial f mod 4 of each step is the same on every lane of a mover
+        // for lpi 1 (all four by j) and 4 (always sub); for lpi 2 it is sub
+        // or sub + 2 by the parity of j.
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        int j = 0;
+        for (int f = sub; f <= last; f += lpi, ++j) {
+            const int k = sc.at(f);
+            const float4 v = t.spos ? t.spos[k] : pg[t.sorted[k]];
+            const float d = delta(v, k, true);
+            const int slot = lpi == 1 ? (j & 3) : lpi == 2 ? 2 * (j & 1) : 0;
+            if (slot == 0)
+                s0 += d;
+            else if (slot == 1)
+                s1 += d;
+            else if (slot == 2)
+                s2 += d;
+            else
+                s3 += d;
+        }
+        if (lpi == 1) {
+            sum = (s0 + s1) + (s2 + s3);
+        } else if (lpi == 2) {
+            // s0: partial sub, s2: partial sub + 2
+            sum = (s0 + quad_xor1(s0)) + (s2 + quad_xor1(s2));
+        } else {
+            const float h = s0 + quad_xor1(s0);
+            sum = h + quad_xor2(h);
+        }
+    } else if (lpi == 1) {
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
         for (int f = 0; f <= last; f += 4) {
             quad(f, f + 1, f + 2, f + 3, d0, d1, d2, d3);
@@ -904,7 +937,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             const int slot = tid >> lsh, sub = tid & (lpi - 1);
             for (int base = a; base < b; base += per)
                 if (base + slot < b)
-                    propose(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1,
+                    propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1,
                             lpi, sub, cC, cP);
             tile_sync<NT>();
         }
@@ -912,7 +945,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             if (tid == 0)
                 for (int q = 0; q < G * n; ++q)
                     if (t.rank[q] == kGroups)
-                        propose(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
+                        propose<(NT > 64)>(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
             tile_sync<NT>();
         }
         float* tmp = t.shx;  // the shift drawn during this sweep becomes current
