@@ -305,8 +305,11 @@ int Server::launch_round(Lane& ln) {
         ln.tiles.clear();
         ln.list.clear();
         ln.lds_big = ln.lds_small = ln.lds_ladder = 0;
+        // the lanes share the device: tiles are sized by the replicas resident
+        // in all of them
         long long total_reps = 0;
-        for (Req* r : ln.active) total_reps += r->spec.p.replicas;
+        for (const Lane& other : lanes)
+            for (const Req* r : other.active) total_reps += r->spec.p.replicas;
         for (Req* r : ln.active) {
             Plan pl;
             if (plan_instance(r->spec, (int)std::min<long long>(total_reps, 1 << 30), pl)) {
