@@ -113,6 +113,9 @@ void portfolio_state(const InstSpec& s, bool jitter, std::vector<double>& q);
 void portfolio_layouts(int n, int R, int ladder, double seed_L, const double* seed, std::vector<double>& out);
 double portfolio_jitter(int cls, double u);
 int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
+// A launch set of this many replicas keeps every compute unit busy (eight per
+// compute unit): below it tiles hold one replica each and stage every size.
+bool device_loaded(long long replicas);
 InstSpec make_spec(const sis_instance* in, const sis_params* p);
 void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);
 const void* sweep_kernel_ptr(int threads);

@@ -222,6 +222,8 @@ void initial_state(const InstSpec& s, std::vector<float4>& out) {
 // ---------------------------------------------------------------------------
 // Tile planning.
 // ---------------------------------------------------------------------------
+bool device_loaded(long long replicas) { return replicas >= 8ll * std::max(1, device_info().cus); }
+
 int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
     const DeviceInfo& dev = device_info();
     const float w = (float)s.p.cell;
@@ -246,10 +248,9 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl) {
         // 7.53; n=150: 6.00, 8.93, 9.99, 8.61; n=200: 7.69, 11.53, 9.77, 9.89;
         // n=300: 9.90, 12.38, 12.09, 11.26; n=500: 14.86, 15.85, 13.63, 14.80;
         // n=700: 14.71, 13.29; n=1000: 17.23, 15.43).
-        const int cus = std::max(1, dev.cus);
         G = 1;
-        pl.wide = (long long)batch_replicas < 8ll * cus;
-        if ((long long)batch_replicas >= 8ll * cus) {
+        pl.wide = !device_loaded(batch_replicas);
+        if (!pl.wide) {
             if (big)
                 G = s.n < 600 ? std::min(4, std::max(2, 512 / s.n)) : 1;
             else

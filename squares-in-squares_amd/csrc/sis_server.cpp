@@ -112,6 +112,7 @@ struct Lane {
     hipEvent_t done_ev = nullptr;
     bool in_flight = false;
     bool dirty = true;
+    bool planned_loaded = false;      // device load the tiles were planned for
     std::vector<Req*> active;         // resident, in list order
     std::vector<Tile> tiles;          // four-wavefront tiles first (n_big of them)
     std::vector<int> list;            // descriptor slots of `active`
@@ -301,15 +302,18 @@ int Server::admit_one(Req* r, Lane& ln) {
 // One round of a lane: intervals of sweep + exchange over its tiles, then the
 // descriptors of its requests into pinned memory, then the lane's event.
 int Server::launch_round(Lane& ln) {
-    if (ln.dirty) {
+    // the lanes share the device: tiles are sized by the replicas resident in
+    // all of them, and re-planned when that load crosses the threshold the
+    // tile rule uses (one replica per tile and staging for every tile size
+    // below it), so the last requests of a drain run as they would alone
+    long long total_reps = 0;
+    for (const Lane& other : lanes)
+        for (const Req* r : other.active) total_reps += r->spec.p.replicas;
+    const bool loaded = device_loaded(total_reps);
+    if (ln.dirty || loaded != ln.planned_loaded) {
         ln.tiles.clear();
         ln.list.clear();
         ln.lds_big = ln.lds_small = ln.lds_ladder = 0;
-        // the lanes share the device: tiles are sized by the replicas resident
-        // in all of them
-        long long total_reps = 0;
-        for (const Lane& other : lanes)
-            for (const Req* r : other.active) total_reps += r->spec.p.replicas;
         for (Req* r : ln.active) {
             Plan pl;
             if (plan_instance(r->spec, (int)std::min<long long>(total_reps, 1 << 30), pl)) {
@@ -331,6 +335,7 @@ int Server::launch_round(Lane& ln) {
                                  hipMemcpyHostToDevice, ln.stream));
         // the host vectors stay unchanged until the lane's next round boundary
         ln.dirty = false;
+        ln.planned_loaded = loaded;
     }
     const int ipr = std::max(1, cfg.intervals_per_round);
     const int nt = (int)ln.tiles.size();
