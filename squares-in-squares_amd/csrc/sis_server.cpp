@@ -5,9 +5,10 @@
 // RAM, cuda/cpp_loop.py:41-45, then the C++ certifier gates,
 // cuda/cpp_loop.py:17-28).  Here every admitted request keeps its replicas
 // resident in one device arena.  Requests are split into size classes
-// ("lanes": n < 64, n < 256, the rest); each lane has its own stream and runs
-// rounds of `intervals_per_round` sweep + exchange steps over the tiles of all
-// its resident requests, so the short launches of small instances are not held
+// (n < 64, n < 256, the rest) served by lanes (one each, two for the last
+// class); each lane has its own stream and runs rounds of
+// `intervals_per_round` sweep + exchange steps over the tiles of all its
+// resident requests, so the short launches of small instances are not held
 // back by the long ones of large instances and the lanes' kernels overlap on
 // the device.  Requests that arrive while others run are admitted at their
 // lane's next round boundary as slots and memory free up.  Finished requests
@@ -103,8 +104,19 @@ size_t request_bytes(int n, int R) {
            (size_t)R * (sizeof(RepState) + sizeof(SlotState) + sizeof(int));
 }
 
-constexpr int kLanes = 3;
-int lane_of(int n) { return n < 64 ? 0 : (n < 256 ? 1 : 2); }
+// Size classes n < 64, 64..255 and n >= 256.  The largest class (most of the
+// work) is served by two lanes that take its requests in turn: each lane is a
+// chain of launches that waits for its own last workgroups, and the two fill
+// each other's tails (served stream at 20/s: 1.42e10 -> 1.52e10 proposals/s,
+// p99 6.0 -> 4.7 s).  More lanes than that (a second mid-size lane, a third
+// large one) lost p99 again (7.2-8.4 s): with the finalize stream they
+// outnumber the device's hardware queues, which then serialise them.
+constexpr int kMidLanes = 1, kBigLanes = 2;
+constexpr int kLanes = 1 + kMidLanes + kBigLanes;
+int size_class(int n) { return n < 64 ? 0 : (n < 256 ? 1 : 2); }
+int lane_of(int cls, uint64_t seq) {
+    return cls == 0 ? 0 : cls == 1 ? 1 + (int)(seq % kMidLanes) : 1 + kMidLanes + (int)(seq % kBigLanes);
+}
 
 // One size class of the loop (loop thread only).
 struct Lane {
@@ -152,6 +164,7 @@ struct Server {
     std::unordered_map<uint64_t, Req*> all;
     std::deque<Req*> cert_queue;
     uint64_t next_ticket = 1;
+    uint64_t class_seq[3] = {0, 0, 0};  // requests submitted per size class (turns of its lanes)
     bool stop = false;
     bool loop_done = false;  // the device loop has exited (set under mu)
     std::thread loop;
@@ -636,7 +649,6 @@ int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params
         delete r;
         return fail(SIS_EINVAL, "request larger than the server arena");
     }
-    r->lane = lane_of(inst->n);
     r->bytes = request_bytes(inst->n, params->replicas);
     initial_state(r->spec, r->init);  // in the caller's thread, off the device loop
     r->t_submit = now_s();
@@ -647,6 +659,8 @@ int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params
             return fail(SIS_EDEVICE, h->s.error.empty() ? "server stopped" : h->s.error);
         }
         r->ticket = h->s.next_ticket++;
+        const int cls = size_class(inst->n);
+        r->lane = lane_of(cls, h->s.class_seq[cls]++);
         h->s.all[r->ticket] = r;
         h->s.queue[r->lane].push_back(r);
         *ticket = r->ticket;
