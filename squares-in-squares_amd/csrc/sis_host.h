@@ -44,6 +44,7 @@ public:
     int build(std::vector<InstSpec>&& in);
     int launch_tiles();
     int enqueue_interval();
+    int enqueue_chain(int c);
     int run(long long intervals, int mode);
     int run_plain(long long intervals);
     int run_graph(int per_graph, long long launches);
@@ -67,6 +68,22 @@ public:
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_intervals = 0;
+
+    // The batched schedule runs the instances as two independent chains of
+    // launches (sweeps + exchange per interval), each on its own streams, so
+    // that one chain's last workgroups and exchange step overlap the other's
+    // sweeps.  Instances are split by work; results do not depend on it.
+    struct Chain {
+        hipStream_t st = nullptr;
+        hipEvent_t done = nullptr;
+        int tile_off = 0, ntiles = 0, n_big = 0;  // into d_ctiles
+        int list_off = 0, ninst = 0;              // into d_clist
+        size_t lds_big = 0, lds_small = 0, lds_ladder = 0;
+    };
+    std::vector<Chain> chains;
+    Tile* d_ctiles = nullptr;
+    int* d_clist = nullptr;
+    hipEvent_t ev_cfork = nullptr;
 
     InstDesc* d_inst = nullptr;
     InstDesc* h_pin = nullptr;  // pinned copy of the descriptors

@@ -282,6 +282,17 @@ void Batch::release() {
         if (p) (void)hipFree(p);
     if (h_pin) (void)hipHostFree(h_pin);
     h_pin = nullptr;
+    for (Chain& c : chains) {
+        if (c.done) (void)hipEventDestroy(c.done);
+        if (c.st) (void)hipStreamDestroy(c.st);
+    }
+    chains.clear();
+    for (void* p : {(void*)d_ctiles, (void*)d_clist})
+        if (p) (void)hipFree(p);
+    d_ctiles = nullptr;
+    d_clist = nullptr;
+    if (ev_cfork) (void)hipEventDestroy(ev_cfork);
+    ev_cfork = nullptr;
     d_inst = nullptr; d_tiles = nullptr; d_reps = nullptr; d_slots = nullptr; d_ros = nullptr;
     d_state = nullptr; d_arch = nullptr; d_jobs = nullptr; d_fout = nullptr; d_qout = nullptr;
     for (hipEvent_t e : {ev0, ev1, evs0, evs1, ev_fork, ev_join})
@@ -388,7 +399,69 @@ int Batch::build(std::vector<InstSpec>&& in) {
         std::copy(h_state.begin() + h_inst[k].state_off,
                   h_state.begin() + h_inst[k].state_off + specs[k].n, h_arch.begin() + h_inst[k].arch_off);
     HIPCHECK(hipMemcpyAsync(d_arch, h_arch.data(), sizeof(float4) * narch, hipMemcpyHostToDevice, stream));
+    if (ni >= 2) {
+        // two chains, instances assigned heaviest first to the lighter one
+        std::vector<int> order(ni);
+        for (int k = 0; k < ni; ++k) order[k] = k;
+        auto work = [&](int k) { return (double)specs[k].n * specs[k].p.replicas * (double)specs[k].p.sweeps; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        std::vector<int> members[2];
+        double load[2] = {0, 0};
+        for (int k : order) {
+            const int c = load[0] <= load[1] ? 0 : 1;
+            members[c].push_back(k);
+            load[c] += work(k);
+        }
+        std::vector<Tile> ctiles;
+        std::vector<int> clist;
+        chains.resize(2);
+        for (int c = 0; c < 2; ++c) {
+            Chain& ch = chains[c];
+            std::vector<Tile> t;
+            for (int k : members[c]) {
+                const Plan& pl = plans[k];
+                for (int r0 = 0; r0 < specs[k].p.replicas; r0 += pl.G)
+                    t.push_back(Tile{k, r0, std::min(pl.G, specs[k].p.replicas - r0), pl.wide});
+                size_t& lds = specs[k].n >= kBigN ? ch.lds_big : ch.lds_small;
+                lds = std::max(lds, pl.lds);
+                ch.lds_ladder = std::max(ch.lds_ladder, ladder_lds_bytes(specs[k].n, specs[k].L, specs[k].p.replicas));
+            }
+            ch.n_big = order_tiles(t, [&](int k) { return specs[k].n; });
+            ch.tile_off = (int)ctiles.size();
+            ch.ntiles = (int)t.size();
+            ctiles.insert(ctiles.end(), t.begin(), t.end());
+            ch.list_off = (int)clist.size();
+            ch.ninst = (int)members[c].size();
+            clist.insert(clist.end(), members[c].begin(), members[c].end());
+            HIPCHECK(hipStreamCreateWithFlags(&ch.st, hipStreamNonBlocking));
+            HIPCHECK(hipEventCreateWithFlags(&ch.done, hipEventDisableTiming));
+        }
+        HIPCHECK(hipEventCreateWithFlags(&ev_cfork, hipEventDisableTiming));
+        HIPCHECK(hipMalloc(&d_ctiles, sizeof(Tile) * ctiles.size()));
+        HIPCHECK(hipMalloc(&d_clist, sizeof(int) * clist.size()));
+        HIPCHECK(hipMemcpyAsync(d_ctiles, ctiles.data(), sizeof(Tile) * ctiles.size(), hipMemcpyHostToDevice,
+                                stream));
+        HIPCHECK(hipMemcpyAsync(d_clist, clist.data(), sizeof(int) * clist.size(), hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipStreamSynchronize(stream));  // the host vectors go out of scope
+    }
     HIPCHECK(hipStreamSynchronize(stream));
+    return SIS_OK;
+}
+
+// One interval of one chain on the chain's stream: its sweep launches
+// (four-wavefront tiles, then one-wavefront tiles), then the exchange step of
+// its instances.  (Forking the two shapes of a chain as well measured slower:
+// four streams, more than the device's hardware queues hold apart.)
+int Batch::enqueue_chain(int ci) {
+    Chain& c = chains[ci];
+    const Tile* tiles = d_ctiles + c.tile_off;
+    if (c.n_big > 0)
+        HIPCHECK(launch_sweep(c.st, c.n_big, c.lds_big, tiles, d_inst, d_reps, d_slots, d_state, kSweepThreadsBig));
+    if (c.ntiles > c.n_big)
+        HIPCHECK(launch_sweep(c.st, c.ntiles - c.n_big, c.lds_small, tiles + c.n_big, d_inst, d_reps, d_slots,
+                              d_state, kSweepThreads));
+    HIPCHECK(launch_ladder(c.st, c.ninst, c.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
+                           d_clist + c.list_off));
     return SIS_OK;
 }
 
@@ -489,14 +562,27 @@ int Batch::run_graph(int per_graph, long long launches) {
         graph_exec = nullptr;
         graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < per_graph; ++k) {
-            int rc = enqueue_interval();
-            if (rc) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(stream, &g);
-                if (g) (void)hipGraphDestroy(g);
-                return rc;
-            }
+        int rc = SIS_OK;
+        if (chains.empty()) {
+            for (int k = 0; k < per_graph && rc == SIS_OK; ++k) rc = enqueue_interval();
+        } else {
+            // the chains fork from the batch stream and join it at the end
+            if (hipEventRecord(ev_cfork, stream) != hipSuccess) rc = fail(SIS_EDEVICE, "graph fork failed");
+            for (Chain& c : chains)
+                if (rc == SIS_OK && hipStreamWaitEvent(c.st, ev_cfork, 0) != hipSuccess)
+                    rc = fail(SIS_EDEVICE, "graph fork failed");
+            for (int k = 0; k < per_graph && rc == SIS_OK; ++k)
+                for (int ci = 0; ci < (int)chains.size() && rc == SIS_OK; ++ci) rc = enqueue_chain(ci);
+            for (Chain& c : chains)
+                if (rc == SIS_OK && (hipEventRecord(c.done, c.st) != hipSuccess ||
+                                     hipStreamWaitEvent(stream, c.done, 0) != hipSuccess))
+                    rc = fail(SIS_EDEVICE, "graph join failed");
+        }
+        if (rc) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
         }
         HIPCHECK(hipStreamEndCapture(stream, &graph));
         HIPCHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
