@@ -102,10 +102,26 @@ __host__ __device__ inline bool sweep_prop_in_lds(int gn, bool wide) {
 // without; n=256..600 unchanged).
 constexpr int kSposLdsMaxSquares = 768;
 __host__ __device__ inline bool sweep_spos_in_lds(int gn) { return gn <= kSposLdsMaxSquares; }
+// One-wavefront tiles that stage their proposals write them as 80-byte mover
+// records in level order (position, move, wall change, uniform, 1/T, indices
+// and the stencil's row offsets), so that a pass reads everything but the
+// candidates in one LDS round trip.  Under load only tiles of up to 64
+// squares use them: at 80 squares the extra LDS costs residency (n = 80,
+// 4096 replicas: 6.3e9 proposals/s with records, 8.9e9 without; n = 64:
+// 8.2e9 with, 8.0e9 without).
+constexpr int kMoverRecBytes = 80;
+constexpr int kRecLoadedMaxSquares = 64;
+__host__ __device__ inline bool sweep_records(int n, int gn, bool wide) {
+    return n < kBigN && sweep_prop_in_lds(gn, wide) && sweep_spos_in_lds(gn) &&
+           (wide || gn <= kRecLoadedMaxSquares);
+}
 __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G, bool wide) {
     size_t gn = (size_t)G * n;
     size_t b = 16 * gn;                         // float4 x, y, cos, sin
-    if (sweep_prop_in_lds((int)gn, wide)) b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
+    if (sweep_records(n, (int)gn, wide))
+        b += kMoverRecBytes * gn;               // mover records
+    else if (sweep_prop_in_lds((int)gn, wide))
+        b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
     if (sweep_spos_in_lds((int)gn)) b += 16 * gn;  // float4 positions in cell-list order
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
     b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot

@@ -199,6 +199,15 @@ __device__ void wave_scan_u16(uint16_t* a, int K) {
     }
 }
 
+// A mover record: five 16-byte words, read in one LDS round trip.
+//   w0  position at the start of the sweep (x, y, cos, sin)
+//   w1  proposed x, y, cos, sin (cos = kNoMove: rejected before the stencil)
+//   w2  wall-energy change, accept uniform, log2(e)/T, square | list slot << 16
+//   w3  stencil a0 | d1 << 16, d2 | d3 << 16, d4 | c1 << 16, c2 | c3 << 16
+//   w4  stencil c4 | total << 16, tile replica, 0, 0
+constexpr int kRecWords = 5;
+static_assert(kRecWords * 16 == kMoverRecBytes, "mover record size");
+
 struct TileLds {
     float4* pos;
     float4* prop;     // per square: proposed x, y, cos, sin of the sweep (cos = kNoMove:
@@ -206,6 +215,8 @@ struct TileLds {
     float2* pw;       // per square: wall-energy change of the proposal, accept uniform
     float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k];
                       // null when the tile reads positions through `sorted`
+    uint4* rec;       // mover records in level order, kRecWords per item (replace prop
+                      // and pw); null unless sweep_records()
     uint16_t* A;      // per replica grid_words(M) cell starts
     uint16_t* sorted;
     uint16_t* cellk;  // kx | ky << 8
@@ -243,7 +254,9 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool w
     float4* next = t.pos + gn;
     t.prop = nullptr;
     t.pw = nullptr;
-    if (sweep_prop_in_lds(gn, wide)) {
+    t.rec = nullptr;
+    const bool recs = sweep_records(n, gn, wide);
+    if (!recs && sweep_prop_in_lds(gn, wide)) {
         t.prop = next;
         next += gn;
     }
@@ -251,6 +264,10 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool w
     if (sweep_spos_in_lds(gn)) {
         t.spos = next;
         next += gn;
+    }
+    if (recs) {
+        t.rec = reinterpret_cast<uint4*>(next);
+        next += kRecWords * gn;
     }
     if (t.prop) {
         t.pw = reinterpret_cast<float2*>(next);
@@ -369,6 +386,27 @@ struct Stencil {
     // rows 1..K telescope to o_K and empty rows add nothing).  Written as
     // conditional increments rather than a select among the row offsets,
     // which the compiler would turn into an indexed table in scratch memory.
+    Stencil() = default;
+    // the packed form of a mover record (all fields below 2^16)
+    __device__ __forceinline__ void pack(uint4& s0, unsigned& s1x) const {
+        s0 = make_uint4((unsigned)a0 | (unsigned)d1 << 16, (unsigned)d2 | (unsigned)d3 << 16,
+                        (unsigned)d4 | (unsigned)c1 << 16, (unsigned)c2 | (unsigned)c3 << 16);
+        s1x = (unsigned)c4 | (unsigned)total << 16;
+    }
+    __device__ __forceinline__ static Stencil unpack(const uint4& s0, unsigned s1x) {
+        Stencil sc;
+        sc.a0 = (int)(s0.x & 0xFFFFu);
+        sc.d1 = (int)(s0.x >> 16);
+        sc.d2 = (int)(s0.y & 0xFFFFu);
+        sc.d3 = (int)(s0.y >> 16);
+        sc.d4 = (int)(s0.z & 0xFFFFu);
+        sc.c1 = (int)(s0.z >> 16);
+        sc.c2 = (int)(s0.w & 0xFFFFu);
+        sc.c3 = (int)(s0.w >> 16);
+        sc.c4 = (int)(s1x & 0xFFFFu);
+        sc.total = (int)(s1x >> 16);
+        return sc;
+    }
     __device__ __forceinline__ int at(int f) const {
         int off = a0;
         off += f >= c1 ? d1 : 0;
@@ -400,42 +438,18 @@ __device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4&
     return true;
 }
 
-// One Metropolis proposal for square i of tile replica g, computed by `lpi`
-// consecutive lanes (1, 2, 4, 8 or 16; this lane is `sub`).  The move comes
-// from t.prop (staged for the whole sweep before the first level) or is
-// drawn here.  The stencil
-// energy change is four partial sums over the flat candidate index f
-// (f mod 4), each in increasing f, combined as (s0 + s1) + (s2 + s3): the
-// same bits whatever the number of lanes sharing the work.
+// Stencil energy change and Metropolis test of one proposal of tile square q
+// (replica g, cell-list slot `self`, position p, move mv, accept uniform u,
+// log2(e)/T ik), computed by `lpi` consecutive lanes (1, 2, 4, 8 or 16; this
+// lane is `sub`).  The stencil energy change is four partial sums over the
+// flat candidate index f (f mod 4), each in increasing f, combined as
+// (s0 + s1) + (s2 + s3): the same bits whatever the number of lanes sharing
+// the work.
 template <bool kQuads>
-__device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
-                                        float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
-                                        int lpi, int sub, unsigned long long& cC,
-                                        unsigned long long& cP) {
-    const int g = rep_of(t, q);
-    const int gw = grid_words(M);
-    const float4 p = t.pos[q];
-    const int self = t.slotq[q];
-    const int ck = t.cellk[q];
-    const int kx = ck & 255, ky = ck >> 8;
-    // the stencil depends only on the current cell: its loads go out first
-    const Stencil sc(t.A + g * gw, M, kx, ky);
-    Move mv;
-    float u;
-    if (t.prop) {
-        float4 a = t.prop[q];
-        float2 b = t.pw[q];
-        // both staged records in one round trip: without this the compiler
-        // reads the flag first and the rest only after the test
-        asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y));
-        if (a.z == kNoMove) return;
-        mv = Move{a.x, a.y, a.z, a.w, b.x};
-        u = b.y;
-    } else {
-        const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
-        if (!make_move(t, g, p, r, kx, ky, m, inv_w, L, mv)) return;
-        u = r.w;
-    }
+__device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int self, const float4& p,
+                                         const Stencil& sc, const Move& mv, float u, float ik, int n,
+                                         int lpi, int sub, unsigned long long& cC,
+                                         unsigned long long& cP) {
     const float nx = mv.nx, ny = mv.ny, nc = mv.nc, ns = mv.ns, wdiff = mv.wdiff;
     const int last = sc.total - 1;  // >= 0: the mover is in its own cell
     unsigned near = 0;
@@ -571,12 +585,98 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     cC += sub == 0 ? (unsigned)last : 0u;
     cP += near;
     const float dE = wdiff + sum;
-    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * t.ik[g]))) {
+    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * ik))) {
         const float4 np = make_float4(nx, ny, nc, ns);
         t.pos[q] = np;
         if (t.spos) t.spos[self] = np;
         atomicAdd(&t.acc[g], 1);
     }
+}
+
+// One Metropolis proposal for square q of its tile replica: the move comes
+// from t.prop (staged for the whole sweep before the first level) or is drawn
+// here.
+template <bool kQuads>
+__device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
+                                        float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
+                                        int lpi, int sub, unsigned long long& cC,
+                                        unsigned long long& cP) {
+    const int g = rep_of(t, q);
+    const int gw = grid_words(M);
+    const float4 p = t.pos[q];
+    const int self = t.slotq[q];
+    const int ck = t.cellk[q];
+    const int kx = ck & 255, ky = ck >> 8;
+    // the stencil depends only on the current cell: its loads go out first
+    const Stencil sc(t.A + g * gw, M, kx, ky);
+    float ik = t.ik[g];
+    Move mv;
+    float u;
+    if (t.prop) {
+        float4 a = t.prop[q];
+        float2 b = t.pw[q];
+        // the staged records and 1/T in one round trip: without this the
+        // compiler reads the flag first, the rest after the test and 1/T
+        // only once the energy change is known
+        asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(ik));
+        if (a.z == kNoMove) return;
+        mv = Move{a.x, a.y, a.z, a.w, b.x};
+        u = b.y;
+    } else {
+        const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
+        if (!make_move(t, g, p, r, kx, ky, m, inv_w, L, mv)) return;
+        u = r.w;
+    }
+    evaluate<kQuads>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
+}
+
+// Mover record j of a level (one-wavefront tiles): the square's position,
+// staged move, wall change, uniform, 1/T, indices and stencil, written once
+// per sweep by one lane per square when the level items are scattered.
+__device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int n, int M, int m,
+                                             float inv_w, float L, int r0, uint32_t sweep, uint32_t k0,
+                                             uint32_t k1) {
+    const int g = rep_of(t, q), i = q - g * n;
+    const int ck = t.cellk[q];
+    const int kx = ck & 255, ky = ck >> 8;
+    const float4 p = t.pos[q];
+    const Stencil sc(t.A + g * grid_words(M), M, kx, ky);
+    const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
+    Move mv;
+    const bool ok = make_move(t, g, p, r, kx, ky, m, inv_w, L, mv);
+    uint4 s0;
+    unsigned s1x;
+    sc.pack(s0, s1x);
+    uint4* w = t.rec + (size_t)j * kRecWords;
+    w[0] = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), __float_as_uint(p.w));
+    w[1] = ok ? make_uint4(__float_as_uint(mv.nx), __float_as_uint(mv.ny), __float_as_uint(mv.nc),
+                           __float_as_uint(mv.ns))
+              : make_uint4(0u, 0u, __float_as_uint(kNoMove), 0u);
+    w[2] = make_uint4(__float_as_uint(ok ? mv.wdiff : 0.0f), __float_as_uint(r.w), __float_as_uint(t.ik[g]),
+                      (unsigned)q | (unsigned)t.slotq[q] << 16);
+    w[3] = s0;
+    w[4] = make_uint4(s1x, (unsigned)g, 0u, 0u);
+}
+
+// The proposal of mover record j: one LDS round trip for everything but the
+// candidates.
+__device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int lpi, int sub,
+                                            unsigned long long& cC, unsigned long long& cP) {
+    const uint4* w = t.rec + (size_t)j * kRecWords;
+    uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    uint2 w4 = *reinterpret_cast<const uint2*>(w + 4);
+    // all five words in flight before the flag test
+    asm volatile("" : "+v"(w0.x), "+v"(w0.y), "+v"(w0.z), "+v"(w0.w), "+v"(w1.x), "+v"(w1.y), "+v"(w1.z),
+                 "+v"(w1.w), "+v"(w2.x), "+v"(w2.y), "+v"(w2.z), "+v"(w2.w));
+    asm volatile("" : "+v"(w3.x), "+v"(w3.y), "+v"(w3.z), "+v"(w3.w), "+v"(w4.x), "+v"(w4.y));
+    if (__uint_as_float(w1.z) == kNoMove) return;
+    const float4 p = make_float4(__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z),
+                                 __uint_as_float(w0.w));
+    const Move mv{__uint_as_float(w1.x), __uint_as_float(w1.y), __uint_as_float(w1.z), __uint_as_float(w1.w),
+                  __uint_as_float(w2.x)};
+    const Stencil sc = Stencil::unpack(w3, w4.x);
+    evaluate<false>(t, (int)(w2.w & 0xFFFFu), (int)w4.y, (int)(w2.w >> 16), p, sc, mv, __uint_as_float(w2.y),
+                    __uint_as_float(w2.z), n, lpi, sub, cC, cP);
 }
 
 // Canonical FP32 energy of every tile replica: per-square wall + pairs with
@@ -870,7 +970,8 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         // group key: occupancy rank x colour; ranks >= kRankCap go serial.
         // Small tiles stage every square's proposal here (one Philox block
         // per square; a square's own state does not change before its level
-        // runs), while the LDS atomics of the key pass are in flight.
+        // runs), while the LDS atomics of the key pass are in flight; tiles
+        // with mover records draw it when the records are written.
         for (int q = tid; q < G * n; q += NT) {
             const int g = rep_of(t, q), i = q - g * n;
             const int ck = t.cellk[q];
@@ -921,7 +1022,13 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         if (NT > 64) gs = lane <= kGroups + 1 ? t.gstart[lane] : 0;
         for (int q = tid; q < G * n; q += NT) {
             const int lvl = t.rank[q];
-            if (lvl < kGroups) t.items[atomicAdd(&t.gcnt[lvl], 1)] = (uint16_t)q;
+            if (lvl < kGroups) {
+                const int j = atomicAdd(&t.gcnt[lvl], 1);
+                if (NT == 64 && t.rec)
+                    write_record(t, j, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1);
+                else
+                    t.items[j] = (uint16_t)q;
+            }
         }
         tile_sync<NT>();
         for (int lv = 0; lv < kGroups; ++lv) {
@@ -935,10 +1042,17 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             const int lpi = 1 << lsh;  // a power of two: shifts, no division
             const int per = NT >> lsh;
             const int slot = tid >> lsh, sub = tid & (lpi - 1);
-            for (int base = a; base < b; base += per)
-                if (base + slot < b)
-                    propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1,
-                            lpi, sub, cC, cP);
+            for (int base = a; base < b; base += per) {
+                if (base + slot >= b) continue;
+                if constexpr (NT == 64) {
+                    if (t.rec) {
+                        propose_rec(t, base + slot, n, lpi, sub, cC, cP);
+                        continue;
+                    }
+                }
+                propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1, lpi,
+                                   sub, cC, cP);
+            }
             tile_sync<NT>();
         }
         if (__builtin_amdgcn_readlane(gs, kGroups + 1) > __builtin_amdgcn_readlane(gs, kGroups)) {
