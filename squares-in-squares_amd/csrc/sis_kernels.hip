@@ -424,18 +424,22 @@ struct Move {
     float nx, ny, nc, ns, wdiff;
 };
 constexpr float kNoMove = 8.0f;  // cos field of a staged proposal rejected early
-__device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4& p, const float4& r, int kx,
-                                          int ky, int m, float inv_w, float L, Move& mv) {
-    const float sxy = t.sxy[g];
+// (replica step sizes sxy, sth and grid shift shx, shy given)
+__device__ __forceinline__ bool make_move(float sxy, float sth, float shx, float shy, const float4& p,
+                                          const float4& r, int kx, int ky, int m, float inv_w, float L,
+                                          Move& mv) {
     mv.nx = p.x + sxy * r.x;
     mv.ny = p.y + sxy * r.y;
     if (!(mv.nx >= 0.0f && mv.nx <= L && mv.ny >= 0.0f && mv.ny <= L)) return false;
-    if (cellcoord(mv.nx, t.shx[g], inv_w, 0, m - 1) + 2 != kx ||
-        cellcoord(mv.ny, t.shy[g], inv_w, 0, m - 1) + 2 != ky)
+    if (cellcoord(mv.nx, shx, inv_w, 0, m - 1) + 2 != kx || cellcoord(mv.ny, shy, inv_w, 0, m - 1) + 2 != ky)
         return false;
-    rot32(p.z, p.w, t.sth[g] * r.z, mv.nc, mv.ns);
+    rot32(p.z, p.w, sth * r.z, mv.nc, mv.ns);
     mv.wdiff = wall32(mv.nx, mv.ny, mv.nc, mv.ns, L) - wall32(p.x, p.y, p.z, p.w, L);
     return true;
+}
+__device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4& p, const float4& r, int kx,
+                                          int ky, int m, float inv_w, float L, Move& mv) {
+    return make_move(t.sxy[g], t.sth[g], t.shx[g], t.shy[g], p, r, kx, ky, m, inv_w, L, mv);
 }
 
 // Stencil energy change and Metropolis test of one proposal of tile square q
@@ -630,20 +634,15 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     evaluate<kQuads>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
 }
 
-// Mover record j of a level (one-wavefront tiles): the square's position,
-// staged move, wall change, uniform, 1/T, indices and stencil, written once
-// per sweep by one lane per square when the level items are scattered.
-__device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int n, int M, int m,
-                                             float inv_w, float L, int r0, uint32_t sweep, uint32_t k0,
-                                             uint32_t k1) {
-    const int g = rep_of(t, q), i = q - g * n;
-    const int ck = t.cellk[q];
-    const int kx = ck & 255, ky = ck >> 8;
-    const float4 p = t.pos[q];
-    const Stencil sc(t.A + g * grid_words(M), M, kx, ky);
-    const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
+// Mover record j of a level (one-wavefront tiles): square q (replica g,
+// index i in its replica, CSR slot `self`) at position p in cell (kx, ky),
+// with its stencil, staged move (variates r), wall change, uniform and 1/T.
+__device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int g, int self, const float4& p,
+                                             int kx, int ky, const Stencil& sc, const float4& r, float sxy,
+                                             float sth, float shx, float shy, float ik, int m, float inv_w,
+                                             float L) {
     Move mv;
-    const bool ok = make_move(t, g, p, r, kx, ky, m, inv_w, L, mv);
+    const bool ok = make_move(sxy, sth, shx, shy, p, r, kx, ky, m, inv_w, L, mv);
     uint4 s0;
     unsigned s1x;
     sc.pack(s0, s1x);
@@ -652,10 +651,23 @@ __device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int
     w[1] = ok ? make_uint4(__float_as_uint(mv.nx), __float_as_uint(mv.ny), __float_as_uint(mv.nc),
                            __float_as_uint(mv.ns))
               : make_uint4(0u, 0u, __float_as_uint(kNoMove), 0u);
-    w[2] = make_uint4(__float_as_uint(ok ? mv.wdiff : 0.0f), __float_as_uint(r.w), __float_as_uint(t.ik[g]),
-                      (unsigned)q | (unsigned)t.slotq[q] << 16);
+    w[2] = make_uint4(__float_as_uint(ok ? mv.wdiff : 0.0f), __float_as_uint(r.w), __float_as_uint(ik),
+                      (unsigned)q | (unsigned)self << 16);
     w[3] = s0;
     w[4] = make_uint4(s1x, (unsigned)g, 0u, 0u);
+}
+// The same from the tile's LDS arrays (square q of level item j), written
+// once per sweep by one lane per square when the level items are scattered.
+__device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int n, int M, int m,
+                                             float inv_w, float L, int r0, uint32_t sweep, uint32_t k0,
+                                             uint32_t k1) {
+    const int g = rep_of(t, q), i = q - g * n;
+    const int ck = t.cellk[q];
+    const int kx = ck & 255, ky = ck >> 8;
+    const Stencil sc(t.A + g * grid_words(M), M, kx, ky);
+    const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
+    write_record(t, j, q, g, t.slotq[q], t.pos[q], kx, ky, sc, r, t.sxy[g], t.sth[g], t.shx[g], t.shy[g],
+                 t.ik[g], m, inv_w, L);
 }
 
 // The proposal of mover record j: one LDS round trip for everything but the
@@ -917,6 +929,96 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
 // one tile.  One sweep = every square proposes exactly once.  NT threads: one
 // wavefront, or four for large instances (the same arithmetic either way).
 // ---------------------------------------------------------------------------
+// Sweep set-up of a one-wavefront tile of at most 64 squares, one square per
+// lane (q = lane): the binning of bin_tile, the group keys, the levels, the
+// level starts and the mover records of the general set-up in k_sweep, with
+// each square's position, cell, CSR slot and rank kept in registers between
+// the phases instead of read back from LDS, and its move's variates drawn
+// while the first reads are in flight.  Leaves the same LDS state and returns
+// the level starts (lane l: start of level l; lane kGroups + 1: the total).
+__device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, float inv_w, float L, float w,
+                                  int r0, uint32_t sweep, bool next_shift, uint32_t k0, uint32_t k1) {
+    const int lane = threadIdx.x;
+    const bool has = lane < G * n;
+    const int gw = grid_words(M);
+    const int q = lane;
+    const int g = has ? rep_of(t, q) : 0;
+    const int i = q - g * n;
+    uint16_t* Ag = t.A + g * gw;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float shx = 0.0f, shy = 0.0f, sxy = 0.0f, sth = 0.0f, ik = 0.0f;
+    if (has) {
+        p = t.pos[q];
+        shx = t.shx[g];
+        shy = t.shy[g];
+        sxy = t.sxy[g];
+        sth = t.sth[g];
+        ik = t.ik[g];
+    }
+    const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
+    if (lane < kGroups + 2) t.gcnt[lane] = 0;
+    if (lane < G) t.gmask[lane] = 0ull;
+    uint4* A128 = reinterpret_cast<uint4*>(t.A);
+    for (int k = lane; k < (G * gw) / 8; k += kSweepThreads) A128[k] = make_uint4(0u, 0u, 0u, 0u);
+    int kx = 0, ky = 0, c = 0;
+    if (has) {
+        kx = cellcoord(p.x, shx, inv_w, 0, m - 1) + 2;
+        ky = cellcoord(p.y, shy, inv_w, 0, m - 1) + 2;
+        c = ky * M + kx;
+        t.cellk[q] = (uint16_t)(kx | (ky << 8));
+    }
+    wave_sync();
+    if (has) add_u16(Ag, c + 2, 1u);
+    wave_sync();
+    wave_scan_u16(t.A, G * gw);
+    wave_sync();
+    if (has) t.sorted[add_u16(Ag, c + 1, 1u)] = (uint16_t)i;
+    wave_sync();
+    // the cell starts are final: the stencil's row ranges are read with the
+    // cell's members
+    Stencil sc;
+    int a = 0, rk = 0;
+    if (has) {
+        sc = Stencil(Ag, M, kx, ky);
+        a = Ag[c];
+        const int b = Ag[c + 1];
+        for (int k = a; k < b; ++k) rk += t.sorted[k] < i;
+    }
+    wave_sync();
+    const int self = a + rk;
+    int key = kGroups;
+    if (has) {
+        t.sorted[self] = (uint16_t)i;
+        t.spos[self] = p;
+        t.slotq[q] = (uint16_t)self;
+        const int colour = (kx % 3) + 3 * (ky % 3);
+        key = rk < kRankCap ? rk * 9 + colour : kGroups;
+        atomicOr(&t.gmask[g], 1ull << key);
+    }
+    if (next_shift && lane < G) {
+        const uint4 o = philox(0u, (uint32_t)(r0 + lane), sweep + 1u, 1u, k0, k1);
+        t.shx_next[lane] = w * u01(o.x);
+        t.shy_next[lane] = w * u01(o.y);
+    }
+    wave_sync();
+    int lvl = kGroups;
+    if (has) {
+        if (key < kGroups) lvl = __popcll(t.gmask[g] & ((1ull << key) - 1ull));
+        t.rank[q] = (uint16_t)lvl;
+        atomicAdd(&t.gcnt[lvl], 1);
+    }
+    wave_sync();
+    const int cnt = lane <= kGroups ? t.gcnt[lane] : 0;
+    const int gs = wave_incl_scan(cnt) - cnt;
+    if (lane <= kGroups) t.gcnt[lane] = gs;
+    wave_sync();
+    if (has && lvl < kGroups)
+        write_record(t, atomicAdd(&t.gcnt[lvl], 1), q, g, self, p, kx, ky, sc, r, sxy, sth, shx, shy, ik, m,
+                     inv_w, L);
+    wave_sync();
+    return gs;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
                                               InstDesc* __restrict__ insts,
@@ -963,74 +1065,80 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
     unsigned long long cC = 0, cP = 0;
     for (int sw = 0; sw < nsw; ++sw) {
         const uint32_t sweep = sweep0 + (uint32_t)sw;
-        for (int k = tid; k < kGroups + 2; k += NT) t.gcnt[k] = 0;
-        for (int g = tid; g < G; g += NT) t.gmask[g] = 0ull;
-        tile_sync<NT>();
-        bin_tile<NT>(t, n, G, M, m, inv_w);
-        // group key: occupancy rank x colour; ranks >= kRankCap go serial.
-        // Small tiles stage every square's proposal here (one Philox block
-        // per square; a square's own state does not change before its level
-        // runs), while the LDS atomics of the key pass are in flight; tiles
-        // with mover records draw it when the records are written.
-        for (int q = tid; q < G * n; q += NT) {
-            const int g = rep_of(t, q), i = q - g * n;
-            const int ck = t.cellk[q];
-            const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
-            const int r = t.rank[q];
-            const int key = r < kRankCap ? r * 9 + colour : kGroups;
-            t.rank[q] = (uint16_t)key;
-            atomicOr(&t.gmask[g], 1ull << key);
-            if (t.prop) {
-                const float4 r = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
-                Move mv;
-                const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
-                t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
-                t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
-            }
-        }
-        if (sw + 1 < nsw)
-            for (int g = tid; g < G; g += NT) {
-                const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep + 1u, 1u, k0, k1);
-                t.shx_next[g] = w * u01(o.x);
-                t.shy_next[g] = w * u01(o.y);
-            }
-        tile_sync<NT>();
-        // level = position of the square's group among its replica's occupied
-        // groups: each replica walks only its own non-empty groups, in key
-        // order, and one pass serves level l of every replica of the tile
-        for (int q = tid; q < G * n; q += NT) {
-            const int key = t.rank[q];
-            const int lvl =
-                key < kGroups ? __popcll(t.gmask[rep_of(t, q)] & ((1ull << key) - 1ull)) : kGroups;
-            t.rank[q] = (uint16_t)lvl;
-            atomicAdd(&t.gcnt[lvl], 1);
-        }
-        tile_sync<NT>();
-        // level starts: one wave-wide exclusive scan of the level counts; lane
-        // l keeps the start of level l (lane kGroups + 1: the total), and the
-        // counts become the scatter cursors.  The other wavefronts of a large
-        // tile read the starts back from LDS.
-        static_assert(kGroups + 2 <= 64, "level starts must fit one wavefront");
         int gs = 0;
-        if (tid < 64) {
-            const int c = lane <= kGroups ? t.gcnt[lane] : 0;
-            gs = wave_incl_scan(c) - c;
-            if (lane <= kGroups) t.gcnt[lane] = gs;
-            if (NT > 64 && lane <= kGroups + 1) t.gstart[lane] = gs;
-        }
-        tile_sync<NT>();
-        if (NT > 64) gs = lane <= kGroups + 1 ? t.gstart[lane] : 0;
-        for (int q = tid; q < G * n; q += NT) {
-            const int lvl = t.rank[q];
-            if (lvl < kGroups) {
-                const int j = atomicAdd(&t.gcnt[lvl], 1);
-                if (NT == 64 && t.rec)
-                    write_record(t, j, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1);
-                else
-                    t.items[j] = (uint16_t)q;
+        if (NT == 64 && t.rec && G * n <= kSweepThreads) {
+            if constexpr (NT == 64)
+                gs = setup_lane_squares(t, n, G, M, m, inv_w, L, w, tl.r0, sweep, sw + 1 < nsw, k0, k1);
+        } else {
+            for (int k = tid; k < kGroups + 2; k += NT) t.gcnt[k] = 0;
+            for (int g = tid; g < G; g += NT) t.gmask[g] = 0ull;
+            tile_sync<NT>();
+            bin_tile<NT>(t, n, G, M, m, inv_w);
+            // group key: occupancy rank x colour; ranks >= kRankCap go serial.
+            // Small tiles stage every square's proposal here (one Philox block
+            // per square; a square's own state does not change before its level
+            // runs), while the LDS atomics of the key pass are in flight; tiles
+            // with mover records draw it when the records are written.
+            for (int q = tid; q < G * n; q += NT) {
+                const int g = rep_of(t, q), i = q - g * n;
+                const int ck = t.cellk[q];
+                const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
+                const int r = t.rank[q];
+                const int key = r < kRankCap ? r * 9 + colour : kGroups;
+                t.rank[q] = (uint16_t)key;
+                atomicOr(&t.gmask[g], 1ull << key);
+                if (t.prop) {
+                    const float4 r = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
+                    Move mv;
+                    const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
+                    t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns)
+                                   : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
+                    t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
+                }
             }
+            if (sw + 1 < nsw)
+                for (int g = tid; g < G; g += NT) {
+                    const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep + 1u, 1u, k0, k1);
+                    t.shx_next[g] = w * u01(o.x);
+                    t.shy_next[g] = w * u01(o.y);
+                }
+            tile_sync<NT>();
+            // level = position of the square's group among its replica's occupied
+            // groups: each replica walks only its own non-empty groups, in key
+            // order, and one pass serves level l of every replica of the tile
+            for (int q = tid; q < G * n; q += NT) {
+                const int key = t.rank[q];
+                const int lvl =
+                    key < kGroups ? __popcll(t.gmask[rep_of(t, q)] & ((1ull << key) - 1ull)) : kGroups;
+                t.rank[q] = (uint16_t)lvl;
+                atomicAdd(&t.gcnt[lvl], 1);
+            }
+            tile_sync<NT>();
+            // level starts: one wave-wide exclusive scan of the level counts; lane
+            // l keeps the start of level l (lane kGroups + 1: the total), and the
+            // counts become the scatter cursors.  The other wavefronts of a large
+            // tile read the starts back from LDS.
+            static_assert(kGroups + 2 <= 64, "level starts must fit one wavefront");
+            if (tid < 64) {
+                const int c = lane <= kGroups ? t.gcnt[lane] : 0;
+                gs = wave_incl_scan(c) - c;
+                if (lane <= kGroups) t.gcnt[lane] = gs;
+                if (NT > 64 && lane <= kGroups + 1) t.gstart[lane] = gs;
+            }
+            tile_sync<NT>();
+            if (NT > 64) gs = lane <= kGroups + 1 ? t.gstart[lane] : 0;
+            for (int q = tid; q < G * n; q += NT) {
+                const int lvl = t.rank[q];
+                if (lvl < kGroups) {
+                    const int j = atomicAdd(&t.gcnt[lvl], 1);
+                    if (NT == 64 && t.rec)
+                        write_record(t, j, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1);
+                    else
+                        t.items[j] = (uint16_t)q;
+                }
+            }
+            tile_sync<NT>();
         }
-        tile_sync<NT>();
         for (int lv = 0; lv < kGroups; ++lv) {
             const int a = __builtin_amdgcn_readlane(gs, lv);
             const int b = __builtin_amdgcn_readlane(gs, lv + 1);

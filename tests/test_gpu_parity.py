@@ -103,15 +103,21 @@ def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
 
 
 @pytest.mark.parametrize("n,L,R,sweeps,group", [(11, 5.0, 4, 6, 1), (50, 9.0, 6, 5, 3), (90, 11.0, 3, 3, 1),
+                                                (20, 6.0, 6, 4, 3), (10, 4.5, 8, 5, 6), (16, 5.0, 4, 4, 4),
+                                                (64, 10.0, 2, 3, 1), (60, 3.0, 2, 3, 1),
                                                 (100, 11.0, 4, 3, 4), (120, 12.5, 3, 3, 2),
                                                 (300, 17.5, 2, 3, 1), (300, 17.5, 4, 2, 2),
                                                 (700, 27.0, 1, 2, 1)])
 def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     """k_sweep == the CPU model of the same colour-checkerboard sweep: same
-    states, accept counts and canonical energies, bit for bit.  n >= 96 runs
-    the four-wavefront tiles, with the proposals staged in LDS (up to 512
-    squares per tile, and 700 in a tile of one replica) and drawn inline (600
-    squares in tiles of two replicas)."""
+    states, accept counts and canonical energies, bit for bit.  Below n = 96
+    one-wavefront tiles: mover records with the register-resident set-up (up
+    to 64 squares per tile, one or several replicas, n = 60 in a 3 x 3 box
+    filling the serial group of ranks >= 4), mover records with the LDS
+    set-up (n = 90 alone) and staged proposal arrays (150 squares in tiles of
+    three replicas).  n >= 96 runs the four-wavefront tiles, with the
+    proposals staged in LDS (up to 512 squares per tile, and 700 in a tile of
+    one replica) and drawn inline (600 squares in tiles of two replicas)."""
     rng = np.random.default_rng(n)
     st = _seeded_state(rng, R, n, L)
     T = np.geomspace(1e-3, 1e-1, R)
