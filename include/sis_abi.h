@@ -202,6 +202,10 @@ int sis_dev_depth32(int npairs, const float* in, float* depth, float* depth2);
  * lo = 0, hi = 15 this is the reference's binning (cuda/packing.cu:22-25). */
 int sis_dev_cellcoord(int count, const float* x, float shift, float w, int lo, int hi,
                       int* out);
+/* The Metropolis test's 2^x (x <= 0), out[k] = 2^x[k] as the sweep computes it
+ * for `u < 2^(-dE/T log2 e)`; replaces the exp of the acceptance in
+ * cuda/packing.cu:80,123 (walk, step).  Same bits as the oracle's FP32 form. */
+int sis_dev_accept_exp2(int count, const float* x, float* out);
 /* Run `sweeps` sweeps of the FP32 sweep kernel on R replicas of one instance,
  * in place: xycs = R*n*4 floats (x, y, cos, sin).  temps/steps per replica.
  * Returns per-replica accepted moves and FP32 energies after the sweeps.

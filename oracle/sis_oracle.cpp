@@ -59,6 +59,28 @@ void o_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
 /* (k + 1/2) * 2^-23 for the top 23 bits k: exact, symmetric, in (0, 1). */
 float o_u01f(uint32_t r) { return ((float)(r >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
+/* 2^x of the Metropolis test (x <= 0), the acceptance of cuda/packing.cu:80,123
+ * written as FP32 operations that each round once or not at all: k = rint(x),
+ * f = x - k (exact) in [-1/2, 1/2], p(f) = the degree-6 polynomial through
+ * 2^f at the 7 Chebyshev nodes of that interval (Horner, fused multiply-adds),
+ * then the exact scaling by 2^k.  Any IEEE FP32 machine with fma gives the
+ * same bits, the device included; relative error below 1e-7 (pinned against
+ * 2^x in tests/test_oracle_geometry.py).  0 below x = -100, where no accept
+ * uniform (>= 2^-24) can pass. */
+float o_accept_exp2(float x) {
+    if (!(x >= -100.0f)) return 0.0f;
+    const float k = rintf(x);
+    const float f = x - k;
+    float p = 0x1.444p-13f;
+    p = fmaf(p, f, 0x1.5f48cp-10f);
+    p = fmaf(p, f, 0x1.3b2a1cp-7f);
+    p = fmaf(p, f, 0x1.c6aeccp-5f);
+    p = fmaf(p, f, 0x1.ebfbep-3f);
+    p = fmaf(p, f, 0x1.62e43p-1f);
+    p = fmaf(p, f, 1.0f);
+    return ldexpf(p, (int)k);
+}
+
 double o_u01d(uint32_t hi, uint32_t lo) {
     uint64_t k = ((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11);
     k &= ((1ull << 52) - 1);
@@ -784,7 +806,7 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
             }
             const float dE = wdiff + ((part[0] + part[1]) + (part[2] + part[3]));
             const float u = o_u01f(o[3]);
-            if (dE <= 0.0f || u < exp2f(-dE * cfg->inv_t_log2e)) {
+            if (dE <= 0.0f || u < o_accept_exp2(-dE * cfg->inv_t_log2e)) {
                 x[i] = nx; y[i] = ny; c[i] = nc; s[i] = ns;
                 accepted++;
             }
@@ -898,7 +920,7 @@ int o_collective32_model(const o_sweep_cfg* cfg, int moves, float* x, float* y, 
         }
         const float e_new = o_energy32_model(n, L, cfg->w, x, y, c, s);
         const float dE = e_new - e_old;
-        if (dE <= 0.0f || uacc < exp2f(-dE * cfg->inv_t_log2e)) {
+        if (dE <= 0.0f || uacc < o_accept_exp2(-dE * cfg->inv_t_log2e)) {
             e_old = e_new;
             accepted++;
         } else {

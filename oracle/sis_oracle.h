@@ -33,6 +33,7 @@ float o_depth32_cs(float dx, float dy, float ci, float si, float cj, float sj);
 float o_depth2_32_cs(float dx, float dy, float ci, float si, float cj, float sj);
 float o_wall32_cs(float x, float y, float c, float s, float L);
 void o_rot32(float c, float s, float t, float* c_out, float* s_out);
+float o_accept_exp2(float x);
 
 /* ---- reference FP64 geometry (cuda/packing.cu:163-297, cuda/seeded.cu:40-113) ---- */
 double o_pair64(const double p[3], const double q[3], double grad[3]);

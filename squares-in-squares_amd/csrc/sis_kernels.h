@@ -46,5 +46,6 @@ hipError_t launch_gather_desc(hipStream_t st, int count, const InstDesc* insts, 
 hipError_t launch_depth32(hipStream_t st, int npairs, const float* in, float* depth, float* d2);
 hipError_t launch_cellcoord(hipStream_t st, int count, const float* x, float shift, float w,
                             int lo, int hi, int* out);
+hipError_t launch_accept_exp2(hipStream_t st, int count, const float* x, float* out);
 
 }  // namespace sis

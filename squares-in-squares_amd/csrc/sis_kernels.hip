@@ -10,6 +10,7 @@
 //              export of one configuration per instance (cuda/packing.cu:
 //              254-297, 624-697).
 //  k_depth32 / k_cellcoord   parity probes of the geometry on caller data.
+//  k_accept_exp2             parity probe of the Metropolis test's 2^x.
 //
 // All arithmetic whose bits are checked against the CPU oracle is written
 // with explicit fmaf and compiled with contraction off.
@@ -39,6 +40,26 @@ __device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, u
 }
 __device__ __forceinline__ float u01(uint32_t r) {
     return ((float)(r >> 9) + 0.5f) * (1.0f / 8388608.0f);
+}
+// 2^x for the Metropolis test (x <= 0), in FP32 operations that each round
+// once or not at all (round to nearest integer, an exact subtraction, fused
+// multiply-adds, an exact power-of-two scaling), so that any IEEE FP32 machine
+// with a fused multiply-add gives the same bits.  k = rint(x), f = x - k in
+// [-1/2, 1/2], and p(f) interpolates 2^f at the 7 Chebyshev nodes of that
+// interval (relative error below 1e-7).  Below x = -100 the value is 0: no
+// accept uniform is smaller than 2^-24.
+__device__ __forceinline__ float accept_exp2(float x) {
+    if (!(x >= -100.0f)) return 0.0f;
+    const float k = rintf(x);
+    const float f = x - k;
+    float p = 0x1.444p-13f;
+    p = __builtin_fmaf(p, f, 0x1.5f48cp-10f);
+    p = __builtin_fmaf(p, f, 0x1.3b2a1cp-7f);
+    p = __builtin_fmaf(p, f, 0x1.c6aeccp-5f);
+    p = __builtin_fmaf(p, f, 0x1.ebfbep-3f);
+    p = __builtin_fmaf(p, f, 0x1.62e43p-1f);
+    p = __builtin_fmaf(p, f, 1.0f);
+    return ldexpf(p, (int)k);
 }
 // Move variates of square i of replica `rep` in sweep `sweep`: fx, fy, ftheta
 // uniform in [-1, 1) and the accept uniform (Philox counter: square, replica,
@@ -589,7 +610,7 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
     cC += sub == 0 ? (unsigned)last : 0u;
     cP += near;
     const float dE = wdiff + sum;
-    if (sub == 0 && (dE <= 0.0f || u < exp2f(-dE * ik))) {
+    if (sub == 0 && (dE <= 0.0f || u < accept_exp2(-dE * ik))) {
         const float4 np = make_float4(nx, ny, nc, ns);
         t.pos[q] = np;
         if (t.spos) t.spos[self] = np;
@@ -815,7 +836,7 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
     for (int g = tid; g < G; g += NT) {
         CollWork& w = t.cw[g];
         const float en = t.eout[g], dE = en - w.eold;
-        w.accept = (dE <= 0.0f || w.uacc < exp2f(-dE * t.ik[g])) ? 1 : 0;
+        w.accept = (dE <= 0.0f || w.uacc < accept_exp2(-dE * t.ik[g])) ? 1 : 0;
         t.eout[g] = w.accept ? en : w.eold;
         tried += 1;
         accd += (unsigned)w.accept;
@@ -1810,6 +1831,11 @@ __global__ void k_depth32(int npairs, const float* __restrict__ in, float* __res
     d2out[k] = depth2(dx, dy, p[2], p[3], p[4], p[5]);
 }
 
+__global__ void k_accept_exp2(int count, const float* __restrict__ x, float* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) out[k] = accept_exp2(x[k]);
+}
+
 __global__ void k_cellcoord(int count, const float* __restrict__ x, float shift, float w, int lo,
                             int hi, int* __restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1873,6 +1899,11 @@ hipError_t launch_cellcoord(hipStream_t st, int count, const float* x, float shi
                             int hi, int* out) {
     hipLaunchKernelGGL(k_cellcoord, dim3((count + 255) / 256), dim3(256), 0, st, count, x, shift, w,
                        lo, hi, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_accept_exp2(hipStream_t st, int count, const float* x, float* out) {
+    hipLaunchKernelGGL(k_accept_exp2, dim3((count + 255) / 256), dim3(256), 0, st, count, x, out);
     return hipGetLastError();
 }
 

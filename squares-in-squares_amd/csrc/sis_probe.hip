@@ -44,6 +44,20 @@ int sis_dev_cellcoord(int count, const float* x, float shift, float w, int lo, i
     return SIS_OK;
 }
 
+int sis_dev_accept_exp2(int count, const float* x, float* out) {
+    if (count < 0 || (count > 0 && (!x || !out))) return fail(SIS_EINVAL, "bad arguments");
+    if (count == 0) return SIS_OK;
+    int rc = ensure_device(-1);
+    if (rc) return rc;
+    DevBuf<float> d_x, d_o;
+    SIS_HIPCHECK(d_x.alloc(count));
+    SIS_HIPCHECK(d_o.alloc(count));
+    SIS_HIPCHECK(hipMemcpy(d_x, x, sizeof(float) * count, hipMemcpyHostToDevice));
+    SIS_HIPCHECK(launch_accept_exp2(nullptr, count, d_x, d_o));
+    SIS_HIPCHECK(hipMemcpy(out, d_o, sizeof(float) * count, hipMemcpyDeviceToHost));
+    return SIS_OK;
+}
+
 int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
                    const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
                    uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,

@@ -117,6 +117,7 @@ def lib() -> ctypes.CDLL:
     L.sis_dev_depth32.argtypes = [ctypes.c_int, fp, fp, fp]
     L.sis_dev_cellcoord.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                     ctypes.c_int, ip]
+    L.sis_dev_accept_exp2.argtypes = [ctypes.c_int, fp, fp]
     L.sis_dev_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                  fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                  fp, ip, fp, u64p]
@@ -391,6 +392,16 @@ def dev_cellcoord(x: np.ndarray, shift: float, w: float, lo: int, hi: int) -> np
     x = np.ascontiguousarray(x, dtype=np.float32)
     out = np.zeros(x.shape[0], dtype=np.int32)
     rc = lib().sis_dev_cellcoord(x.shape[0], _ptr(x, ctypes.c_float), shift, w, lo, hi, _ptr(out, ctypes.c_int))
+    if rc:
+        _err(rc)
+    return out
+
+
+def dev_accept_exp2(x: np.ndarray) -> np.ndarray:
+    """2^x of the sweep's Metropolis test, computed on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(x.shape[0], dtype=np.float32)
+    rc = lib().sis_dev_accept_exp2(x.shape[0], _ptr(x, ctypes.c_float), _ptr(out, ctypes.c_float))
     if rc:
         _err(rc)
     return out

@@ -57,6 +57,8 @@ class Oracle:
         L.o_wall32_cs.argtypes = [f32] * 5
         L.o_wall32_cs.restype = f32
         L.o_rot32.argtypes = [f32, f32, f32, P(f32), P(f32)]
+        L.o_accept_exp2.argtypes = [f32]
+        L.o_accept_exp2.restype = f32
         L.o_pair64.argtypes = [P(f64), P(f64), P(f64)]
         L.o_pair64.restype = f64
         L.o_boundary64.argtypes = [P(f64), f64, P(f64)]

@@ -77,6 +77,18 @@ def test_cell_binning_bit_exact(sis, oracle):
             assert out[k] == oracle.lib.o_cellcoord(x[k], shift, w, 0, 60)
 
 
+def test_accept_exp2_bit_exact(sis, oracle):
+    """The sweep's 2^x for the Metropolis test equals the oracle's FP32 form
+    bit for bit, round-to-nearest ties, -0 and the cut-off at -100 included."""
+    from test_oracle_geometry import _accept_exp2_inputs
+    x = _accept_exp2_inputs(np.random.default_rng(5))
+    say("accept_exp2", x.shape)
+    d = sis.dev_accept_exp2(x)
+    f = oracle.lib.o_accept_exp2
+    ref = np.array([f(float(v)) for v in x], dtype=np.float32)
+    assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
+
+
 def _seeded_state(rng, R, n, L):
     x = rng.uniform(0.2, L - 0.2, (R, n))
     y = rng.uniform(0.2, L - 0.2, (R, n))

@@ -260,3 +260,37 @@ def test_grid_packing_matches_reference_grid(oracle):
         width = math.isqrt(n - 1) + 1
         for i in range(n):
             assert q[i, 0] == i % width + 0.5 and q[i, 1] == i // width + 0.5 and q[i, 2] == 0.0
+
+
+def _accept_exp2_inputs(rng):
+    """x <= 0 over the whole useful range, with the round-to-nearest ties of
+    the range reduction (half integers), integers, -0, the cut-off at -100 and
+    values just inside and outside it."""
+    x = np.concatenate([-rng.exponential(3.0, 200000), -rng.uniform(0, 110, 50000),
+                        -np.arange(0, 101, 0.5), -np.arange(0, 1, 1 / 64),
+                        [-0.0, -1e-30, -1e-7, -99.99999, -100.0, -100.00001, -120.0]])
+    return x.astype(np.float32)
+
+
+def test_accept_exp2_against_two_to_the_x(oracle):
+    """The Metropolis test's 2^x (o_accept_exp2, the device's accept_exp2):
+    relative error below 1.5e-7 of 2^x in FP64 over [-100, 0]; exact powers of
+    two at the integers (the polynomial is 1 at f = 0 and the scaling is
+    exact); 0 below -100, where every accept uniform (>= 2^-24) rejects."""
+    f = oracle.lib.o_accept_exp2
+    rng = np.random.default_rng(5)
+    x = _accept_exp2_inputs(rng)
+    worst = 0.0
+    for v in x:
+        got = f(float(v))
+        if v < -100.0:
+            assert got == 0.0
+            continue
+        ref = 2.0 ** float(v)
+        worst = max(worst, abs(got - ref) / ref)
+    assert worst < 1.5e-7, worst
+    for k in range(0, 101):
+        assert f(-float(k)) == 2.0 ** -k
+    assert f(float("nan")) == 0.0
+    assert f(-25.0) < 2.0 ** -24  # from here down no accept uniform passes
+
