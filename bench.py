@@ -262,6 +262,7 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         lat.append(tc)
         cert_ok += results[tk].status == 0
         Ls[tk] = L50
+        srv.release(tk)
     srv.close()
     props = sum(results[tk].proposals for tk in tickets)
     flops = sum(12.0 * results[tk].candidates + 28.0 * results[tk].near_pairs + 45.0 * results[tk].proposals

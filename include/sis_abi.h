@@ -178,6 +178,11 @@ int sis_poll(sis_server* s, uint64_t ticket, sis_result* out);
 /* Seconds from submit to a certified result, and the certified L (50 decimals). */
 int sis_ticket_times(sis_server* s, uint64_t ticket, double* queued_s, double* done_s,
                      double* certified_s, char* L50, int L50_len);
+/* Forget a finished ticket (its result, packing and times): a long-running
+ * server keeps every ticket until released or destroyed.  SIS_PENDING (and the
+ * ticket kept) while it is queued or running.  No counterpart in the reference,
+ * whose request loop ends with its process pool. */
+int sis_ticket_release(sis_server* s, uint64_t ticket);
 int sis_server_destroy(sis_server* s);
 
 /* Exact certificate of one packing: decimal strings in, 50-decimal L out.

@@ -705,6 +705,18 @@ int sis_ticket_times(sis_server* h, uint64_t ticket, double* queued_s, double* d
     return r->done ? SIS_OK : SIS_PENDING;
 }
 
+int sis_ticket_release(sis_server* h, uint64_t ticket) {
+    if (!h) return fail(SIS_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(h->s.mu);
+    auto it = h->s.all.find(ticket);
+    if (it == h->s.all.end()) return fail(SIS_EINVAL, "unknown ticket");
+    // done is set once the device loop and the certifier are through with it
+    if (!it->second->done) return SIS_PENDING;
+    delete it->second;
+    h->s.all.erase(it);
+    return SIS_OK;
+}
+
 int sis_server_destroy(sis_server* h) {
     if (!h) return SIS_OK;
     {

@@ -109,6 +109,7 @@ def lib() -> ctypes.CDLL:
     L.sis_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), u64p]
     L.sis_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(SisResult)]
     L.sis_ticket_times.argtypes = [ctypes.c_void_p, ctypes.c_uint64, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
+    L.sis_ticket_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
     L.sis_server_destroy.argtypes = [ctypes.c_void_p]
     L.sis_certify_strings.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
@@ -351,6 +352,16 @@ class Server:
         if rc not in (SIS_OK, SIS_PENDING):
             _err(rc)
         return a.value, b.value, c.value, buf.value.decode()
+
+    def release(self, ticket: int) -> bool:
+        """Forget a finished ticket; False (ticket kept) while it is pending."""
+        rc = lib().sis_ticket_release(self.h, ticket)
+        if rc == SIS_PENDING:
+            return False
+        if rc:
+            _err(rc)
+        self.ns.pop(ticket, None)
+        return True
 
     def close(self):
         if self.h:

@@ -71,6 +71,15 @@ def test_server_admission_under_slot_cap(sis):
     tickets = [srv.submit(12 + 3 * k, 5.0, None, replicas=8, sweeps=100, init_mode=1, rng_seed=k) for k in range(6)]
     res = _wait(srv, tickets)
     admits = sorted(srv.times(t)[0] for t in tickets)
-    srv.close()
     assert len(res) == 6
     assert admits[-1] > admits[0]  # the last ones waited for a slot
+    # finished tickets can be forgotten; a running one is kept
+    late = srv.submit(30, 6.5, None, replicas=8, sweeps=20000, init_mode=1, rng_seed=99)
+    assert srv.release(late) is False
+    for t in tickets:
+        assert srv.release(t) is True
+        with pytest.raises(sis.SisError):
+            srv.times(t)
+    assert _wait(srv, [late])[late] is not None
+    assert srv.release(late) is True
+    srv.close()
