@@ -733,14 +733,28 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const float4* pg = t.pos + g * n;
         float ei = wall32(p.x, p.y, p.z, p.w, L);
         for (int f = 0; f < sc.total; ++f) {
-            const int j = t.sorted[sc.at(f)];
+            const int k = sc.at(f);
+            const int j = t.sorted[k];
+            // with the cell-ordered copy the position does not wait for the index
+            const float4 v = t.spos ? t.spos[k] : pg[j];
             if (j <= i) continue;
-            const float4 v = pg[j];
             ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
         }
         e[q] = ei;
     }
     tile_sync<NT>();
+    if (NT == 64 && G * n <= 64) {
+        // one square per lane (q = lane): the same tree by lane shifts
+        const int q = tid, g = q < G * n ? rep_of(t, q) : 0, i = q - g * n;
+        float v = q < G * n ? e[q] : 0.0f;
+        for (int s = 1; s < n; s <<= 1) {
+            const float o = __shfl_down(v, (unsigned)s, 64);
+            if (q < G * n && (i & (2 * s - 1)) == 0 && i + s < n) v += o;
+        }
+        if (q < G * n && i == 0) out[g] = v;
+        tile_sync<NT>();
+        return;
+    }
     for (int s = 1; s < n; s <<= 1) {
         for (int q = tid; q < G * n; q += NT) {
             const int g = rep_of(t, q), i = q - g * n;
