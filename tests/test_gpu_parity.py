@@ -145,6 +145,30 @@ def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     assert dcnt[0] == R * n * sweeps and np.array_equal(dcnt, mcnt)
 
 
+@pytest.mark.parametrize("n,L,R,sweeps,group,w", [(1, 1.5, 4, 5, 1, 0.7075), (2, 2.5, 4, 5, 2, 0.7075),
+                                                  (7, 5.0, 3, 4, 1, 2.0), (30, 8.0, 2, 3, 1, 4.0),
+                                                  (1000, 64.0, 1, 1, 1, 0.7075)])
+def test_sweeps_match_cpu_model_at_the_limits(sis, oracle, n, L, R, sweeps, group, w):
+    """The input limits of cuda/packing.cu:772-775 (n 1..1000, L up to 64) and the
+    cell widths the library accepts (the reference's own width 2, the maximum 4):
+    one and two squares (a stencil holding only the mover), n = 1000 in the
+    largest box (96 cells per side, sparse levels), wide cells with many
+    squares per cell (occupancy ranks up to the serial group)."""
+    rng = np.random.default_rng(1000 + n)
+    st = _seeded_state(rng, R, n, L)
+    T = np.geomspace(1e-3, 1e-1, R)
+    ik = (LOG2E / T).astype(np.float32)
+    sxy = np.full(R, 0.12, np.float32)
+    sth = np.full(R, 0.1, np.float32)
+    say("sweeps at the limits", n, L, R, sweeps, group, w)
+    dst, dacc, den, dcnt = sis.dev_sweeps(st, L, w, ik, sxy, sth, 7, 0, 11, sweeps, group=group)
+    mst, macc, men, mcnt = _model_run(oracle, st, L, w, ik, sxy, sth, 7, 0, 11, sweeps)
+    assert np.array_equal(dacc, macc)
+    assert np.array_equal(dst, mst)
+    assert np.array_equal(den, men)
+    assert dcnt[0] == R * n * sweeps and np.array_equal(dcnt, mcnt)
+
+
 @pytest.mark.parametrize("n,L,R,sweeps,group,moves,kick_every,interval", [
     (40, 8.0, 6, 3, 2, 4, 1, 0),      # kick 1: relocating
     (40, 8.0, 4, 2, 1, 2, 1, 1),      # kick 2: jitter only
