@@ -106,6 +106,11 @@ typedef struct sis_server sis_server;
  * device 0 implicitly, cuda/packing.cu:787). */
 int sis_init(int device);
 const char* sis_last_error(void);
+/* Status code of the calling thread's last failure: the code of an entry point
+ * that returns a handle (sis_server_create, sis_batch_create) and NULL on
+ * failure.  The reference reports failures only as a process exit code
+ * (cuda/packing.cu:13,769-775). */
+int sis_last_error_code(void);
 /* Compute units and marketing name of the device in use (the reference prints
  * prop.multiProcessorCount and prop.name into raw.json, cuda/packing.cu:862-863). */
 int sis_device_info(int* compute_units, char* name, int name_len);

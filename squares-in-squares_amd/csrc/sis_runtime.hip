@@ -18,9 +18,11 @@
 namespace sis {
 
 thread_local std::string t_err;
+thread_local int t_code = 0;
 
 int fail(int code, const std::string& msg) {
     t_err = msg;
+    t_code = code;
     return code;
 }
 
@@ -711,6 +713,7 @@ extern "C" {
 int sis_init(int device) { return ensure_device(device); }
 
 const char* sis_last_error(void) { return t_err.c_str(); }
+int sis_last_error_code(void) { return t_code; }
 
 int sis_device_info(int* compute_units, char* name, int name_len) {
     int rc = ensure_device(-1);

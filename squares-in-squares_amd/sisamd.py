@@ -89,6 +89,7 @@ def lib() -> ctypes.CDLL:
     u64p = ctypes.POINTER(ctypes.c_uint64)
     L.sis_init.argtypes = [ctypes.c_int]
     L.sis_last_error.restype = ctypes.c_char_p
+    L.sis_last_error_code.restype = ctypes.c_int
     L.sis_device_info.argtypes = [ip, ctypes.c_char_p, ctypes.c_int]
     L.sis_default_params.argtypes = [ctypes.POINTER(SisParams)]
     L.sis_default_params.restype = None
@@ -274,7 +275,7 @@ class BatchSession:
         ps = (SisParams * k)(*params)
         self.h = lib().sis_batch_create(insts, k, ps)
         if not self.h:
-            raise SisError(SIS_EINVAL, lib().sis_last_error().decode(errors="replace"))
+            _err(lib().sis_last_error_code())
 
     def run(self, intervals: int, mode: int = 0):
         """Returns (sweep_kernel_ms, sweep_launches, total_ms)."""
@@ -317,7 +318,7 @@ class Server:
         cfg = SisServerCfg(hbm_budget_bytes, max_instances, intervals_per_round, certifier_threads, int(certify))
         self.h = lib().sis_server_create(ctypes.byref(cfg))
         if not self.h:
-            raise SisError(SIS_EDEVICE, lib().sis_last_error().decode(errors="replace"))
+            _err(lib().sis_last_error_code())
         self.ns = {}
 
     def submit(self, n, L, q=None, params: SisParams | None = None, **kw) -> int:

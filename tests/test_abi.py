@@ -190,3 +190,23 @@ def test_server_without_device_fails_loudly(sis):
     with pytest.raises(sis.SisError) as e:
         sis.Server(max_instances=4)
     assert e.value.code == sis.SIS_EDEVICE
+
+
+def test_handle_constructors_report_the_failure_code(sis):
+    """Entry points that return a handle give NULL on failure and the code
+    through sis_last_error_code: invalid arguments are SIS_EINVAL (checked
+    before any device work), not a device error."""
+    with pytest.raises(sis.SisError) as e:
+        sis.Server(max_instances=0)
+    assert e.value.code == sis.SIS_EINVAL
+    with pytest.raises(sis.SisError) as e:
+        sis.Server(intervals_per_round=0)
+    assert e.value.code == sis.SIS_EINVAL
+    p = sis.default_params(replicas=8)
+    with pytest.raises(sis.SisError) as e:
+        sis.BatchSession([(0, 5.0, None)], p)
+    assert e.value.code == sis.SIS_EINVAL
+    with pytest.raises(sis.SisError) as e:
+        sis.BatchSession([(10, 70.0, None)], p)
+    assert e.value.code == sis.SIS_EINVAL
+
