@@ -401,6 +401,27 @@ def test_batch_equals_alone(sis):
             assert a.fields[k] == m.fields[k], k
 
 
+def test_autotune_returns_a_candidate_that_runs(sis):
+    """The autotuner tries every (tile size, sweeps per exchange, cell width)
+    candidate on the batch and returns the fastest; its choice is one of the
+    candidates and a batch run with it proposes the requested sweeps."""
+    say("autotune")
+    inst = [(30, 7.0, None), (120, 13.0, None), (60, 9.5, None)]
+    p = sis.default_params(replicas=16, sweeps=100, init_mode=1, tmin=1e-4, tmax=3e-2)
+    t = sis.autotune(inst, p, probe_sweeps=20)
+    assert t["candidates_tried"] == 30
+    assert t["replicas_per_group"] in (0, 1, 2, 4, 8)
+    assert t["sweeps_per_exchange"] in (5, 10, 20)
+    assert t["cell"] in (0.7075, 0.8)
+    assert t["proposals_per_s"] > 0
+    tuned = sis.default_params(replicas=16, sweeps=100, init_mode=1, tmin=1e-4, tmax=3e-2,
+                               replicas_per_group=t["replicas_per_group"],
+                               sweeps_per_exchange=t["sweeps_per_exchange"], cell=t["cell"])
+    res = sis.solve_batch(inst, tuned)
+    for (n, _, _), r in zip(inst, res):
+        assert r.proposals == n * 16 * 100
+
+
 def test_equal_temperatures_accept_every_exchange(sis):
     """Forced exchange (cuda/tests/globaltest.cu:39-46): with tmin == tmax the
     criterion is log u < 0, so every attempted swap is taken."""
