@@ -401,23 +401,23 @@ int Batch::build(std::vector<InstSpec>&& in) {
         std::copy(h_state.begin() + h_inst[k].state_off,
                   h_state.begin() + h_inst[k].state_off + specs[k].n, h_arch.begin() + h_inst[k].arch_off);
     HIPCHECK(hipMemcpyAsync(d_arch, h_arch.data(), sizeof(float4) * narch, hipMemcpyHostToDevice, stream));
-    if (ni >= 2) {
-        // two chains, instances assigned heaviest first to the lighter one
+    if (ni >= kGraphChains) {
+        // independent chains, instances assigned heaviest first to the lightest
         std::vector<int> order(ni);
         for (int k = 0; k < ni; ++k) order[k] = k;
         auto work = [&](int k) { return (double)specs[k].n * specs[k].p.replicas * (double)specs[k].p.sweeps; };
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-        std::vector<int> members[2];
-        double load[2] = {0, 0};
+        std::vector<int> members[kGraphChains];
+        double load[kGraphChains] = {};
         for (int k : order) {
-            const int c = load[0] <= load[1] ? 0 : 1;
+            const int c = (int)(std::min_element(load, load + kGraphChains) - load);
             members[c].push_back(k);
             load[c] += work(k);
         }
         std::vector<Tile> ctiles;
         std::vector<int> clist;
-        chains.resize(2);
-        for (int c = 0; c < 2; ++c) {
+        chains.resize(kGraphChains);
+        for (int c = 0; c < kGraphChains; ++c) {
             Chain& ch = chains[c];
             std::vector<Tile> t;
             for (int k : members[c]) {
