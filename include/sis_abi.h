@@ -62,6 +62,9 @@ typedef struct sis_params {
                                    moves of cuda/packing.cu:564-616), 0..64 */
     int kick_every;             /* intervals between kicks (worst-square relocation + jitter,
                                    cuda/packing.cu:696-745); 0 = none */
+    int sweeps_per_shift;       /* sweeps between random shifts of the cell grid, 1..100000: a
+                                   move never leaves its cell, so the cell lists of a shift stay
+                                   exact and the sweeps after the first reuse them */
 } sis_params;
 
 /* Result of one instance.  Replaces raw{n}.json (cuda/packing.cu:859-865). */
@@ -217,13 +220,14 @@ int sis_dev_cellcoord(int count, const float* x, float shift, float w, int lo, i
  * cuda/packing.cu:80,123 (walk, step).  Same bits as the oracle's FP32 form. */
 int sis_dev_accept_exp2(int count, const float* x, float* out);
 /* Run `sweeps` sweeps of the FP32 sweep kernel on R replicas of one instance,
- * in place: xycs = R*n*4 floats (x, y, cos, sin).  temps/steps per replica.
+ * in place: xycs = R*n*4 floats (x, y, cos, sin).  temps/steps per replica;
+ * the cell grid gets a new random shift every `sweeps_per_shift` sweeps.
  * Returns per-replica accepted moves and FP32 energies after the sweeps.
  * Semantics: cuda/packing.cu:52-95 (walk). */
 int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
                    const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
-                   uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,
-                   uint64_t counts[3]);
+                   uint32_t sweep0, int sweeps, int sweeps_per_shift, float* xycs, int* accepted,
+                   float* energy, uint64_t counts[3]);
 /* One sweep launch as above followed by `collective` collective proposals per
  * replica (cuda/packing.cu:564-616) and, when (interval + 1) is a multiple of
  * kick_every > 0, the kick (cuda/packing.cu:696-745; relocating on odd kick
@@ -231,8 +235,8 @@ int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* in
  * collective accepts, kicked replicas. */
 int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
-                  uint32_t sweep0, int sweeps, int collective, int kick_every, uint32_t interval,
-                  float* xycs, int* accepted, float* energy, uint64_t counts[6]);
+                  uint32_t sweep0, int sweeps, int sweeps_per_shift, int collective, int kick_every,
+                  uint32_t interval, float* xycs, int* accepted, float* energy, uint64_t counts[6]);
 /* The FP64 zero-overlap test that gates archiving and shrinking after each
  * exchange step (cuda/packing.cu:625-632, where every pair is summed): the
  * grid test used by the exchange kernel (zero = 1 when the configuration has

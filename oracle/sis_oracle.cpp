@@ -754,7 +754,10 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
     const uint32_t key[2] = {cfg->key0, cfg->key1};
     uint32_t o[4];
     {
-        uint32_t ctr[4] = {0u, cfg->replica, cfg->sweep, 1u};
+        /* blocks of k sweeps share one shift (a move never leaves its cell, so
+           the cell lists built here equal the ones the device keeps) */
+        const uint32_t k = cfg->sweeps_per_shift > 1 ? cfg->sweeps_per_shift : 1u;
+        uint32_t ctr[4] = {0u, cfg->replica, cfg->sweep / k * k, 1u};
         o_philox(ctr, key, o);
     }
     const float shx = w * o_u01f(o[0]), shy = w * o_u01f(o[1]);

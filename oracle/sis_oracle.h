@@ -90,6 +90,8 @@ typedef struct o_sweep_cfg {
     uint32_t key0, key1;     /* Philox key (request seed) */
     uint32_t replica;        /* replica index within the request */
     uint32_t sweep;          /* global sweep counter of this replica */
+    uint32_t sweeps_per_shift; /* the grid shift of sweep s is drawn at counter
+                                  (s / k) k (0 or 1: every sweep) */
 } o_sweep_cfg;
 
 /* x, y, c, s: n floats each, updated in place.  Returns accepted moves. */

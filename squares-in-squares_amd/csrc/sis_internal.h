@@ -29,6 +29,7 @@ struct InstDesc {
     int rep_off;                // first global replica / temperature-slot index
     int M;                      // cell-grid side (row stride), sized for the initial L
     int spe;                    // sweeps per exchange interval
+    int spsh;                   // sweeps per random shift of the cell grid
     long long sweeps_total;
     long long sweeps_done;
     double L;                   // current box side

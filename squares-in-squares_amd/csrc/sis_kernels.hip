@@ -964,6 +964,38 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
 // one tile.  One sweep = every square proposes exactly once.  NT threads: one
 // wavefront, or four for large instances (the same arithmetic either way).
 // ---------------------------------------------------------------------------
+// Moves of a sweep that keeps the cell lists of the previous one (same grid
+// shift): every square's staged move is drawn again from its current position,
+// either into prop / pw or into its mover record j (one-wavefront tiles with
+// records; square, CSR slot, replica, 1/T and stencil stay).
+__device__ __forceinline__ void stage_prop(const TileLds& t, int q, int n, int m, float inv_w, float L, int r0,
+                                           uint32_t sweep, uint32_t k0, uint32_t k1) {
+    const int g = rep_of(t, q), i = q - g * n;
+    const int ck = t.cellk[q];
+    const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
+    Move mv;
+    const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
+    t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
+    t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
+}
+__device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, int m, float inv_w, float L,
+                                               int r0, uint32_t sweep, uint32_t k0, uint32_t k1) {
+    uint4* w = t.rec + (size_t)j * kRecWords;
+    const uint4 w2 = w[2];
+    const int g = (int)w[4].y;
+    const int q = (int)(w2.w & 0xFFFFu);
+    const float4 p = t.pos[q];
+    const int ck = t.cellk[q];
+    const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
+    Move mv;
+    const bool ok = make_move(t, g, p, r, ck & 255, ck >> 8, m, inv_w, L, mv);
+    w[0] = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), __float_as_uint(p.w));
+    w[1] = ok ? make_uint4(__float_as_uint(mv.nx), __float_as_uint(mv.ny), __float_as_uint(mv.nc),
+                           __float_as_uint(mv.ns))
+              : make_uint4(0u, 0u, __float_as_uint(kNoMove), 0u);
+    w[2] = make_uint4(__float_as_uint(ok ? mv.wdiff : 0.0f), __float_as_uint(r.w), w2.z, w2.w);
+}
+
 // Sweep set-up of a one-wavefront tile of at most 64 squares, one square per
 // lane (q = lane): the binning of bin_tile, the group keys, the levels, the
 // level starts and the mover records of the general set-up in k_sweep, with
@@ -972,7 +1004,8 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
 // while the first reads are in flight.  Leaves the same LDS state and returns
 // the level starts (lane l: start of level l; lane kGroups + 1: the total).
 __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, float inv_w, float L, float w,
-                                  int r0, uint32_t sweep, bool next_shift, uint32_t k0, uint32_t k1) {
+                                  int r0, uint32_t sweep, bool next_shift, uint32_t next_sweep, uint32_t k0,
+                                  uint32_t k1) {
     const int lane = threadIdx.x;
     const bool has = lane < G * n;
     const int gw = grid_words(M);
@@ -1031,7 +1064,7 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
         atomicOr(&t.gmask[g], 1ull << key);
     }
     if (next_shift && lane < G) {
-        const uint4 o = philox(0u, (uint32_t)(r0 + lane), sweep + 1u, 1u, k0, k1);
+        const uint4 o = philox(0u, (uint32_t)(r0 + lane), next_sweep, 1u, k0, k1);
         t.shx_next[lane] = w * u01(o.x);
         t.shy_next[lane] = w * u01(o.y);
     }
@@ -1054,8 +1087,10 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
     return gs;
 }
 
+// Four waves per SIMD: at most 128 VGPRs (the tiles under load need the
+// residency more than the kernel needs registers).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(const Tile* __restrict__ tiles,
                                               InstDesc* __restrict__ insts,
                                               RepState* __restrict__ reps,
                                               const SlotState* __restrict__ slots,
@@ -1074,6 +1109,7 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
     const int ncoll = D->collective, kick_every = D->kick_every;
     const uint32_t interval = D->interval;
+    const uint32_t spsh = (uint32_t)max(1, D->spsh);
     TileLds t = carve(smem, n, M, G, tl.wide != 0);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
@@ -1090,20 +1126,36 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
         t.sth[g] = ss.step_th;
         t.ik[g] = ss.inv_t_log2e;
         t.acc[g] = 0;
-        // grid shift of the first sweep (later ones are drawn a sweep ahead)
-        const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep0, 1u, k0, k1);
+        // grid shift of the first sweep's block (blocks of spsh sweeps share
+        // one shift, drawn at the block's first sweep counter; the next
+        // block's is drawn ahead)
+        const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep0 / spsh * spsh, 1u, k0, k1);
         t.shx[g] = w * u01(o.x);
         t.shy[g] = w * u01(o.y);
     }
     tile_sync<NT>();
 
     unsigned long long cC = 0, cP = 0;
+    int gs = 0;  // level starts (lane l: level l), kept with the cell lists
     for (int sw = 0; sw < nsw; ++sw) {
         const uint32_t sweep = sweep0 + (uint32_t)sw;
-        int gs = 0;
-        if (NT == 64 && t.rec && G * n <= kSweepThreads) {
+        const uint32_t next_block = (sweep / spsh + 1u) * spsh;
+        const bool next_in_launch = (long long)next_block < (long long)sweep0 + nsw;
+        if (sw > 0 && sweep % spsh != 0u) {
+            // same grid shift as the previous sweep: no move left its cell, so
+            // the cell lists, ranks and levels still hold; only the moves are
+            // drawn again
+            if (NT == 64 && t.rec) {
+                const int items = __builtin_amdgcn_readlane(gs, kGroups);
+                for (int j = tid; j < items; j += NT) restage_record(t, j, n, m, inv_w, L, tl.r0, sweep, k0, k1);
+            } else if (t.prop) {
+                for (int q = tid; q < G * n; q += NT) stage_prop(t, q, n, m, inv_w, L, tl.r0, sweep, k0, k1);
+            }
+            tile_sync<NT>();
+        } else if (NT == 64 && t.rec && G * n <= kSweepThreads) {
             if constexpr (NT == 64)
-                gs = setup_lane_squares(t, n, G, M, m, inv_w, L, w, tl.r0, sweep, sw + 1 < nsw, k0, k1);
+                gs = setup_lane_squares(t, n, G, M, m, inv_w, L, w, tl.r0, sweep, next_in_launch, next_block, k0,
+                                        k1);
         } else {
             for (int k = tid; k < kGroups + 2; k += NT) t.gcnt[k] = 0;
             for (int g = tid; g < G; g += NT) t.gmask[g] = 0ull;
@@ -1115,25 +1167,18 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
             // runs), while the LDS atomics of the key pass are in flight; tiles
             // with mover records draw it when the records are written.
             for (int q = tid; q < G * n; q += NT) {
-                const int g = rep_of(t, q), i = q - g * n;
+                const int g = rep_of(t, q);
                 const int ck = t.cellk[q];
                 const int colour = ((ck & 255) % 3) + 3 * ((ck >> 8) % 3);
                 const int r = t.rank[q];
                 const int key = r < kRankCap ? r * 9 + colour : kGroups;
                 t.rank[q] = (uint16_t)key;
                 atomicOr(&t.gmask[g], 1ull << key);
-                if (t.prop) {
-                    const float4 r = move_variates((uint32_t)i, (uint32_t)(tl.r0 + g), sweep, k0, k1);
-                    Move mv;
-                    const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
-                    t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns)
-                                   : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
-                    t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
-                }
+                if (t.prop) stage_prop(t, q, n, m, inv_w, L, tl.r0, sweep, k0, k1);
             }
-            if (sw + 1 < nsw)
+            if (next_in_launch)
                 for (int g = tid; g < G; g += NT) {
-                    const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep + 1u, 1u, k0, k1);
+                    const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), next_block, 1u, k0, k1);
                     t.shx_next[g] = w * u01(o.x);
                     t.shy_next[g] = w * u01(o.y);
                 }
@@ -1205,12 +1250,14 @@ __global__ __launch_bounds__(NT) void k_sweep(const Tile* __restrict__ tiles,
                         propose<(NT > 64)>(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
             tile_sync<NT>();
         }
-        float* tmp = t.shx;  // the shift drawn during this sweep becomes current
-        t.shx = t.shx_next;
-        t.shx_next = tmp;
-        tmp = t.shy;
-        t.shy = t.shy_next;
-        t.shy_next = tmp;
+        if ((sweep + 1u) % spsh == 0u) {  // the next sweep starts a block: its shift becomes current
+            float* tmp = t.shx;
+            t.shx = t.shx_next;
+            t.shx_next = tmp;
+            tmp = t.shy;
+            t.shy = t.shy_next;
+            t.shy_next = tmp;
+        }
     }
 
     tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);

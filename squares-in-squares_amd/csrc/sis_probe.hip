@@ -60,11 +60,11 @@ int sis_dev_accept_exp2(int count, const float* x, float* out) {
 
 int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
                    const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
-                   uint32_t sweep0, int sweeps, float* xycs, int* accepted, float* energy,
-                   uint64_t counts[3]) {
+                   uint32_t sweep0, int sweeps, int sweeps_per_shift, float* xycs, int* accepted,
+                   float* energy, uint64_t counts[3]) {
     uint64_t c6[6] = {0, 0, 0, 0, 0, 0};
     const int rc = sis_dev_epoch(n, R, L, cell, group, inv_t_log2e, step_xy, step_th, key0, key1, sweep0,
-                                 sweeps, 0, 0, 0u, xycs, accepted, energy, c6);
+                                 sweeps, sweeps_per_shift, 0, 0, 0u, xycs, accepted, energy, c6);
     if (counts)
         for (int k = 0; k < 3; ++k) counts[k] = c6[k];
     return rc;
@@ -72,10 +72,10 @@ int sis_dev_sweeps(int n, int R, float L, float cell, int group, const float* in
 
 int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv_t_log2e,
                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1,
-                  uint32_t sweep0, int sweeps, int collective, int kick_every, uint32_t interval,
-                  float* xycs, int* accepted, float* energy, uint64_t counts[6]) {
+                  uint32_t sweep0, int sweeps, int sweeps_per_shift, int collective, int kick_every,
+                  uint32_t interval, float* xycs, int* accepted, float* energy, uint64_t counts[6]) {
     if (n < 1 || n > kMaxN || R < 1 || R > 4096 || !(L > 0 && L <= 64) || !(cell >= kMinCell) ||
-        sweeps < 1 || group < 0 || collective < 0 || collective > 64 || kick_every < 0 || !inv_t_log2e ||
+        sweeps < 1 || sweeps_per_shift < 1 || sweeps_per_shift > 100000 || group < 0 || collective < 0 || collective > 64 || kick_every < 0 || !inv_t_log2e ||
         !step_xy || !step_th || !xycs)
         return fail(SIS_EINVAL, "bad arguments");
     int rc = ensure_device(-1);
@@ -95,7 +95,8 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
             return fail(SIS_EINVAL, "centres must lie in [0, L]");
     InstDesc D{};
     D.n = n; D.R = R; D.ladder = R; D.G = G; D.state_off = 0; D.rep_off = 0; D.M = M;
-    D.spe = sweeps; D.sweeps_total = (long long)sweep0 + sweeps; D.sweeps_done = sweep0;
+    D.spe = sweeps; D.spsh = sweeps_per_shift;
+    D.sweeps_total = (long long)sweep0 + sweeps; D.sweeps_done = sweep0;
     D.L = L; D.L_arch = L; D.Lf = L; D.scale = 1.0f; D.w = cell; D.key0 = key0; D.key1 = key1;
     D.collective = collective; D.kick_every = kick_every; D.interval = interval;
     std::vector<Tile> tiles;

@@ -84,6 +84,7 @@ void default_params(sis_params* p) {
     p->replicas = 64;
     p->ladder = 0;
     p->sweeps_per_exchange = 10;
+    p->sweeps_per_shift = 4;
     p->replicas_per_group = 0;
     p->polish_iters = 200;
     p->init_mode = 0;
@@ -110,6 +111,8 @@ int validate(const sis_instance* in, const sis_params* p) {
     if (p->sweeps < 0 || p->sweeps > (1ll << 31)) return fail(SIS_EINVAL, "sweeps out of range");
     if (p->sweeps_per_exchange < 1 || p->sweeps_per_exchange > 100000)
         return fail(SIS_EINVAL, "sweeps_per_exchange out of range");
+    if (p->sweeps_per_shift < 1 || p->sweeps_per_shift > 100000)
+        return fail(SIS_EINVAL, "sweeps_per_shift out of range");
     if (p->replicas_per_group < 0 || p->replicas_per_group > 256)
         return fail(SIS_EINVAL, "replicas_per_group out of range");
     if (p->polish_iters < 0 || p->polish_iters > 100000) return fail(SIS_EINVAL, "polish_iters out of range");
@@ -325,7 +328,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         InstDesc& D = h_inst[k];
         D.n = s.n; D.R = s.p.replicas; D.ladder = pl.ladder; D.G = pl.G;
         D.state_off = (int)nsq; D.rep_off = (int)nrep; D.arch_off = (int)narch;
-        D.M = pl.M; D.spe = pl.spe;
+        D.M = pl.M; D.spe = pl.spe; D.spsh = specs[k].p.sweeps_per_shift;
         D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
         D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
         D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;

@@ -280,7 +280,7 @@ int Server::admit_one(Req* r, Lane& ln) {
     InstDesc D{};
     D.n = n; D.R = R; D.ladder = r->plan.ladder; D.G = r->plan.G;
     D.state_off = (int)r->sq_off; D.rep_off = (int)r->rep_off; D.arch_off = (int)r->arch_off;
-    D.M = r->plan.M; D.spe = r->plan.spe;
+    D.M = r->plan.M; D.spe = r->plan.spe; D.spsh = s.p.sweeps_per_shift;
     D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
     D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
     D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;

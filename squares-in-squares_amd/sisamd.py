@@ -47,6 +47,7 @@ class SisParams(ctypes.Structure):
         ("sweeps_per_exchange", ctypes.c_int), ("replicas_per_group", ctypes.c_int),
         ("polish_iters", ctypes.c_int), ("init_mode", ctypes.c_int), ("flags", ctypes.c_int),
         ("rng_seed", ctypes.c_uint64), ("collective_moves", ctypes.c_int), ("kick_every", ctypes.c_int),
+        ("sweeps_per_shift", ctypes.c_int),
     ]
 
 
@@ -122,10 +123,10 @@ def lib() -> ctypes.CDLL:
     L.sis_dev_accept_exp2.argtypes = [ctypes.c_int, fp, fp]
     L.sis_dev_sweeps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                  fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
-                                 fp, ip, fp, u64p]
+                                 ctypes.c_int, fp, ip, fp, u64p]
     L.sis_dev_epoch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                 fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
-                                ctypes.c_int, ctypes.c_int, ctypes.c_uint32, fp, ip, fp, u64p]
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, fp, ip, fp, u64p]
     L.sis_dev_polish.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
     L.sis_dev_zero64.argtypes = [ctypes.c_int, ctypes.c_double, fp, ip, dp]
     _lib = L
@@ -421,10 +422,11 @@ def dev_accept_exp2(x: np.ndarray) -> np.ndarray:
 
 def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, step_th, key0: int,
                key1: int, sweep0: int, sweeps: int, group: int = 1, collective: int = 0, kick_every: int = 0,
-               interval: int = 0):
-    """xycs: (R, n, 4) float32 (x, y, cos, sin); returns (state, accepted, energy, counts).  With
-    collective / kick_every the launch ends with collective moves and a kick (counts then has six
-    entries: proposals, candidates, near pairs, collective proposals, collective accepts, kicks)."""
+               interval: int = 0, shift_every: int = 4):
+    """xycs: (R, n, 4) float32 (x, y, cos, sin); returns (state, accepted, energy, counts).  The
+    cell grid gets a new random shift every `shift_every` sweeps.  With collective / kick_every the
+    launch ends with collective moves and a kick (counts then has six entries: proposals,
+    candidates, near pairs, collective proposals, collective accepts, kicks)."""
     R, n, _ = xycs.shape
     st = np.ascontiguousarray(xycs, dtype=np.float32).copy()
     ik = np.ascontiguousarray(np.broadcast_to(np.asarray(inv_t_log2e, np.float32), (R,)))
@@ -435,13 +437,13 @@ def dev_sweeps(xycs: np.ndarray, L: float, cell: float, inv_t_log2e, step_xy, st
     if collective or kick_every:
         counts = np.zeros(6, dtype=np.uint64)
         rc = lib().sis_dev_epoch(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
-                                 _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps, collective, kick_every,
-                                 interval, _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int),
+                                 _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps, shift_every, collective,
+                                 kick_every, interval, _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int),
                                  _ptr(en, ctypes.c_float), _ptr(counts, ctypes.c_uint64))
     else:
         counts = np.zeros(3, dtype=np.uint64)
         rc = lib().sis_dev_sweeps(n, R, L, cell, group, _ptr(ik, ctypes.c_float), _ptr(sx, ctypes.c_float),
-                                  _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps,
+                                  _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps, shift_every,
                                   _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int), _ptr(en, ctypes.c_float),
                                   _ptr(counts, ctypes.c_uint64))
     if rc:

@@ -30,7 +30,8 @@ class PtsaOut(ctypes.Structure):
 
 class SweepCfg(ctypes.Structure):
     _fields_ = [("n", i32), ("L", f32), ("w", f32), ("step_xy", f32), ("step_th", f32),
-                ("inv_t_log2e", f32), ("key0", u32), ("key1", u32), ("replica", u32), ("sweep", u32)]
+                ("inv_t_log2e", f32), ("key0", u32), ("key1", u32), ("replica", u32), ("sweep", u32),
+                ("sweeps_per_shift", u32)]
 
 
 def ptr(a, t):
@@ -115,8 +116,9 @@ class Oracle:
         it = self.lib.o_polish64(q.shape[0], ptr(q, f64), L, iters, ctypes.byref(e))
         return q, e.value, it
 
-    def sweep_model(self, x, y, c, s, L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep):
-        cfg = SweepCfg(len(x), L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep)
+    def sweep_model(self, x, y, c, s, L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep,
+                    shift_every=1):
+        cfg = SweepCfg(len(x), L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep, shift_every)
         counts = np.zeros(3, dtype=np.uint64)
         acc = self.lib.o_sweep32_model(ctypes.byref(cfg), ptr(x, f32), ptr(y, f32), ptr(c, f32), ptr(s, f32),
                                        ptr(counts, u64))

@@ -97,7 +97,7 @@ def _seeded_state(rng, R, n, L):
     return np.ascontiguousarray(st)
 
 
-def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
+def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps, shift_every=4):
     st = st.copy()
     R, n, _ = st.shape
     acc = np.zeros(R, np.int64)
@@ -106,7 +106,7 @@ def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps):
     for r in range(R):
         x, y, c, s = (np.ascontiguousarray(st[r, :, k]) for k in range(4))
         for sw in range(sweep0, sweep0 + sweeps):
-            a, cc = oracle.sweep_model(x, y, c, s, L, w, sxy[r], sth[r], ik[r], key0, key1, r, sw)
+            a, cc = oracle.sweep_model(x, y, c, s, L, w, sxy[r], sth[r], ik[r], key0, key1, r, sw, shift_every)
             acc[r] += a
             cnt += cc
         st[r, :, 0], st[r, :, 1], st[r, :, 2], st[r, :, 3] = x, y, c, s
@@ -143,6 +143,33 @@ def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     assert np.array_equal(dst, mst)
     assert np.array_equal(den, men)
     assert dcnt[0] == R * n * sweeps and np.array_equal(dcnt, mcnt)
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+@pytest.mark.parametrize("n,L,R,group", [(20, 6.0, 6, 3), (90, 11.0, 3, 1), (50, 9.0, 6, 3), (150, 14.0, 4, 2),
+                                         (300, 17.5, 4, 2)])
+def test_sweeps_match_cpu_model_for_any_shift_period(sis, oracle, n, L, R, group, k):
+    """Blocks of k sweeps share one grid shift; the sweeps after a block's
+    first keep its cell lists, ranks and levels and only draw new moves.  The
+    model builds everything from scratch every sweep with the same shift, so
+    equality holds only if no kept list went stale.  Seven sweeps from sweep 5
+    (a launch starting inside a block) on every set-up path: register set-up
+    with records (60 squares), LDS set-up with records (90), staged arrays in
+    one-wavefront tiles (150), four-wavefront tiles staged (300) and inline
+    (600)."""
+    rng = np.random.default_rng(500 + n + k)
+    st = _seeded_state(rng, R, n, L)
+    T = np.geomspace(1e-3, 1e-1, R)
+    ik = (LOG2E / T).astype(np.float32)
+    sxy = np.full(R, 0.12, np.float32)
+    sth = np.full(R, 0.1, np.float32)
+    say("sweeps shift period", n, L, R, group, k)
+    dst, dacc, den, dcnt = sis.dev_sweeps(st, L, 0.7075, ik, sxy, sth, 7, 0, 5, 7, group=group, shift_every=k)
+    mst, macc, men, mcnt = _model_run(oracle, st, L, 0.7075, ik, sxy, sth, 7, 0, 5, 7, shift_every=k)
+    assert np.array_equal(dacc, macc)
+    assert np.array_equal(dst, mst)
+    assert np.array_equal(den, men)
+    assert np.array_equal(dcnt, mcnt)
 
 
 @pytest.mark.parametrize("n,L,R,sweeps,group,w", [(1, 1.5, 4, 5, 1, 0.7075), (2, 2.5, 4, 5, 2, 0.7075),
