@@ -65,6 +65,23 @@ typedef struct sis_params {
     int sweeps_per_shift;       /* sweeps between random shifts of the cell grid, 1..100000: a
                                    move never leaves its cell, so the cell lists of a shift stay
                                    exact and the sweeps after the first reuse them */
+    int improve;                /* how the box side moves after an exchange interval:
+                                   0: when the coldest replica with FP32 energy exactly zero passes
+                                      the FP64 zero-overlap test, the box shrinks by `shrink` and
+                                      every centre is rescaled with it;
+                                   1: the reference's improvement path (cuda/packing.cu:254-297,
+                                      503-504, 831-848): every replica's FP64 certified side is
+                                      computed after each sweep launch; when the smallest beats the
+                                      best side by 1e-11, the best side moves to it, every
+                                      temperature slot gets the box best - shrink * mults[class]
+                                      (mults 0.02 .. 20 over eight classes by ladder position), the
+                                      coldest slot of every ladder (every slot of the first ladder
+                                      when the gain exceeds 5e-4) restarts from the best
+                                      configuration scaled to its box, and that interval does no
+                                      exchange */
+    int cool_period;            /* reheating cycle of cuda/packing.cu:516-517,756-758 in exchange
+                                   intervals: temperatures times 0.002^((k % P) / (P - 1)) and step
+                                   sizes times its square root in interval k; 0 = none (P >= 2) */
 } sis_params;
 
 /* Result of one instance.  Replaces raw{n}.json (cuda/packing.cu:859-865). */
@@ -92,6 +109,10 @@ typedef struct sis_result {
     int fp64_rejects;           /* zero-overlap configurations in FP32 that the FP64 test rejected */
     int kicks;                  /* replica kicks applied */
     uint64_t collective_tried, collective_accepted;
+    uint64_t feasible_checkpoints; /* improve = 1: replica checkpoints (end of a sweep launch) whose
+                                   configuration certified (dilation <= 1.008), the count the
+                                   reference reports as feasible_checkpoints (cuda/packing.cu:506-507,
+                                   689-693, 857); improve = 0: zero-overlap configurations archived */
 } sis_result;
 
 /* Server configuration for the continuous-batching request loop. */
@@ -244,9 +265,69 @@ int sis_dev_epoch(int n, int R, float L, float cell, int group, const float* inv
  * xycs: n float4 (x, y, cos, sin). */
 int sis_dev_zero64(int n, double L, const float* xycs, int* zero, double* e64);
 /* FP64 polish + certified side of one configuration (cuda/packing.cu:624-697):
- * q in/out n*3; writes the exported (translated, dilated) configuration. */
+ * q in/out n*3; writes the exported (translated, dilated) configuration: the
+ * certified side before and after the polish, whichever is smaller, as the
+ * reference keeps the smaller of init_cert and post_cert (:506-507, :689-693). */
 int sis_dev_polish(int n, double L, int iters, double* q, double* energy_before,
                    double* energy_after, double* lam, double* L_cert, double* q_export);
+
+/* ---- probes of the exchange step and of a resident batch's state ---- */
+
+/* Per-replica record of the engine (the reference keeps energies[b] and
+ * best_Ls[b] per chain, cuda/packing.cu:479, 506). */
+typedef struct sis_rep_state {
+    float E;                    /* FP32 energy at the end of the last sweep launch */
+    int slot;                   /* temperature slot held */
+    unsigned acc;               /* accepted moves in the last interval */
+    unsigned pad;
+    double Lc;                  /* improve = 1: FP64 certified side of the configuration at the end of
+                                   the last launch (1e9 when the dilation would exceed 1.008) */
+    double lam;                 /* its centre dilation */
+    double minx, miny;          /* lower corner of the dilated configuration's bounding box */
+} sis_rep_state;
+
+/* Per temperature slot.  The sweep reads the first four fields. */
+typedef struct sis_slot_state {
+    float inv_t_log2e;          /* log2(e) / (T cool) of the current interval */
+    float step_xy, step_th;     /* step half-widths of the current interval: base x sqrt(cool) */
+    float L;                    /* box side of the slot */
+    float T;                    /* ladder temperature */
+    float base_xy, base_th;     /* adapted step half-widths */
+    float pad;
+} sis_slot_state;
+
+/* Instance fields the exchange step reads and writes. */
+typedef struct sis_ladder_desc {
+    int n, R, ladder, improve, cool_period, spe;
+    int64_t sweeps_total, sweeps_done;
+    uint32_t key0, key1, interval;
+    float shrink, target_acc, scale, Lf;
+    double L, L_arch, L_box_arch;
+    int arch_valid, shrinks, feasible, fp64_rejects;
+    uint64_t ex_tried, ex_acc, certs;
+} sis_ladder_desc;
+
+/* One exchange step (the kernel run after every sweep launch) on caller data:
+ * step adaptation, replica exchange (cuda/packing.cu:141-152, 750-765) or the
+ * improvement path (improve = 1, :831-848), the FP64 zero-overlap shrink
+ * (improve = 0), the cooling cycle.  reps, slots, rep_of_slot: R each;
+ * xycs: R*n*4 floats (x, y, cos, sin), arch: n*4; all updated in place. */
+int sis_dev_ladder(sis_ladder_desc* d, sis_rep_state* reps, sis_slot_state* slots, int* rep_of_slot,
+                   float* xycs, float* arch);
+/* The device state of instance k of a resident batch (sis_batch_create):
+ * descriptor, replica and slot records, slot owners, replica states (R*n*4
+ * floats) and archive (n*4).  Any output may be NULL. */
+int sis_dev_batch_state(sis_batch* b, int k, sis_ladder_desc* d, sis_rep_state* reps, sis_slot_state* slots,
+                        int* rep_of_slot, float* xycs, float* arch);
+/* Sweep launch of R replicas of one instance as sis_dev_epoch, with a box per
+ * replica (box[r] <= Lgrid, the side the cell grid is sized for) and, when
+ * certify is 1, every replica's FP64 certified side at the end (Lc, lam,
+ * minx, miny of sis_rep_state; pass reps, R records, or NULL). */
+int sis_dev_launch(int n, int R, float Lgrid, const float* box, float cell, int group, const float* inv_t_log2e,
+                   const float* step_xy, const float* step_th, uint32_t key0, uint32_t key1, uint32_t sweep0,
+                   int sweeps, int sweeps_per_shift, int collective, int kick_every, uint32_t interval,
+                   int certify, float* xycs, int* accepted, float* energy, uint64_t counts[6],
+                   sis_rep_state* reps);
 
 #ifdef __cplusplus
 }

@@ -17,12 +17,24 @@
  *  - certified side: every emitted configuration passes the reference's
  *    interval certifier (oracle/_ref); lambda >= 1 and L >= sqrt(n) - 1e-7.
  *  - CG polish: energy never increases (accepts only strictly lower energy).
- *  - PT-SA engine: forced-exchange invariant, detailed-balance histogram on a
- *    two-level toy, shrink bookkeeping; trajectories are parity unpinned
- *    against the reference (cuRAND streams cannot be reproduced).
+ *  - PT-SA engine (CPU baseline): forced-exchange invariant and shrink
+ *    bookkeeping; trajectories are parity unpinned against the reference
+ *    (cuRAND streams cannot be reproduced).
  *  - sweep model (o_sweep32_model): a CPU restatement of the product's own
  *    colour-checkerboard sweep; pinned by FP64 recomputation of every
- *    accepted move's energy change and by exhaustive-neighbour checks.
+ *    proposal's energy change (1e-5 relative above an FP32 floor of 2e-6 x
+ *    the move's overlap depths), by the stencil energy against all pairs,
+ *    and by detailed balance: one square in a walled box samples
+ *    exp(-E/T) (mean energy and position histogram against quadrature,
+ *    cell lists kept for 1 and 4 sweeps).
+ *  - exchange step (o_ladder_model): the fixed FP64 2^x against pow within
+ *    2 ulp; the exchange criterion by hand cases that a sign error fails and
+ *    by the stationary distribution of the slot permutation on a four-level
+ *    toy (detailed balance of parallel tempering); pairing inside ladders;
+ *    adaptation by its closed form; the improvement path and zero-overlap
+ *    shrink by their bookkeeping.
+ *  - certified side of an FP32 state (o_certify_xycs): the reference's
+ *    all-pairs rule on orientations scaled to unit length.
  */
 #include "sis_oracle.h"
 
@@ -709,7 +721,8 @@ int o_ptsa_run(const o_ptsa_cfg* cfg, o_ptsa_out* out) {
 /* width w >= sqrt(2)/2 (Anderson et al., "Scalable Metropolis Monte Carlo   */
 /* for simulation of hard shapes", Comput. Phys. Commun. 204 (2016) 21-30):  */
 /* moves that leave their cell are rejected, so same-colour cells never      */
-/* interact, and the grid origin is shifted at random every sweep.           */
+/* interact, and the grid origin is shifted at random every k sweeps         */
+/* (sweeps_per_shift).                                                       */
 /* ------------------------------------------------------------------------ */
 namespace {
 
@@ -743,6 +756,27 @@ void build_grid(Grid& g, int n, float L, float w, float shx, float shy, const fl
         g.rank[i] = fill[g.cell[i]]++;
         g.sorted[g.start[g.cell[i]] + g.rank[i]] = i;
     }
+}
+
+/* FP64 energy of square i at pose (px, py, pc, ps) against every other
+ * square and the walls of a box of side L, from the FP32 state converted
+ * exactly; *depth accumulates the overlap depths of the nonzero terms. */
+double move_energy64(int n, const float* x, const float* y, const float* c, const float* s, int i, float px,
+                     float py, float pc, float ps, float L, double* depth) {
+    double e = wall64_cs(px, py, pc, ps, L);
+    const double r = .5 * (fabs((double)pc) + fabs((double)ps));
+    const double v[4] = {r - px, px + r - L, r - py, py + r - L};
+    for (double t : v)
+        if (t > 0) *depth += t;
+    const double pi[3] = {px, py, 0.0};
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double pj[3] = {x[j], y[j], 0.0};
+        const double t = pair64_impl(pi, pj, pc, ps, c[j], s[j], nullptr, false);
+        e += t;
+        *depth += sqrt(t);
+    }
+    return e;
 }
 
 }  // namespace
@@ -781,6 +815,8 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
             const float ft = fmaf(2.0f, o_u01f(o[2]), -1.0f);
             const float nx = x[i] + cfg->step_xy * fx;
             const float ny = y[i] + cfg->step_xy * fy;
+            if (cfg->dE32) cfg->dE32[i] = NAN;
+            if (cfg->dE64) cfg->dE64[i] = NAN;
             if (!(nx >= 0.0f && nx <= L && ny >= 0.0f && ny <= L)) continue;
             if (o_cellcoord(nx, g.shx, w, 0, g.m - 1) + 2 != g.kx[i] ||
                 o_cellcoord(ny, g.shy, w, 0, g.m - 1) + 2 != g.ky[i])
@@ -808,6 +844,14 @@ int o_sweep32_model(const o_sweep_cfg* cfg, float* x, float* y, float* c, float*
                 }
             }
             const float dE = wdiff + ((part[0] + part[1]) + (part[2] + part[3]));
+            if (cfg->dE32) cfg->dE32[i] = dE;
+            if (cfg->dE64) {
+                double dn = 0.0, d_o = 0.0;
+                const double en = move_energy64(n, x, y, c, s, i, nx, ny, nc, ns, L, &dn);
+                const double eo = move_energy64(n, x, y, c, s, i, x[i], y[i], c[i], s[i], L, &d_o);
+                cfg->dE64[i] = en - eo;
+                if (cfg->depth64) cfg->depth64[i] = dn + d_o;
+            }
             const float u = o_u01f(o[3]);
             if (dE <= 0.0f || u < o_accept_exp2(-dE * cfg->inv_t_log2e)) {
                 x[i] = nx; y[i] = ny; c[i] = nc; s[i] = ns;
@@ -1009,4 +1053,235 @@ int o_grid_packing(int n, double* q) {
         q[3 * i + 2] = 0.0;
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Certified side of an FP32 state (cuda/packing.cu:254-297).  The engine   */
+/* keeps orientations as FP32 (cos, sin); they are scaled to unit length in */
+/* FP64 first.  All pairs i < j nearer than sqrt(2.05), as the reference.   */
+/* ------------------------------------------------------------------------ */
+namespace {
+inline void unit64(float c, float s, double& cd, double& sd) {
+    const double x = c, y = s;
+    const double h = 1.0 / sqrt(x * x + y * y);
+    cd = x * h;
+    sd = y * h;
+}
+}  // namespace
+
+void o_certify_xycs(int n, const float* q, double out[4]) {
+    const double tau = 2.6e-8;
+    std::vector<double> c(n), s(n);
+    for (int i = 0; i < n; ++i) unit64(q[4 * i + 2], q[4 * i + 3], c[i], s[i]);
+    double lam = 1.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = (double)q[4 * i] - (double)q[4 * j], dy = (double)q[4 * i + 1] - (double)q[4 * j + 1];
+            if (dx * dx + dy * dy >= 2.05) continue;
+            const double r = 0.5 + 0.5 * (fabs(c[i] * c[j] + s[i] * s[j]) + fabs(-s[i] * c[j] + c[i] * s[j]));
+            const double v1 = fabs(dx * c[i] + dy * s[i]), v2 = fabs(-dx * s[i] + dy * c[i]);
+            const double v3 = fabs(dx * c[j] + dy * s[j]), v4 = fabs(-dx * s[j] + dy * c[j]);
+            const double vmax = std::max(std::max(v1, v2), std::max(v3, v4));
+            const double req = vmax > 1e-12 ? (r + tau) / vmax : 1e9;
+            if (req > lam) lam = req;
+        }
+    double lox = 1e9, hix = -1e9, loy = 1e9, hiy = -1e9;
+    for (int i = 0; i < n; ++i) {
+        const double r = 0.5 * (fabs(c[i]) + fabs(s[i]));
+        const double x = (double)q[4 * i] * lam, y = (double)q[4 * i + 1] * lam;
+        lox = std::min(lox, x - r); hix = std::max(hix, x + r);
+        loy = std::min(loy, y - r); hiy = std::max(hiy, y + r);
+    }
+    out[0] = lam > 1.008 ? 1e9 : std::max(hix - lox, hiy - loy);
+    out[1] = lam;
+    out[2] = lox;
+    out[3] = loy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The exchange step: the engine's kernel run after every sweep launch,     */
+/* restated on the host.  Reference semantics: acceptance adaptation        */
+/* (cuda/packing.cu:618), replica exchange on adjacent temperatures of one  */
+/* ladder with log u < (1/Ta - 1/Tb)(Ea - Eb) (:141-152, :750-765), the     */
+/* reheating cycle (:516-517, :756-758) and the improvement path of the     */
+/* host loop (:503-504, :831-851).  The engine swaps temperature slots      */
+/* instead of coordinates, tests u < 2^x with x in log2 units and 2^x the   */
+/* FP32 sequence of o_accept_exp2, and adapts by 2^((rate - target)         */
+/* log2(e) / 2) with the same sequence.                                     */
+/* ------------------------------------------------------------------------ */
+/* 2^x in FP64: k = rint(x), f = x - k, the Taylor polynomial of 2^f of     */
+/* degree 13 (coefficients (ln 2)^j / j!) by fused multiply-adds, ldexp.    */
+double o_exp2_fixed64(double x) {
+    const double k = rint(x);
+    const double f = x - k;
+    double p = 1.3691488853904128e-12;
+    p = fma(p, f, 2.5678435993488206e-11);
+    p = fma(p, f, 4.4455382718708116e-10);
+    p = fma(p, f, 7.054911620801123e-09);
+    p = fma(p, f, 1.01780860092397e-07);
+    p = fma(p, f, 1.321548679014431e-06);
+    p = fma(p, f, 1.5252733804059841e-05);
+    p = fma(p, f, 0.0001540353039338161);
+    p = fma(p, f, 0.0013333558146428443);
+    p = fma(p, f, 0.009618129107628477);
+    p = fma(p, f, 0.05550410866482158);
+    p = fma(p, f, 0.24022650695910072);
+    p = fma(p, f, 0.6931471805599453);
+    p = fma(p, f, 1.0);
+    return ldexp(p, (int)k);
+}
+
+void o_cool_factor(uint32_t k, int P, double* cool, double* root) {
+    if (P < 2) {
+        *cool = 1.0;
+        *root = 1.0;
+        return;
+    }
+    /* log2(0.002) = -8.965784284662087 */
+    const double x = (double)(k % (uint32_t)P) / (double)(P - 1) * -8.965784284662087;
+    *cool = o_exp2_fixed64(x);
+    *root = o_exp2_fixed64(0.5 * x);
+}
+
+int o_ladder_model(o_ladder_desc* d, o_rep_state* reps, o_slot_state* slots, int* ros, float* xycs,
+                   float* arch) {
+    static const double mults[8] = {0.02, 0.1, 0.4, 1.0, 2.5, 5.0, 10.0, 20.0};
+    const int n = d->n, R = d->R, lad = d->ladder;
+    const int64_t left = d->sweeps_total - d->sweeps_done;
+    if (left <= 0) return 0;
+    const int ran = (int)std::min<int64_t>(d->spe, left);
+    std::vector<float> eS(R), kS(R);
+    std::vector<int> rS(R);
+    uint64_t best_lc = ~0ull;
+    /* 1. adaptation of every slot from its replica's acceptance */
+    for (int r = 0; r < R; ++r) {
+        const o_rep_state& rs = reps[r];
+        const float rate = (float)rs.acc / (float)((int64_t)n * ran);
+        const float f = o_accept_exp2((rate - d->target_acc) * 0.72134752f);
+        o_slot_state& sl = slots[rs.slot];
+        eS[rs.slot] = rs.E;
+        kS[rs.slot] = sl.inv_t_log2e;
+        rS[rs.slot] = r;
+        sl.base_xy = fminf(fmaxf(sl.base_xy * f, 1.7320508e-7f), 0.8660254f);
+        sl.base_th = fminf(fmaxf(sl.base_th * f, 1.7320508e-7f), 0.7794229f);
+        uint64_t bits;
+        memcpy(&bits, &rs.Lc, 8);
+        if (d->improve && bits < best_lc) best_lc = bits;
+    }
+    const double L0 = d->L, lmin = sqrt((double)n) + 1e-6;
+    bool improved = false;
+    /* 2. improvement path */
+    if (d->improve) {
+        double lc;
+        memcpy(&lc, &best_lc, 8);
+        improved = lc < L0 - 1e-11;
+        if (improved) {
+            int rb = 0;
+            for (int r = 0; r < R; ++r) {
+                uint64_t bits;
+                memcpy(&bits, &reps[r].Lc, 8);
+                if (bits == best_lc) { rb = r; break; }
+            }
+            const double Lb = reps[rb].Lc, lam = reps[rb].lam, mx = reps[rb].minx, my = reps[rb].miny;
+            const float box_b = slots[reps[rb].slot].L;
+            memcpy(arch, xycs + (size_t)4 * rb * n, sizeof(float) * 4 * n);
+            const double shrink = (double)d->shrink;
+            for (int s = 0; s < R; ++s)
+                slots[s].L = (float)std::max(lmin, Lb - shrink * mults[((s % lad) * 8) / lad]);
+            const int heads = R / lad, extra = (L0 - Lb > 5e-4) ? lad - 1 : 0;
+            for (int h = 0; h < heads + extra; ++h) {
+                const int s = h < heads ? h * lad : h - heads + 1;
+                const double tL = std::max(lmin, Lb - shrink * mults[((s % lad) * 8) / lad]);
+                const double sc = tL / Lb;
+                float* dst = xycs + (size_t)4 * rS[s] * n;
+                for (int i = 0; i < n; ++i) {
+                    dst[4 * i] = (float)(((double)arch[4 * i] * lam - mx) * sc);
+                    dst[4 * i + 1] = (float)(((double)arch[4 * i + 1] * lam - my) * sc);
+                    dst[4 * i + 2] = arch[4 * i + 2];
+                    dst[4 * i + 3] = arch[4 * i + 3];
+                }
+            }
+            d->L = Lb;
+            d->Lf = (float)Lb;
+            d->L_arch = Lb;
+            d->L_box_arch = (double)box_b;
+            d->arch_valid = 1;
+            d->feasible += 1;
+            d->shrinks += 1;
+        }
+    }
+    /* 3. even/odd exchange inside each ladder */
+    if (!improved) {
+        const int parity = (int)(d->interval & 1u);
+        const uint32_t key[2] = {d->key0, d->key1};
+        for (int s = parity; s + 1 < R; s += 2) {
+            if ((s % lad) + 1 >= lad) continue;
+            const int ra = rS[s], rb = rS[s + 1];
+            const float Ea = eS[s], Eb = eS[s + 1];
+            const float x = (kS[s] - kS[s + 1]) * (Ea - Eb);
+            uint32_t ctr[4] = {(uint32_t)s, 0u, d->interval, 2u}, o[4];
+            o_philox(ctr, key, o);
+            d->ex_tried++;
+            if (x >= 0.0f || o_u01f(o[0]) < o_accept_exp2(x)) {
+                rS[s] = rb; rS[s + 1] = ra;
+                eS[s] = Eb; eS[s + 1] = Ea;
+                ros[s] = rb; ros[s + 1] = ra;
+                reps[ra].slot = s + 1;
+                reps[rb].slot = s;
+                d->ex_acc++;
+            }
+        }
+    }
+    /* 4. improve = 0: FP64 zero-overlap test of the coldest zero-energy replica */
+    double scale = 1.0, Ln = L0;
+    bool archive = false;
+    if (!d->improve) {
+        int zero = -1;
+        for (int s = 0; s < R && zero < 0; ++s)
+            if (eS[s] == 0.0f) zero = s;
+        if (zero >= 0) {
+            const int r = rS[zero];
+            const float* src = xycs + (size_t)4 * r * n;
+            std::vector<double> x(n), y(n), c(n), s(n);
+            for (int i = 0; i < n; ++i) {
+                x[i] = src[4 * i];
+                y[i] = src[4 * i + 1];
+                unit64(src[4 * i + 2], src[4 * i + 3], c[i], s[i]);
+            }
+            if (o_total64_cs(n, x.data(), y.data(), c.data(), s.data(), L0) != 0.0) {
+                d->fp64_rejects += 1;
+            } else {
+                archive = !d->arch_valid || L0 < d->L_arch;
+                if (archive) memcpy(arch, src, sizeof(float) * 4 * n);
+                Ln = std::max(L0 - (double)d->shrink, sqrt((double)n));
+                if (Ln < L0) scale = Ln / L0;
+            }
+        }
+        if (Ln < L0)
+            for (int s = 0; s < R; ++s) slots[s].L = (float)Ln;
+    }
+    /* 5. next interval's temperatures and step sizes */
+    double cool, root;
+    o_cool_factor(d->interval + 1u, d->cool_period, &cool, &root);
+    for (int s = 0; s < R; ++s) {
+        o_slot_state& sl = slots[s];
+        sl.inv_t_log2e = (float)(1.4426950408889634 / ((double)sl.T * cool));
+        sl.step_xy = (float)((double)sl.base_xy * root);
+        sl.step_th = (float)((double)sl.base_th * root);
+    }
+    if (archive) {
+        d->arch_valid = 1;
+        d->L_arch = L0;
+        d->L_box_arch = L0;
+        d->feasible += 1;
+    }
+    if (Ln < L0) {
+        d->L = Ln;
+        d->Lf = (float)Ln;
+        d->shrinks += 1;
+    }
+    d->scale = (float)scale;
+    d->sweeps_done += ran;
+    d->interval += 1u;
+    return improved ? 1 : 0;
 }

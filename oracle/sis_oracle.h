@@ -92,6 +92,13 @@ typedef struct o_sweep_cfg {
     uint32_t sweep;          /* global sweep counter of this replica */
     uint32_t sweeps_per_shift; /* the grid shift of sweep s is drawn at counter
                                   (s / k) k (0 or 1: every sweep) */
+    float* dE32;             /* optional, n: the FP32 energy change of square i's proposal this
+                                sweep (NaN when rejected before the stencil: it leaves the box
+                                or its cell) */
+    double* dE64;            /* optional, n: the same change recomputed in FP64 over every pair
+                                and the wall, from the same FP32 inputs */
+    double* depth64;         /* optional, n: the summed overlap depths of the pairs and wall
+                                terms of that change (old and new pose), in FP64 */
 } o_sweep_cfg;
 
 /* x, y, c, s: n floats each, updated in place.  Returns accepted moves. */
@@ -110,6 +117,38 @@ int o_collective32_model(const o_sweep_cfg* cfg, int moves, float* x, float* y, 
  * the replica had energy (and was kicked); *energy = canonical energy after. */
 int o_kick32_model(const o_sweep_cfg* cfg, uint32_t interval, int relocate, float* x, float* y, float* c,
                    float* s, float* energy);
+
+/* ---- certified side of an FP32 state (cuda/packing.cu:254-297) ---- */
+/* xycs: n float4 (x, y, cos, sin); the orientation is scaled to unit length
+ * in FP64.  out: side (1e9 when the dilation exceeds 1.008), dilation, lower
+ * corner x, y of the dilated configuration. */
+void o_certify_xycs(int n, const float* xycs, double out[4]);
+
+/* ---- the exchange step (cuda/packing.cu:141-152, 750-765, 823-853) ---- */
+/* Same layouts as the engine's sis_rep_state / sis_slot_state / sis_ladder_desc. */
+typedef struct o_rep_state {
+    float E; int slot; unsigned acc; unsigned pad;
+    double Lc, lam, minx, miny;
+} o_rep_state;
+typedef struct o_slot_state {
+    float inv_t_log2e, step_xy, step_th, L, T, base_xy, base_th, pad;
+} o_slot_state;
+typedef struct o_ladder_desc {
+    int n, R, ladder, improve, cool_period, spe;
+    int64_t sweeps_total, sweeps_done;
+    uint32_t key0, key1, interval;
+    float shrink, target_acc, scale, Lf;
+    double L, L_arch, L_box_arch;
+    int arch_valid, shrinks, feasible, fp64_rejects;
+    uint64_t ex_tried, ex_acc, certs;
+} o_ladder_desc;
+/* One exchange step of one instance, in place (xycs: R*n*4, arch: n*4). */
+int o_ladder_model(o_ladder_desc* d, o_rep_state* reps, o_slot_state* slots, int* rep_of_slot, float* xycs,
+                   float* arch);
+/* 2^x as the fixed FP64 sequence of the exchange step, and the reheating
+ * factor 0.002^((k mod P) / (P - 1)) with its square root. */
+double o_exp2_fixed64(double x);
+void o_cool_factor(uint32_t k, int P, double* cool, double* root);
 
 /* ---- analytic fixtures (cuda/scheduler.py:25-74, src/squares/core.py:27-35) ---- */
 int o_grid_packing(int n, double* q);

@@ -133,6 +133,12 @@ void portfolio_state(const InstSpec& s, bool jitter, std::vector<double>& q);
 void portfolio_layouts(int n, int R, int ladder, double seed_L, const double* seed, std::vector<double>& out);
 double portfolio_jitter(int cls, double u);
 int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
+// Initial device records of one instance (the caller sets the descriptor's
+// state, replica and archive offsets).
+void instance_records(const InstSpec& s, const Plan& pl, InstDesc& D, std::vector<RepState>& reps,
+                      std::vector<SlotState>& slots, std::vector<int>& ros);
+double slot_box(const InstSpec& s, int ladder, int slot, double Lb);
+void finalize_source(const InstDesc& D, double& box, double& target_L);
 // A launch set of this many replicas keeps every compute unit busy (eight per
 // compute unit): below it tiles hold one replica each and stage every size.
 bool device_loaded(long long replicas);

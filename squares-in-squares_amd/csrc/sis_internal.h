@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/sis_abi.h"
+
 namespace sis {
 
 constexpr int kMaxN = 1000;
@@ -32,9 +34,10 @@ struct InstDesc {
     int spsh;                   // sweeps per random shift of the cell grid
     long long sweeps_total;
     long long sweeps_done;
-    double L;                   // current box side
-    double L_arch;              // box side of the archived zero-overlap configuration
-    float Lf;                   // L rounded to float, used by the sweep kernel
+    double L;                   // improve 0: current box side; 1: best certified side so far
+    double L_arch;              // improve 0: box side of the archived zero-overlap configuration;
+                                // 1: certified side of the archived configuration
+    float Lf;                   // L rounded to float: the cell grid covers [0, Lf]; every slot box is <= Lf
     float scale;                // centre scale applied when the next sweep stages the state
     float w;                    // cell width
     float shrink;
@@ -49,8 +52,14 @@ struct InstDesc {
     int collective;             // collective proposals per replica after each launch's sweeps
     int kick_every;             // intervals between kicks (0: none)
     int kicks;                  // kicks applied (counted once per replica)
+    int improve;                // 0: shrink on a zero-overlap replica; 1: certified-side improvement path
+    int cool_period;            // reheating cycle in intervals (0: none)
+    int pad1;
+    double L_box_arch;          // box side the archived configuration was in
+    double Lmin;                // sqrt(n): no box goes below it
     unsigned long long proposals, accepted, cand, near, ex_tried, ex_acc;
     unsigned long long coll_tried, coll_acc;
+    unsigned long long certs;   // improve = 1: replica checkpoints with a certified side (dilation <= 1.008)
 };
 
 // Per-replica scratch of the collective moves and the kick (LDS).
@@ -66,19 +75,63 @@ struct Tile {
     int wide;                   // stage proposals up to kMaxN squares (sweep_prop_in_lds)
 };
 
-struct RepState {               // per global replica
-    float E;                    // FP32 energy after the last sweep launch
-    int slot;                   // temperature slot held (0..R-1 within the instance)
-    unsigned acc;               // accepted moves in the last interval
-    unsigned pad;
-};
+// Per global replica and per global temperature slot: the records of the C
+// ABI's exchange-step probe, used as they are.
+using RepState = sis_rep_state;
+using SlotState = sis_slot_state;
+static_assert(sizeof(RepState) == 48 && sizeof(SlotState) == 32, "record layouts");
 
-struct SlotState {              // per global temperature slot
-    float inv_t_log2e;          // log2(e) / T
-    float step_xy;              // half-width of the uniform translation step
-    float step_th;              // half-width of the uniform rotation step
-    float T;
-};
+// The reference's retarget multipliers by temperature class (cuda/packing.cu:503, 841).
+constexpr double kTargetMults[8] = {0.02, 0.1, 0.4, 1.0, 2.5, 5.0, 10.0, 20.0};
+// Certified side (cuda/packing.cu:254-297): pairs nearer than sqrt(2.05), margin
+// tau, dilations above 1.008 rejected (side 1e9).
+constexpr double kCertTau = 2.6e-8;
+constexpr double kCertNear2 = 2.05;
+constexpr double kCertMaxLam = 1.008;
+constexpr double kCertNone = 1e9;
+// 2^x in FP64 operations that each round once or not at all (round to
+// nearest integer, an exact subtraction, a Horner chain of fused
+// multiply-adds over the Taylor coefficients (ln 2)^j / j!, j <= 13, an exact
+// power-of-two scaling), so the host, the device and the oracle compute the
+// same bits.  Relative error below 1e-16 for |x| < 1000.
+__host__ __device__ inline double exp2_fixed64(double x) {
+    const double k = rint(x);
+    const double f = x - k;
+    double p = 1.3691488853904128e-12;
+    p = fma(p, f, 2.5678435993488206e-11);
+    p = fma(p, f, 4.4455382718708116e-10);
+    p = fma(p, f, 7.054911620801123e-09);
+    p = fma(p, f, 1.01780860092397e-07);
+    p = fma(p, f, 1.321548679014431e-06);
+    p = fma(p, f, 1.5252733804059841e-05);
+    p = fma(p, f, 0.0001540353039338161);
+    p = fma(p, f, 0.0013333558146428443);
+    p = fma(p, f, 0.009618129107628477);
+    p = fma(p, f, 0.05550410866482158);
+    p = fma(p, f, 0.24022650695910072);
+    p = fma(p, f, 0.6931471805599453);
+    p = fma(p, f, 1.0);
+    return ldexp(p, (int)k);
+}
+// Reheating factor of interval k (cuda/packing.cu:516): 0.002^((k mod P) /
+// (P - 1)), and its square root for the step sizes (:523); 1 without a cycle.
+__host__ __device__ inline void cool_factor(uint32_t k, int P, double& cool, double& root) {
+    if (P < 2) {
+        cool = 1.0;
+        root = 1.0;
+        return;
+    }
+    const double x = (double)(k % (uint32_t)P) / (double)(P - 1) * -8.965784284662087;  // log2(0.002)
+    cool = exp2_fixed64(x);
+    root = exp2_fixed64(0.5 * x);
+}
+// Effective slot values of an interval: 1/T in log2 units and the step
+// half-widths, from the ladder temperature and the adapted bases.
+__host__ __device__ inline void slot_effective(SlotState& s, double cool, double root) {
+    s.inv_t_log2e = (float)(1.4426950408889634 / ((double)s.T * cool));
+    s.step_xy = (float)((double)s.base_xy * root);
+    s.step_th = (float)((double)s.base_th * root);
+}
 
 // LDS of one sweep tile (G replicas of n squares, M x M cells).
 // Cell starts per replica: M*M + 2 u16, padded to 16 bytes.
@@ -129,7 +182,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G, bool wide
     b = (b + 15) & ~(size_t)15;
     b += (size_t)G * 8;                         // per-replica occupied-group mask
     b += (size_t)G * sizeof(CollWork);          // collective-move / kick scratch
-    b += (size_t)G * 9 * 4;                     // shift x/y (this and next sweep), steps, 1/T, accepts, energy
+    b += (size_t)G * 10 * 4;                    // shift x/y (this and next sweep), steps, 1/T, box, accepts, energy
     b += 2 * (kGroups + 2) * 4;                 // level counts / cursors, level starts
     return b;
 }

@@ -7,7 +7,7 @@ namespace sis {
 struct FinalizeJob {
     int n;
     int iters;                  // FP64 CG polish iterations
-    double L;                   // box side the source configuration was feasible in
+    double L;                   // box side the polish runs in
     const float4* src32;        // n (x, y, cos, sin) or null
     const double* src64;        // n*3 (x, y, angle) when src32 is null
     long long out_off;          // double index into q_out
@@ -22,6 +22,8 @@ struct FinalizeOut {
     int iters;
     int best;                   // replica taken when the job named replicas
     double e32;                 // its FP32 energy
+    int pre;                    // 1: the export is the configuration before the polish
+    int pad;
 };
 
 // threads: kSweepThreads for instances below kBigN squares, kSweepThreadsBig

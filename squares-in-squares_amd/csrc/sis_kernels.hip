@@ -3,9 +3,12 @@
 //  k_sweep     FP32 Metropolis sweeps of replica tiles staged in LDS, colour
 //              checkerboard of cells so that non-interacting squares move at
 //              once (reference semantics: cuda/packing.cu:52-95, `walk`).
+//              Optionally ends with every replica's FP64 certified side.
 //  k_ladder    per instance: step adaptation, replica exchange by swapping
-//              temperature slots, FP64 zero-overlap test, archive and box
-//              shrink (cuda/packing.cu:141-152, 750-765, 823-853).
+//              temperature slots, then either the FP64 zero-overlap test and
+//              box shrink or the certified-side improvement path, and the
+//              reheating cycle (cuda/packing.cu:141-152, 503-517, 750-765,
+//              823-853).
 //  k_finalize  FP64 conjugate-gradient polish, certified-side dilation and
 //              export of one configuration per instance (cuda/packing.cu:
 //              254-297, 624-697).
@@ -251,6 +254,7 @@ struct TileLds {
     float* sxy;
     float* sth;
     float* ik;
+    float* Lr;        // box side of each replica (its temperature slot's)
     float* eout;      // canonical energy per replica at the end of the launch
     CollWork* cw;     // per replica: collective-move / kick scratch
     int* acc;
@@ -314,8 +318,9 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool w
     t.sxy = f + 4 * G;
     t.sth = f + 5 * G;
     t.ik = f + 6 * G;
-    t.eout = f + 7 * G;
-    t.acc = reinterpret_cast<int*>(f + 8 * G);
+    t.Lr = f + 7 * G;
+    t.eout = f + 8 * G;
+    t.acc = reinterpret_cast<int*>(f + 9 * G);
     t.gcnt = t.acc + G;
     t.gstart = t.gcnt + (kGroups + 2);
     return t;
@@ -445,7 +450,7 @@ struct Move {
     float nx, ny, nc, ns, wdiff;
 };
 constexpr float kNoMove = 8.0f;  // cos field of a staged proposal rejected early
-// (replica step sizes sxy, sth and grid shift shx, shy given)
+// (replica step sizes sxy, sth, grid shift shx, shy and box side L given)
 __device__ __forceinline__ bool make_move(float sxy, float sth, float shx, float shy, const float4& p,
                                           const float4& r, int kx, int ky, int m, float inv_w, float L,
                                           Move& mv) {
@@ -459,8 +464,8 @@ __device__ __forceinline__ bool make_move(float sxy, float sth, float shx, float
     return true;
 }
 __device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4& p, const float4& r, int kx,
-                                          int ky, int m, float inv_w, float L, Move& mv) {
-    return make_move(t.sxy[g], t.sth[g], t.shx[g], t.shy[g], p, r, kx, ky, m, inv_w, L, mv);
+                                          int ky, int m, float inv_w, Move& mv) {
+    return make_move(t.sxy[g], t.sth[g], t.shx[g], t.shy[g], p, r, kx, ky, m, inv_w, t.Lr[g], mv);
 }
 
 // Stencil energy change and Metropolis test of one proposal of tile square q
@@ -623,7 +628,7 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
 // here.
 template <bool kQuads>
 __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
-                                        float L, int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
+                                        int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
                                         int lpi, int sub, unsigned long long& cC,
                                         unsigned long long& cP) {
     const int g = rep_of(t, q);
@@ -649,7 +654,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         u = b.y;
     } else {
         const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
-        if (!make_move(t, g, p, r, kx, ky, m, inv_w, L, mv)) return;
+        if (!make_move(t, g, p, r, kx, ky, m, inv_w, mv)) return;
         u = r.w;
     }
     evaluate<kQuads>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
@@ -680,7 +685,7 @@ __device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int
 // The same from the tile's LDS arrays (square q of level item j), written
 // once per sweep by one lane per square when the level items are scattered.
 __device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int n, int M, int m,
-                                             float inv_w, float L, int r0, uint32_t sweep, uint32_t k0,
+                                             float inv_w, int r0, uint32_t sweep, uint32_t k0,
                                              uint32_t k1) {
     const int g = rep_of(t, q), i = q - g * n;
     const int ck = t.cellk[q];
@@ -688,7 +693,7 @@ __device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int
     const Stencil sc(t.A + g * grid_words(M), M, kx, ky);
     const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
     write_record(t, j, q, g, t.slotq[q], t.pos[q], kx, ky, sc, r, t.sxy[g], t.sth[g], t.shx[g], t.shy[g],
-                 t.ik[g], m, inv_w, L);
+                 t.ik[g], m, inv_w, t.Lr[g]);
 }
 
 // The proposal of mover record j: one LDS round trip for everything but the
@@ -716,7 +721,7 @@ __device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int 
 // larger index inside the 5x5 stencil, then a fixed pairwise tree over the
 // square index, so the bits do not depend on the tile shape.
 template <int NT>
-__device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float inv_w, float L,
+__device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float inv_w,
                             float* out /* G floats, LDS or global */) {
     const int tid = threadIdx.x;
     for (int g = tid; g < G; g += NT) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
@@ -731,7 +736,7 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const int kx = ck & 255, ky = ck >> 8;
         const Stencil sc(t.A + g * gw, M, kx, ky);
         const float4* pg = t.pos + g * n;
-        float ei = wall32(p.x, p.y, p.z, p.w, L);
+        float ei = wall32(p.x, p.y, p.z, p.w, t.Lr[g]);
         for (int f = 0; f < sc.total; ++f) {
             const int k = sc.at(f);
             const int j = t.sorted[k];
@@ -802,8 +807,7 @@ __device__ __forceinline__ bool coll_take(const CollWork& w, float4 z) {
 // is kept in the replica's own HBM slot, written and read back by the same
 // thread.
 template <int NT>
-__device__ void collective_step(const TileLds& t, int n, int G, int M, int m, float inv_w, float L,
-                                int r0, uint32_t sweep_end, uint32_t c, uint32_t k0, uint32_t k1,
+__device__ void collective_step(const TileLds& t, int n, int G, int M, int m, float inv_w, int r0, uint32_t sweep_end, uint32_t c, uint32_t k0, uint32_t k1,
                                 float4* gstate, unsigned& tried, unsigned& accd) {
     const int tid = threadIdx.x;
     for (int g = tid; g < G; g += NT) {
@@ -843,10 +847,11 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
         }
         float nc, ns;
         rot32(z.z, z.w, w.th, nc, ns);
+        const float L = t.Lr[g];
         t.pos[q] = make_float4(fminf(L - 0.45f, fmaxf(0.45f, nx)), fminf(L - 0.45f, fmaxf(0.45f, ny)), nc, ns);
     }
     tile_sync<NT>();
-    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+    tile_energy<NT>(t, n, G, M, m, inv_w, t.eout);
     for (int g = tid; g < G; g += NT) {
         CollWork& w = t.cw[g];
         const float en = t.eout[g], dE = en - w.eold;
@@ -870,7 +875,7 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
 // +-0.015 rad, centres clamped to [0.48, L - 0.48].  t.eout holds the
 // canonical energy on entry and on exit.
 template <int NT>
-__device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, float L, int r0,
+__device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, int r0,
                      uint32_t interval, bool relocate, uint32_t k0, uint32_t k1, unsigned& kicked) {
     const int tid = threadIdx.x, gw = grid_words(M);
     float* sq = reinterpret_cast<float*>(t.rank);  // rank + items: 4 bytes per square
@@ -888,7 +893,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         const int ck = t.cellk[q];
         const Stencil sc(t.A + g * gw, M, ck & 255, ck >> 8);
         const float4* pg = t.pos + (size_t)g * n;
-        float ei = wall32(p.x, p.y, p.z, p.w, L);
+        float ei = wall32(p.x, p.y, p.z, p.w, t.Lr[g]);
         for (int f = 0; f < sc.total; ++f) {
             const int j = t.sorted[sc.at(f)];
             if (j == i) continue;
@@ -914,6 +919,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
             const int g = k >> 6, pidx = k & 63;
             const CollWork& w = t.cw[g];
             if (!w.accept) continue;
+            const float L = t.Lr[g];
             const float vx = __builtin_fmaf(L - 1.1f, ((pidx & 7) + 0.5f) * 0.125f, 0.55f);
             const float vy = __builtin_fmaf(L - 1.1f, ((pidx >> 3) + 0.5f) * 0.125f, 0.55f);
             const float4* pg = t.pos + (size_t)g * n;
@@ -930,6 +936,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         for (int g = tid; g < G; g += NT) {
             const CollWork& w = t.cw[g];
             if (!w.accept) continue;
+            const float L = t.Lr[g];
             const int pidx = (int)(0xFFFFFFFFu - (unsigned)(t.gmask[g] & 0xFFFFFFFFull));
             const float vx = __builtin_fmaf(L - 1.1f, ((pidx & 7) + 0.5f) * 0.125f, 0.55f);
             const float vy = __builtin_fmaf(L - 1.1f, ((pidx >> 3) + 0.5f) * 0.125f, 0.55f);
@@ -949,6 +956,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         const float axy = hot ? 0.08f : 0.008f, aa = hot ? 0.18f : 0.015f;
         const uint4 o = philox((uint32_t)i, (uint32_t)(r0 + g), interval, 8u, k0, k1);
         const float4 p = t.pos[q];
+        const float L = t.Lr[g];
         const float x = fminf(L - 0.48f, fmaxf(0.48f, p.x + axy * __builtin_fmaf(2.0f, u01(o.x), -1.0f)));
         const float y = fminf(L - 0.48f, fmaxf(0.48f, p.y + axy * __builtin_fmaf(2.0f, u01(o.y), -1.0f)));
         float c, s;
@@ -956,7 +964,111 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         t.pos[q] = make_float4(x, y, c, s);
     }
     tile_sync<NT>();
-    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+    tile_energy<NT>(t, n, G, M, m, inv_w, t.eout);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 certified side of every tile replica (cuda/packing.cu:254-297), from
+// the cell lists of the final positions.  The dilation is the largest
+// (R + tau) / vmax over pairs nearer than sqrt(2.05); a pair in cells more
+// than two apart (2w > sqrt(2) + 6e-4) cannot overlap and needs a dilation
+// below 1, so the 5 x 5 stencil finds every pair that raises it.
+// ---------------------------------------------------------------------------
+// Order-preserving map of a double onto 64-bit unsigned integers, so LDS
+// integer min / max atomics order doubles.
+__device__ __forceinline__ unsigned long long dkey(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dval(unsigned long long k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
+}
+// FP64 cos / sin of a stored FP32 orientation, scaled to unit length.
+__device__ __forceinline__ void unit64(float c, float s, double& cd, double& sd) {
+    const double x = c, y = s;
+    const double h = 1.0 / sqrt(x * x + y * y);
+    cd = x * h;
+    sd = y * h;
+}
+// Dilation a pair needs to be separated by tau (cuda/packing.cu:266-271).
+__device__ __forceinline__ double cert_need(double dx, double dy, double ci, double si, double cj, double sj) {
+    const double r = 0.5 + 0.5 * (fabs(ci * cj + si * sj) + fabs(-si * cj + ci * sj));
+    const double v1 = fabs(dx * ci + dy * si), v2 = fabs(-dx * si + dy * ci);
+    const double v3 = fabs(dx * cj + dy * sj), v4 = fabs(-dx * sj + dy * cj);
+    const double vmax = fmax(fmax(v1, v2), fmax(v3, v4));
+    return vmax > 1e-12 ? (r + kCertTau) / vmax : 1e9;
+}
+// rs: the tile's first replica record.  Writes Lc (1e9 when the dilation
+// exceeds 1.008), lam, minx, miny per replica; certs counts the certified.
+template <int NT>
+__device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs, unsigned& certs) {
+    static_assert(sizeof(CollWork) >= 5 * 8, "the per-replica scratch holds five 64-bit keys");
+    const int tid = threadIdx.x, gw = grid_words(M);
+    // per replica (the collective-move scratch, free now): dilation, min x,
+    // max x, min y, max y as ordered keys
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(t.cw);
+    constexpr int kStride = (int)(sizeof(CollWork) / 8);
+    for (int g = tid; g < G; g += NT) {
+        unsigned long long* a = acc + kStride * g;
+        a[0] = dkey(1.0);
+        a[1] = ~0ull;
+        a[2] = 0ull;
+        a[3] = ~0ull;
+        a[4] = 0ull;
+    }
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
+        const int g = rep_of(t, q), i = q - g * n;
+        const float4 p = t.pos[q];
+        const int ck = t.cellk[q];
+        const Stencil sc(t.A + g * gw, M, ck & 255, ck >> 8);
+        const float4* pg = t.pos + (size_t)g * n;
+        const double xi = p.x, yi = p.y;
+        double ci, si;
+        unit64(p.z, p.w, ci, si);
+        double lam = 1.0;
+        for (int f = 0; f < sc.total; ++f) {
+            const int k = sc.at(f);
+            const int j = t.sorted[k];
+            if (j <= i) continue;
+            const float4 v = t.spos ? t.spos[k] : pg[j];
+            const double dx = xi - (double)v.x, dy = yi - (double)v.y;
+            if (dx * dx + dy * dy >= kCertNear2) continue;
+            double cj, sj;
+            unit64(v.z, v.w, cj, sj);
+            lam = fmax(lam, cert_need(dx, dy, ci, si, cj, sj));
+        }
+        if (lam > 1.0) atomicMax(acc + kStride * g, dkey(lam));
+    }
+    tile_sync<NT>();
+    for (int q = tid; q < G * n; q += NT) {
+        const int g = rep_of(t, q);
+        unsigned long long* a = acc + kStride * g;
+        const double lam = dval(a[0]);
+        const float4 p = t.pos[q];
+        double ci, si;
+        unit64(p.z, p.w, ci, si);
+        const double r = 0.5 * (fabs(ci) + fabs(si));
+        const double x = (double)p.x * lam, y = (double)p.y * lam;
+        atomicMin(a + 1, dkey(x - r));
+        atomicMax(a + 2, dkey(x + r));
+        atomicMin(a + 3, dkey(y - r));
+        atomicMax(a + 4, dkey(y + r));
+    }
+    tile_sync<NT>();
+    for (int g = tid; g < G; g += NT) {
+        const unsigned long long* a = acc + kStride * g;
+        const double lam = dval(a[0]), minx = dval(a[1]), maxx = dval(a[2]), miny = dval(a[3]),
+                     maxy = dval(a[4]);
+        const double Lc = lam > kCertMaxLam ? kCertNone : fmax(maxx - minx, maxy - miny);
+        RepState* r = rs + g;
+        r->Lc = Lc;
+        r->lam = lam;
+        r->minx = minx;
+        r->miny = miny;
+        certs += Lc < kCertNone ? 1u : 0u;
+    }
+    tile_sync<NT>();
 }
 
 // ---------------------------------------------------------------------------
@@ -968,18 +1080,17 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
 // shift): every square's staged move is drawn again from its current position,
 // either into prop / pw or into its mover record j (one-wavefront tiles with
 // records; square, CSR slot, replica, 1/T and stencil stay).
-__device__ __forceinline__ void stage_prop(const TileLds& t, int q, int n, int m, float inv_w, float L, int r0,
+__device__ __forceinline__ void stage_prop(const TileLds& t, int q, int n, int m, float inv_w, int r0,
                                            uint32_t sweep, uint32_t k0, uint32_t k1) {
     const int g = rep_of(t, q), i = q - g * n;
     const int ck = t.cellk[q];
     const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
     Move mv;
-    const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, L, mv);
+    const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, mv);
     t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
     t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
 }
-__device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, int m, float inv_w, float L,
-                                               int r0, uint32_t sweep, uint32_t k0, uint32_t k1) {
+__device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, int m, float inv_w, int r0, uint32_t sweep, uint32_t k0, uint32_t k1) {
     uint4* w = t.rec + (size_t)j * kRecWords;
     const uint4 w2 = w[2];
     const int g = (int)w[4].y;
@@ -988,7 +1099,7 @@ __device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, i
     const int ck = t.cellk[q];
     const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
     Move mv;
-    const bool ok = make_move(t, g, p, r, ck & 255, ck >> 8, m, inv_w, L, mv);
+    const bool ok = make_move(t, g, p, r, ck & 255, ck >> 8, m, inv_w, mv);
     w[0] = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), __float_as_uint(p.w));
     w[1] = ok ? make_uint4(__float_as_uint(mv.nx), __float_as_uint(mv.ny), __float_as_uint(mv.nc),
                            __float_as_uint(mv.ns))
@@ -1003,7 +1114,7 @@ __device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, i
 // the phases instead of read back from LDS, and its move's variates drawn
 // while the first reads are in flight.  Leaves the same LDS state and returns
 // the level starts (lane l: start of level l; lane kGroups + 1: the total).
-__device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, float inv_w, float L, float w,
+__device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, float inv_w, float w,
                                   int r0, uint32_t sweep, bool next_shift, uint32_t next_sweep, uint32_t k0,
                                   uint32_t k1) {
     const int lane = threadIdx.x;
@@ -1014,7 +1125,7 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
     const int i = q - g * n;
     uint16_t* Ag = t.A + g * gw;
     float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float shx = 0.0f, shy = 0.0f, sxy = 0.0f, sth = 0.0f, ik = 0.0f;
+    float shx = 0.0f, shy = 0.0f, sxy = 0.0f, sth = 0.0f, ik = 0.0f, L = 0.0f;
     if (has) {
         p = t.pos[q];
         shx = t.shx[g];
@@ -1022,6 +1133,7 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
         sxy = t.sxy[g];
         sth = t.sth[g];
         ik = t.ik[g];
+        L = t.Lr[g];
     }
     const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
     if (lane < kGroups + 2) t.gcnt[lane] = 0;
@@ -1102,9 +1214,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int nsw = (int)min((long long)D->spe, left);
     if (nsw <= 0) return;
     const int n = D->n, G = tl.g, M = D->M, tid = threadIdx.x, lane = tid & 63;
-    const float L = D->Lf, w = D->w, scale = D->scale;
+    const float w = D->w, scale = D->scale;
     const float inv_w = 1.0f / w;
-    const int m = grid_cells(L, w);
+    const int m = grid_cells(D->Lf, w);  // every replica's box is at most Lf
     const uint32_t k0 = D->key0, k1 = D->key1;
     const uint32_t sweep0 = (uint32_t)D->sweeps_done;
     const int ncoll = D->collective, kick_every = D->kick_every;
@@ -1113,18 +1225,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     TileLds t = carve(smem, n, M, G, tl.wide != 0);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
-    for (int q = tid; q < G * n; q += NT) {
-        float4 v = gstate[q];
-        v.x = fminf(fmaxf(v.x * scale, 0.0f), L);
-        v.y = fminf(fmaxf(v.y * scale, 0.0f), L);
-        t.pos[q] = v;
-    }
     for (int g = tid; g < G; g += NT) {
-        const RepState rs = reps[D->rep_off + tl.r0 + g];
-        const SlotState ss = slots[D->rep_off + rs.slot];
-        t.sxy[g] = ss.step_xy;
-        t.sth[g] = ss.step_th;
-        t.ik[g] = ss.inv_t_log2e;
+        const int slot = reps[D->rep_off + tl.r0 + g].slot;
+        const float4 ss = *reinterpret_cast<const float4*>(slots + D->rep_off + slot);
+        t.ik[g] = ss.x;
+        t.sxy[g] = ss.y;
+        t.sth[g] = ss.z;
+        t.Lr[g] = ss.w;  // the box of the slot the replica holds
         t.acc[g] = 0;
         // grid shift of the first sweep's block (blocks of spsh sweeps share
         // one shift, drawn at the block's first sweep counter; the next
@@ -1132,6 +1239,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint4 o = philox(0u, (uint32_t)(tl.r0 + g), sweep0 / spsh * spsh, 1u, k0, k1);
         t.shx[g] = w * u01(o.x);
         t.shy[g] = w * u01(o.y);
+    }
+    tile_sync<NT>();
+    // stage the state: centres rescaled (after a box shrink) and clamped into
+    // the replica's box
+    for (int q = tid; q < G * n; q += NT) {
+        const float L = t.Lr[rep_of(t, q)];
+        float4 v = gstate[q];
+        v.x = fminf(fmaxf(v.x * scale, 0.0f), L);
+        v.y = fminf(fmaxf(v.y * scale, 0.0f), L);
+        t.pos[q] = v;
     }
     tile_sync<NT>();
 
@@ -1147,14 +1264,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // drawn again
             if (NT == 64 && t.rec) {
                 const int items = __builtin_amdgcn_readlane(gs, kGroups);
-                for (int j = tid; j < items; j += NT) restage_record(t, j, n, m, inv_w, L, tl.r0, sweep, k0, k1);
+                for (int j = tid; j < items; j += NT) restage_record(t, j, n, m, inv_w, tl.r0, sweep, k0, k1);
             } else if (t.prop) {
-                for (int q = tid; q < G * n; q += NT) stage_prop(t, q, n, m, inv_w, L, tl.r0, sweep, k0, k1);
+                for (int q = tid; q < G * n; q += NT) stage_prop(t, q, n, m, inv_w, tl.r0, sweep, k0, k1);
             }
             tile_sync<NT>();
         } else if (NT == 64 && t.rec && G * n <= kSweepThreads) {
             if constexpr (NT == 64)
-                gs = setup_lane_squares(t, n, G, M, m, inv_w, L, w, tl.r0, sweep, next_in_launch, next_block, k0,
+                gs = setup_lane_squares(t, n, G, M, m, inv_w, w, tl.r0, sweep, next_in_launch, next_block, k0,
                                         k1);
         } else {
             for (int k = tid; k < kGroups + 2; k += NT) t.gcnt[k] = 0;
@@ -1174,7 +1291,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int key = r < kRankCap ? r * 9 + colour : kGroups;
                 t.rank[q] = (uint16_t)key;
                 atomicOr(&t.gmask[g], 1ull << key);
-                if (t.prop) stage_prop(t, q, n, m, inv_w, L, tl.r0, sweep, k0, k1);
+                if (t.prop) stage_prop(t, q, n, m, inv_w, tl.r0, sweep, k0, k1);
             }
             if (next_in_launch)
                 for (int g = tid; g < G; g += NT) {
@@ -1212,7 +1329,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (lvl < kGroups) {
                     const int j = atomicAdd(&t.gcnt[lvl], 1);
                     if (NT == 64 && t.rec)
-                        write_record(t, j, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1);
+                        write_record(t, j, q, n, M, m, inv_w, tl.r0, sweep, k0, k1);
                     else
                         t.items[j] = (uint16_t)q;
                 }
@@ -1238,8 +1355,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         continue;
                     }
                 }
-                propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, L, tl.r0, sweep, k0, k1, lpi,
-                                   sub, cC, cP);
+                propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, tl.r0, sweep, k0, k1, lpi, sub, cC,
+                                   cP);
             }
             tile_sync<NT>();
         }
@@ -1247,7 +1364,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (tid == 0)
                 for (int q = 0; q < G * n; ++q)
                     if (t.rank[q] == kGroups)
-                        propose<(NT > 64)>(t, q, n, M, m, inv_w, L, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
+                        propose<(NT > 64)>(t, q, n, M, m, inv_w, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
             tile_sync<NT>();
         }
         if ((sweep + 1u) % spsh == 0u) {  // the next sweep starts a block: its shift becomes current
@@ -1260,14 +1377,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     }
 
-    tile_energy<NT>(t, n, G, M, m, inv_w, L, t.eout);
+    tile_energy<NT>(t, n, G, M, m, inv_w, t.eout);
     unsigned ctried = 0, caccd = 0, kicked = 0;
     for (int c = 0; c < ncoll; ++c)
-        collective_step<NT>(t, n, G, M, m, inv_w, L, tl.r0, sweep0 + (uint32_t)nsw, (uint32_t)c, k0, k1,
-                            gstate, ctried, caccd);
-    if (kick_every > 0 && (interval + 1u) % (uint32_t)kick_every == 0u)
-        kick<NT>(t, n, G, M, m, inv_w, L, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u,
-                 k0, k1, kicked);
+        collective_step<NT>(t, n, G, M, m, inv_w, tl.r0, sweep0 + (uint32_t)nsw, (uint32_t)c, k0, k1, gstate, ctried,
+                            caccd);
+    const bool kick_now = kick_every > 0 && (interval + 1u) % (uint32_t)kick_every == 0u;
+    if (kick_now)
+        kick<NT>(t, n, G, M, m, inv_w, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u, k0, k1,
+                 kicked);
+    unsigned certs = 0;
+    if (D->improve) {
+        // the cell lists must hold the final positions: the last binning is
+        // the energy's (after the sweeps, or the kick's), unless rejected
+        // collective moves restored positions after it
+        if (ncoll > 0 && !kick_now) {
+            for (int g = tid; g < G; g += NT) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
+            tile_sync<NT>();
+            bin_tile<NT>(t, n, G, M, m, inv_w);
+        }
+        tile_certify<NT>(t, n, G, M, reps + D->rep_off + tl.r0, certs);
+    }
     for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
     unsigned long long accsum = 0;
     for (int g = tid; g < G; g += NT) {
@@ -1282,6 +1412,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         cP += __shfl_xor(cP, off, 64);
         accsum += __shfl_xor(accsum, off, 64);
         ctried += __shfl_xor(ctried, off, 64);
+        certs += __shfl_xor(certs, off, 64);
         caccd += __shfl_xor(caccd, off, 64);
         kicked += __shfl_xor(kicked, off, 64);
     }
@@ -1294,6 +1425,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             atomicAdd(&D->coll_acc, (unsigned long long)caccd);
         }
         if (kicked) atomicAdd(&D->kicks, (int)kicked);
+        if (certs) atomicAdd(&D->certs, (unsigned long long)certs);
         if (tid == 0) atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
     }
 }
@@ -1537,7 +1669,26 @@ __device__ bool overlap_free64(const double* x, const double* y, const double* c
 
 // ---------------------------------------------------------------------------
 // k_ladder: one workgroup per instance (insts[list[b]], or insts[b] without a
-// list), after every sweep launch.
+// list), after every sweep launch.  In order:
+//  1. step adaptation of every temperature slot from the acceptance of the
+//     replica that held it: base x e^((rate - target) / 2), clamped;
+//  2. improve = 1: the smallest certified side over the replicas (the lowest
+//     replica index on ties); when it beats the best side by 1e-11
+//     (cuda/packing.cu:831-833) it becomes the best side, the replica's state
+//     the archive, every slot the box max(sqrt n + 1e-6, best - shrink *
+//     mults[8 position / ladder]) (:503-504, :841), and the coldest slot of
+//     every ladder -- every slot of the first ladder when the gain exceeds
+//     5e-4 -- restarts from the archive dilated, translated and scaled to its
+//     box (:836-846); such an interval does no exchange (:849-851);
+//  3. otherwise (and always with improve = 0) even/odd exchange of adjacent
+//     slots inside each ladder: take when log u < (1/Ta - 1/Tb)(Ea - Eb)
+//     (cuda/packing.cu:141-152, 750-765), written u < 2^x with
+//     x = (log2e/Ta - log2e/Tb)(Ea - Eb) and the sweep's FP32 2^x;
+//  4. improve = 0: the coldest replica (lowest slot) with FP32 energy exactly
+//     zero goes through the FP64 zero-overlap test; if it passes it becomes
+//     the archive when its box is the smallest yet, and the box shrinks by
+//     `shrink` (every centre rescaled on the next staging);
+//  5. the next interval's temperatures and step sizes (cooling cycle).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict__ insts,
                                                            const int* __restrict__ list,
@@ -1548,105 +1699,179 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                                                            float4* __restrict__ arch) {
     extern __shared__ __align__(16) unsigned char lsm[];
     __shared__ int zero_slot;
+    __shared__ int best_r;
+    __shared__ unsigned long long best_lc;
     InstDesc* D = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
     const long long left = D->sweeps_total - D->sweeps_done;
     if (left <= 0) return;
     const int ran = (int)min((long long)D->spe, left);
     const int tid = threadIdx.x;
     const int R = D->R, n = D->n, lad = D->ladder, base = D->rep_off;
+    const int improve = D->improve;
 
     // The ladder of the instance in LDS, by temperature slot: energy and
-    // temperature of the slot, replica holding it.  One round of global loads
-    // (replica records, then each replica's slot record) instead of a chain
-    // per phase.
+    // 1/T (log2 units) of the slot, replica holding it.  One round of global
+    // loads (replica records, then each replica's slot record).
     float* eS = reinterpret_cast<float*>(lsm);
-    float* tS = eS + R;
-    int* rS = reinterpret_cast<int*>(tS + R);
+    float* kS = eS + R;
+    int* rS = reinterpret_cast<int*>(kS + R);
     const float target = D->target_acc;
+    if (tid == 0) {
+        zero_slot = R;
+        best_r = R;
+        best_lc = ~0ull;
+    }
+    __syncthreads();
 
     // 1. step adaptation per temperature slot, from the replica that held it
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
         const float rate = (float)rs.acc / (float)((long long)n * ran);
-        const float f = expf((rate - target) * 0.5f);
+        // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
+        const float f = accept_exp2((rate - target) * 0.72134752f);
         SlotState* sl = slots + base + rs.slot;
         const SlotState ss = *sl;
-        sl->step_xy = fminf(fmaxf(ss.step_xy * f, 1.7320508e-7f), 0.8660254f);
-        sl->step_th = fminf(fmaxf(ss.step_th * f, 1.7320508e-7f), 0.7794229f);
+        sl->base_xy = fminf(fmaxf(ss.base_xy * f, 1.7320508e-7f), 0.8660254f);
+        sl->base_th = fminf(fmaxf(ss.base_th * f, 1.7320508e-7f), 0.7794229f);
         eS[rs.slot] = rs.E;
-        tS[rs.slot] = ss.T;
+        kS[rs.slot] = ss.inv_t_log2e;
         rS[rs.slot] = r;
+        if (improve) atomicMin(&best_lc, (unsigned long long)__double_as_longlong(rs.Lc));
     }
     __syncthreads();
 
-    // 2. even/odd exchange of adjacent temperatures inside each ladder (the
-    //    pairs are disjoint, so each thread owns the slots it swaps)
+    // 2. improvement: the replica with the smallest certified side
     const uint32_t interval = D->interval, key0 = D->key0, key1 = D->key1;
-    const int parity = (int)(interval & 1u);
-    unsigned long long tried = 0, accd = 0;
-    for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
-        const int s = 2 * p + parity;
-        if ((s % lad) + 1 >= lad) continue;
-        const int ra = rS[s], rb = rS[s + 1];
-        const float Ta = tS[s], Tb = tS[s + 1];
-        const float Ea = eS[s], Eb = eS[s + 1];
-        const uint4 o = philox((uint32_t)s, 0u, interval, 2u, key0, key1);
-        const float lhs = logf(u01(o.x));
-        const float rhs = (1.0f / Ta - 1.0f / Tb) * (Ea - Eb);
-        ++tried;
-        if (lhs < rhs) {
-            rS[s] = rb;
-            rS[s + 1] = ra;
-            eS[s] = Eb;
-            eS[s + 1] = Ea;
-            rep_of_slot[base + s] = rb;
-            rep_of_slot[base + s + 1] = ra;
-            reps[base + ra].slot = s + 1;
-            reps[base + rb].slot = s;
-            ++accd;
+    const double L0 = D->L;
+    bool improved = false;
+    if (improve) {
+        const double lc = __longlong_as_double((long long)best_lc);
+        improved = lc < L0 - 1e-11;
+        if (improved) {
+            for (int r = tid; r < R; r += kLadderThreads)
+                if ((unsigned long long)__double_as_longlong(reps[base + r].Lc) == best_lc) atomicMin(&best_r, r);
+            __syncthreads();
+            const int rb = best_r;
+            const RepState rs = reps[base + rb];
+            const double Lb = rs.Lc, lam = rs.lam, mx = rs.minx, my = rs.miny;
+            const float box_b = slots[base + rs.slot].L;
+            const float4* src = state + D->state_off + (size_t)rb * n;
+            for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
+            __syncthreads();
+            // every slot's box below the new best side
+            const double shrink = (double)D->shrink, lmin = D->Lmin + 1e-6;
+            for (int s = tid; s < R; s += kLadderThreads)
+                slots[base + s].L = (float)fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
+            // restarts: slot 0 of every ladder, and the whole first ladder on
+            // a large gain
+            const int heads = R / lad, extra = (L0 - Lb > 5e-4) ? lad - 1 : 0;
+            const float4* best = arch + D->arch_off;
+            for (int k = tid; k < (heads + extra) * n; k += kLadderThreads) {
+                const int h = k / n, i = k - h * n;
+                const int s = h < heads ? h * lad : h - heads + 1;
+                const double tL = fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
+                const double sc = tL / Lb;
+                const float4 v = best[i];
+                float4* dst = state + D->state_off + (size_t)rS[s] * n;
+                dst[i] = make_float4((float)(((double)v.x * lam - mx) * sc), (float)(((double)v.y * lam - my) * sc),
+                                     v.z, v.w);
+            }
+            if (tid == 0) {
+                D->L = Lb;
+                D->Lf = (float)Lb;
+                D->L_arch = Lb;
+                D->L_box_arch = (double)box_b;
+                D->arch_valid = 1;
+                D->feasible += 1;
+                D->shrinks += 1;
+            }
         }
     }
-    if (tried) atomicAdd(&D->ex_tried, tried);
-    if (accd) atomicAdd(&D->ex_acc, accd);
-    if (tid == 0) zero_slot = R;
+
+    // 3. even/odd exchange of adjacent temperatures inside each ladder (the
+    //    pairs are disjoint, so each thread owns the slots it swaps)
+    if (!improved) {
+        const int parity = (int)(interval & 1u);
+        unsigned long long tried = 0, accd = 0;
+        for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
+            const int s = 2 * p + parity;
+            if ((s % lad) + 1 >= lad) continue;
+            const int ra = rS[s], rb = rS[s + 1];
+            const float Ea = eS[s], Eb = eS[s + 1];
+            const float x = (kS[s] - kS[s + 1]) * (Ea - Eb);
+            const uint4 o = philox((uint32_t)s, 0u, interval, 2u, key0, key1);
+            ++tried;
+            if (x >= 0.0f || u01(o.x) < accept_exp2(x)) {
+                rS[s] = rb;
+                rS[s + 1] = ra;
+                eS[s] = Eb;
+                eS[s + 1] = Ea;
+                rep_of_slot[base + s] = rb;
+                rep_of_slot[base + s + 1] = ra;
+                reps[base + ra].slot = s + 1;
+                reps[base + rb].slot = s;
+                ++accd;
+            }
+        }
+        if (tried) atomicAdd(&D->ex_tried, tried);
+        if (accd) atomicAdd(&D->ex_acc, accd);
+    }
     __syncthreads();
 
-    // 3. zero-overlap test: coldest replica with FP32 energy exactly zero.
+    // 4. zero-overlap test: coldest replica with FP32 energy exactly zero.
     //    Descriptor fields are read into registers before any thread writes.
-    const double L0 = D->L;
-    const int had_arch = D->arch_valid;
-    const double L_arch = D->L_arch;
-    for (int s = tid; s < R; s += kLadderThreads)
-        if (eS[s] == 0.0f) atomicMin(&zero_slot, s);
-    __syncthreads();
     double scale = 1.0, Ln = L0;
     bool archive = false;
-    if (zero_slot < R) {
-        const int r = rS[zero_slot];
-        __syncthreads();  // the ladder's LDS is reused below
-        const float4* src = state + D->state_off + (size_t)r * n;
-        const LadderLds t = carve_ladder(lsm, n, L0);
-        for (int i = tid; i < n; i += kLadderThreads) {
-            const float4 v = src[i];
-            const double c = v.z, s = v.w, h = 1.0 / sqrt(c * c + s * s);
-            t.x[i] = v.x; t.y[i] = v.y; t.c[i] = c * h; t.s[i] = s * h;
+    if (!improve) {
+        const int had_arch = D->arch_valid;
+        const double L_arch = D->L_arch;
+        for (int s = tid; s < R; s += kLadderThreads)
+            if (eS[s] == 0.0f) atomicMin(&zero_slot, s);
+        __syncthreads();
+        if (zero_slot < R) {
+            const int r = rS[zero_slot];
+            __syncthreads();  // the ladder's LDS is reused below
+            const float4* src = state + D->state_off + (size_t)r * n;
+            const LadderLds t = carve_ladder(lsm, n, L0);
+            for (int i = tid; i < n; i += kLadderThreads) {
+                const float4 v = src[i];
+                double c, s;
+                unit64(v.z, v.w, c, s);
+                t.x[i] = v.x; t.y[i] = v.y; t.c[i] = c; t.s[i] = s;
+            }
+            __syncthreads();
+            const bool zero64 = overlap_free64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L0, t.zg);
+            if (!zero64 && tid == 0) D->fp64_rejects += 1;
+            if (zero64) {
+                archive = !had_arch || L0 < L_arch;
+                if (archive)
+                    for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
+                Ln = fmax(L0 - (double)D->shrink, D->Lmin);
+                if (Ln < L0) scale = Ln / L0;
+            }
         }
         __syncthreads();
-        const bool zero64 = overlap_free64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L0, t.zg);
-        if (!zero64 && tid == 0) D->fp64_rejects += 1;
-        if (zero64) {
-            archive = !had_arch || L0 < L_arch;
-            if (archive)
-                for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
-            Ln = fmax(L0 - (double)D->shrink, sqrt((double)n));
-            if (Ln < L0) scale = Ln / L0;
+        if (Ln < L0)
+            for (int s = tid; s < R; s += kLadderThreads) slots[base + s].L = (float)Ln;
+    }
+
+    // 5. the next interval's temperatures and step sizes (after every base
+    //    write of step 1: the barriers above order them)
+    {
+        double cool, root;
+        cool_factor(interval + 1u, D->cool_period, cool, root);
+        for (int s = tid; s < R; s += kLadderThreads) {
+            SlotState ss = slots[base + s];
+            slot_effective(ss, cool, root);
+            *reinterpret_cast<float4*>(slots + base + s) =
+                make_float4(ss.inv_t_log2e, ss.step_xy, ss.step_th, ss.L);
         }
     }
-    __syncthreads();
     if (tid == 0) {
         if (archive) {
             D->arch_valid = 1;
             D->L_arch = L0;
+            D->L_box_arch = L0;
             D->feasible += 1;
         }
         if (Ln < L0) {
@@ -1658,6 +1883,38 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         D->sweeps_done += ran;
         D->interval += 1u;
     }
+}
+
+// Certified side of the configuration held in LDS by one workgroup
+// (cuda/packing.cu:254-297): all pairs i < j nearer than sqrt(2.05).  Returns
+// the side (1e9 when the dilation exceeds 1.008) and the dilation and lower
+// corner of the dilated configuration.
+template <int NT>
+__device__ double certified_side_block(const double* x, const double* y, const double* c, const double* s, int n,
+                                       double* tmp, double& lam_out, double& minx_out, double& miny_out) {
+    double lam = 1.0;
+    for (int i = threadIdx.x; i < n; i += NT)
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = x[i] - x[j], dy = y[i] - y[j];
+            if (dx * dx + dy * dy >= kCertNear2) continue;
+            lam = fmax(lam, cert_need(dx, dy, c[i], s[i], c[j], s[j]));
+        }
+    lam = block_max_d<NT>(lam, tmp);
+    double lox = 1e9, hix = -1e9, loy = 1e9, hiy = -1e9;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double r = 0.5 * (fabs(c[i]) + fabs(s[i]));
+        const double xi = x[i] * lam, yi = y[i] * lam;
+        lox = fmin(lox, xi - r); hix = fmax(hix, xi + r);
+        loy = fmin(loy, yi - r); hiy = fmax(hiy, yi + r);
+    }
+    lox = block_min_d<NT>(lox, tmp);
+    hix = block_max_d<NT>(hix, tmp);
+    loy = block_min_d<NT>(loy, tmp);
+    hiy = block_max_d<NT>(hiy, tmp);
+    lam_out = lam;
+    minx_out = lox;
+    miny_out = loy;
+    return lam > kCertMaxLam ? kCertNone : fmax(hix - lox, hiy - loy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1716,9 +1973,32 @@ __global__ __launch_bounds__(kPolishThreads) void k_finalize(const FinalizeJob* 
             xi = job.src64[3 * i]; yi = job.src64[3 * i + 1]; ai = job.src64[3 * i + 2];
         }
         x[i] = xi; y[i] = yi; a[i] = ai;
-        cc[i] = cos(ai); ss[i] = sin(ai);
+        if (src32) {
+            // the orientation as the sweeps' certified side reads it
+            unit64(src32[i].z, src32[i].w, cc[i], ss[i]);
+        } else {
+            cc[i] = cos(ai); ss[i] = sin(ai);
+        }
     }
     __syncthreads();
+    double* qo = q_out + job.out_off;
+    // certified side before the polish, exported; the polish's wins only when
+    // smaller (the reference keeps the smaller of its checkpoints,
+    // cuda/packing.cu:506-507, 689-693)
+    double lam0, mx0, my0;
+    const double side0 = certified_side_block<NT>(x, y, cc, ss, n, tmp, lam0, mx0, my0);
+    {
+        const double lx = lam0 > kCertMaxLam ? 1.0 : lam0;
+        for (int i = tid; i < n; i += NT) {
+            qo[3 * i] = x[i] * lx - (lam0 > kCertMaxLam ? 0.0 : mx0);
+            qo[3 * i + 1] = y[i] * lx - (lam0 > kCertMaxLam ? 0.0 : my0);
+            qo[3 * i + 2] = a[i];
+        }
+    }
+    if (src32) {
+        for (int i = tid; i < n; i += NT) { cc[i] = cos(a[i]); ss[i] = sin(a[i]); }
+        __syncthreads();
+    }
     double energy = total64<NT>(x, y, cc, ss, n, L, tmp);
     const double e_before = energy;
 
@@ -1807,46 +2087,38 @@ __global__ __launch_bounds__(kPolishThreads) void k_finalize(const FinalizeJob* 
         __syncthreads();
     }
 
-    // Certified side, cuda/packing.cu:254-297.
-    const double tau = 2.6e-8;
-    double lam = 1.0;
-    for (int i = tid; i < n; i += NT)
-        for (int j = i + 1; j < n; ++j) {
-            const double dx = x[i] - x[j], dy = y[i] - y[j];
-            if (dx * dx + dy * dy >= 2.05) continue;
-            const double R = 0.5 + 0.5 * (fabs(cc[i] * cc[j] + ss[i] * ss[j]) +
-                                          fabs(-ss[i] * cc[j] + cc[i] * ss[j]));
-            const double v = fmax(fmax(fabs(dx * cc[i] + dy * ss[i]), fabs(-dx * ss[i] + dy * cc[i])),
-                                  fmax(fabs(dx * cc[j] + dy * ss[j]), fabs(-dx * ss[j] + dy * cc[j])));
-            const double need = v > 1e-12 ? (R + tau) / v : 1e9;
-            lam = fmax(lam, need);
+    // Certified side after the polish.
+    double lam1, mx1, my1;
+    const double side1 = certified_side_block<NT>(x, y, cc, ss, n, tmp, lam1, mx1, my1);
+    const bool post = side1 < side0 || (side0 >= kCertNone && side1 >= kCertNone);
+    if (post) {
+        // translated, dilated export of the polished configuration (undilated
+        // when neither checkpoint certifies)
+        const double lx = lam1 > kCertMaxLam ? 1.0 : lam1;
+        double lox = mx1, loy = my1;
+        if (lam1 > kCertMaxLam) {
+            lox = 1e9; loy = 1e9;
+            for (int i = tid; i < n; i += NT) {
+                const double r = 0.5 * (fabs(cc[i]) + fabs(ss[i]));
+                lox = fmin(lox, x[i] - r); loy = fmin(loy, y[i] - r);
+            }
+            lox = block_min_d<NT>(lox, tmp);
+            loy = block_min_d<NT>(loy, tmp);
         }
-    lam = block_max_d<NT>(lam, tmp);
-    double lox = 1e9, hix = -1e9, loy = 1e9, hiy = -1e9;
-    const double lx = lam > 1.008 ? 1.0 : lam;
-    for (int i = tid; i < n; i += NT) {
-        const double r = 0.5 * (fabs(cc[i]) + fabs(ss[i]));
-        lox = fmin(lox, x[i] * lx - r); hix = fmax(hix, x[i] * lx + r);
-        loy = fmin(loy, y[i] * lx - r); hiy = fmax(hiy, y[i] * lx + r);
-    }
-    lox = block_min_d<NT>(lox, tmp);
-    hix = block_max_d<NT>(hix, tmp);
-    loy = block_min_d<NT>(loy, tmp);
-    hiy = block_max_d<NT>(hiy, tmp);
-    const double side = fmax(hix - lox, hiy - loy);
-    double* qo = q_out + job.out_off;
-    for (int i = tid; i < n; i += NT) {
-        qo[3 * i] = x[i] * lx - lox;
-        qo[3 * i + 1] = y[i] * lx - loy;
-        qo[3 * i + 2] = a[i];
+        for (int i = tid; i < n; i += NT) {
+            qo[3 * i] = x[i] * lx - lox;
+            qo[3 * i + 1] = y[i] * lx - loy;
+            qo[3 * i + 2] = a[i];
+        }
     }
     if (tid == 0) {
         out->e64_before = e_before;
         out->e64_after = energy;
-        out->lambda = lam;
-        out->L_cert = lam > 1.008 ? 1e9 : side;
-        out->span = side;
+        out->lambda = post ? lam1 : lam0;
+        out->L_cert = post ? side1 : side0;
+        out->span = post ? side1 : side0;
         out->iters = iters;
+        out->pre = post ? 0 : 1;
     }
 }
 

@@ -90,6 +90,8 @@ void default_params(sis_params* p) {
     p->init_mode = 0;
     p->flags = 0;
     p->rng_seed = 7;
+    p->improve = 0;
+    p->cool_period = 0;
 }
 
 int validate(const sis_instance* in, const sis_params* p) {
@@ -120,6 +122,9 @@ int validate(const sis_instance* in, const sis_params* p) {
     if (p->init_mode < 0 || p->init_mode > 2) return fail(SIS_EINVAL, "init_mode must be 0, 1 or 2");
     if (p->collective_moves < 0 || p->collective_moves > 64) return fail(SIS_EINVAL, "collective_moves in 0..64");
     if (p->kick_every < 0 || p->kick_every > 1000000) return fail(SIS_EINVAL, "kick_every in 0..1e6");
+    if (p->improve < 0 || p->improve > 1) return fail(SIS_EINVAL, "improve must be 0 or 1");
+    if (p->cool_period < 0 || p->cool_period == 1 || p->cool_period > 1000000)
+        return fail(SIS_EINVAL, "cool_period must be 0 or 2..1e6");
     const float w = (float)p->cell;
     const int M = grid_cells((float)in->L, w) + 4;
     if (M > 255) return fail(SIS_EINVAL, "cell grid wider than 255 cells: raise the cell width");
@@ -224,6 +229,61 @@ void initial_state(const InstSpec& s, std::vector<float4>& out) {
         }
 }
 
+// Box side of temperature slot s (position s % ladder) with best side Lb:
+// the reference's per-chain target max(sqrt n + 1e-6, best - shrink *
+// mults[b % 8]) (cuda/packing.cu:503-504, 841), the eight classes spread over
+// a ladder of any length.  improve = 0: every slot has the instance's box.
+double slot_box(const InstSpec& s, int ladder, int slot, double Lb) {
+    if (s.p.improve == 0) return Lb;
+    return std::max(std::sqrt((double)s.n) + 1e-6, Lb - s.p.shrink * kTargetMults[((slot % ladder) * 8) / ladder]);
+}
+
+void instance_records(const InstSpec& s, const Plan& pl, InstDesc& D, std::vector<RepState>& reps,
+                      std::vector<SlotState>& slots, std::vector<int>& ros) {
+    const int n = s.n, R = s.p.replicas;
+    D.n = n; D.R = R; D.ladder = pl.ladder; D.G = pl.G;
+    D.M = pl.M; D.spe = pl.spe; D.spsh = s.p.sweeps_per_shift;
+    D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
+    D.L = s.L; D.L_arch = s.L; D.L_box_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
+    D.Lmin = std::sqrt((double)n);
+    D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
+    D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
+    D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
+    D.improve = s.p.improve; D.cool_period = s.p.cool_period;
+    reps.resize(R);
+    slots.resize(R);
+    ros.resize(R);
+    const int lad = pl.ladder;
+    double cool, root;
+    cool_factor(0u, s.p.cool_period, cool, root);
+    for (int k = 0; k < R; ++k) {
+        const int pos = k % lad;
+        const double T = lad == 1 ? s.p.tmin : s.p.tmin * std::pow(s.p.tmax / s.p.tmin, (double)pos / (lad - 1));
+        SlotState& sl = slots[k];
+        sl = SlotState{};
+        sl.T = (float)T;
+        sl.base_xy = (float)(std::sqrt(3.0) * s.p.move_sigma);
+        sl.base_th = (float)std::min(0.7794229, std::sqrt(3.0) * s.p.rot_sigma);
+        sl.L = (float)slot_box(s, lad, k, s.L);
+        slot_effective(sl, cool, root);
+        reps[k] = RepState{0.0f, k, 0u, 0u, kCertNone, 1.0, 0.0, 0.0};
+        ros[k] = k;
+    }
+}
+
+// The FP64 finalisation of an instance: the archive when there is one
+// (polished in the box it was found in), else the best replica by FP32
+// energy in the instance's box.
+void finalize_source(const InstDesc& D, double& box, double& target_L) {
+    if (D.arch_valid) {
+        box = D.improve ? D.L_box_arch : D.L_arch;
+        target_L = D.L_arch;
+    } else {
+        box = D.L;
+        target_L = D.L;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Tile planning.
 // ---------------------------------------------------------------------------
@@ -326,14 +386,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
         int rc = plan_instance(s, (int)std::min<long long>(total_reps, 1 << 30), pl);
         if (rc) return rc;
         InstDesc& D = h_inst[k];
-        D.n = s.n; D.R = s.p.replicas; D.ladder = pl.ladder; D.G = pl.G;
         D.state_off = (int)nsq; D.rep_off = (int)nrep; D.arch_off = (int)narch;
-        D.M = pl.M; D.spe = pl.spe; D.spsh = specs[k].p.sweeps_per_shift;
-        D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
-        D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
-        D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
-        D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
-        D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
         nsq += (size_t)s.n * s.p.replicas;
         nrep += s.p.replicas;
         narch += s.n;
@@ -356,23 +409,17 @@ int Batch::build(std::vector<InstSpec>&& in) {
     std::vector<int> h_ros(nrep);
     for (int k = 0; k < ni; ++k) {
         const InstSpec& s = specs[k];
-        const InstDesc& D = h_inst[k];
+        InstDesc& D = h_inst[k];
         std::vector<float4> st;
         initial_state(s, st);
         std::copy(st.begin(), st.end(), h_state.begin() + D.state_off);
-        const int lad = D.ladder;
-        for (int r = 0; r < D.R; ++r) {
-            const int pos = r % lad;
-            const double T = lad == 1 ? s.p.tmin
-                                      : s.p.tmin * std::pow(s.p.tmax / s.p.tmin, (double)pos / (lad - 1));
-            SlotState& sl = h_slots[D.rep_off + r];
-            sl.T = (float)T;
-            sl.inv_t_log2e = (float)(1.4426950408889634 / T);
-            sl.step_xy = (float)(std::sqrt(3.0) * s.p.move_sigma);
-            sl.step_th = (float)std::min(0.7794229, std::sqrt(3.0) * s.p.rot_sigma);
-            h_reps[D.rep_off + r] = RepState{0.0f, r, 0u, 0u};
-            h_ros[D.rep_off + r] = r;
-        }
+        std::vector<RepState> reps;
+        std::vector<SlotState> slots;
+        std::vector<int> ros;
+        instance_records(s, plans[k], D, reps, slots, ros);
+        std::copy(reps.begin(), reps.end(), h_reps.begin() + D.rep_off);
+        std::copy(slots.begin(), slots.end(), h_slots.begin() + D.rep_off);
+        std::copy(ros.begin(), ros.end(), h_ros.begin() + D.rep_off);
     }
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -626,16 +673,14 @@ int Batch::finalize(std::vector<FinalRecord>& rec) {
         int best = 0;
         for (int r = 1; r < D.R; ++r)
             if (h_reps[D.rep_off + r].E < h_reps[D.rep_off + best].E) best = r;
+        finalize_source(D, j.L, rec[k].target_L);
         if (D.arch_valid) {
             j.src32 = d_arch + D.arch_off;
-            j.L = D.L_arch;
             rec[k].e32 = 0.0;
         } else {
             j.src32 = d_state + D.state_off + (size_t)best * D.n;
-            j.L = D.L;
             rec[k].e32 = h_reps[D.rep_off + best].E;
         }
-        rec[k].target_L = j.L;
     }
     for (void* p : {(void*)d_jobs, (void*)d_fout, (void*)d_qout})
         if (p) (void)hipFree(p);
@@ -692,6 +737,7 @@ void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis
     out->kicks = D.kicks;
     out->collective_tried = D.coll_tried;
     out->collective_accepted = D.coll_acc;
+    out->feasible_checkpoints = D.improve ? D.certs : (uint64_t)D.feasible;
     (void)k;
 }
 
@@ -883,6 +929,31 @@ int sis_batch_finalize(sis_batch* h, sis_result* out) {
 
 int sis_batch_destroy(sis_batch* h) {
     delete h;
+    return SIS_OK;
+}
+
+int sis_dev_batch_state(sis_batch* h, int k, sis_ladder_desc* d, sis_rep_state* reps, sis_slot_state* slots,
+                        int* rep_of_slot, float* xycs, float* arch) {
+    if (!h || k < 0 || k >= (int)h->b.specs.size()) return fail(SIS_EINVAL, "bad batch handle or instance");
+    Batch& b = h->b;
+    InstDesc D;
+    SIS_HIPCHECK(hipMemcpyAsync(&D, b.d_inst + k, sizeof(InstDesc), hipMemcpyDeviceToHost, b.stream));
+    SIS_HIPCHECK(hipStreamSynchronize(b.stream));
+    const size_t R = (size_t)D.R, n = (size_t)D.n;
+    if (reps) SIS_HIPCHECK(hipMemcpy(reps, b.d_reps + D.rep_off, sizeof(RepState) * R, hipMemcpyDeviceToHost));
+    if (slots) SIS_HIPCHECK(hipMemcpy(slots, b.d_slots + D.rep_off, sizeof(SlotState) * R, hipMemcpyDeviceToHost));
+    if (rep_of_slot) SIS_HIPCHECK(hipMemcpy(rep_of_slot, b.d_ros + D.rep_off, sizeof(int) * R, hipMemcpyDeviceToHost));
+    if (xycs) SIS_HIPCHECK(hipMemcpy(xycs, b.d_state + D.state_off, sizeof(float4) * R * n, hipMemcpyDeviceToHost));
+    if (arch) SIS_HIPCHECK(hipMemcpy(arch, b.d_arch + D.arch_off, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    if (d) {
+        d->n = D.n; d->R = D.R; d->ladder = D.ladder; d->improve = D.improve; d->cool_period = D.cool_period;
+        d->spe = D.spe; d->sweeps_total = D.sweeps_total; d->sweeps_done = D.sweeps_done;
+        d->key0 = D.key0; d->key1 = D.key1; d->interval = D.interval;
+        d->shrink = D.shrink; d->target_acc = D.target_acc; d->scale = D.scale; d->Lf = D.Lf;
+        d->L = D.L; d->L_arch = D.L_arch; d->L_box_arch = D.L_box_arch;
+        d->arch_valid = D.arch_valid; d->shrinks = D.shrinks; d->feasible = D.feasible;
+        d->fp64_rejects = D.fp64_rejects; d->ex_tried = D.ex_tried; d->ex_acc = D.ex_acc; d->certs = D.certs;
+    }
     return SIS_OK;
 }
 

@@ -278,26 +278,11 @@ int Server::admit_one(Req* r, Lane& ln) {
     const InstSpec& s = r->spec;
     const int n = s.n, R = s.p.replicas;
     InstDesc D{};
-    D.n = n; D.R = R; D.ladder = r->plan.ladder; D.G = r->plan.G;
+    std::vector<RepState> reps;
+    std::vector<SlotState> slots;
+    std::vector<int> ros;
+    instance_records(s, r->plan, D, reps, slots, ros);
     D.state_off = (int)r->sq_off; D.rep_off = (int)r->rep_off; D.arch_off = (int)r->arch_off;
-    D.M = r->plan.M; D.spe = r->plan.spe; D.spsh = s.p.sweeps_per_shift;
-    D.sweeps_total = s.p.sweeps; D.sweeps_done = 0;
-    D.L = s.L; D.L_arch = s.L; D.Lf = (float)s.L; D.scale = 1.0f;
-    D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
-    D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
-    D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
-    std::vector<RepState> reps(R);
-    std::vector<SlotState> slots(R);
-    std::vector<int> ros(R);
-    const int lad = D.ladder;
-    for (int k = 0; k < R; ++k) {
-        const int pos = k % lad;
-        const double T = lad == 1 ? s.p.tmin : s.p.tmin * std::pow(s.p.tmax / s.p.tmin, (double)pos / (lad - 1));
-        slots[k] = SlotState{(float)(1.4426950408889634 / T), (float)(std::sqrt(3.0) * s.p.move_sigma),
-                             (float)std::min(0.7794229, std::sqrt(3.0) * s.p.rot_sigma), (float)T};
-        reps[k] = RepState{0.0f, k, 0u, 0u};
-        ros[k] = k;
-    }
     h_inst[r->slot] = D;
     const std::vector<float4>& st = r->init;
     hipStream_t sm = ln.stream;
@@ -414,15 +399,15 @@ int Server::finalize_start(std::vector<Req*>& done) {
             job.n = D.n;
             job.iters = r->spec.p.polish_iters;
             job.out_off = 3ll * kMaxN * j;
+            double target_L;
+            finalize_source(D, job.L, target_L);
             if (D.arch_valid) {
                 job.src32 = d_arch + D.arch_off;
-                job.L = D.L_arch;
             } else {
-                // the best replica at the current box side, chosen on the device
+                // the best replica, chosen on the device
                 job.src32 = d_state + D.state_off;
                 job.reps = d_reps + D.rep_off;
                 job.R = D.R;
-                job.L = D.L;
             }
         }
         SRV_CHECK(hipMemcpyAsync(fb->d_jobs, fb->h_jobs, sizeof(FinalizeJob) * k, hipMemcpyHostToDevice, fin_stream));
@@ -445,7 +430,8 @@ int Server::finalize_done(FinBlock& fb) {
         const FinalizeOut& fo = fb.h_fo[j];
         FinalRecord rec;
         rec.q.assign(fb.h_q + 3 * kMaxN * j, fb.h_q + 3 * kMaxN * j + 3ll * D.n);
-        rec.target_L = D.arch_valid ? D.L_arch : D.L;
+        double box;
+        finalize_source(D, box, rec.target_L);
         rec.L_cert = fo.L_cert;
         rec.lambda = fo.lambda;
         rec.e32 = D.arch_valid ? 0.0 : fo.e32;

@@ -47,7 +47,7 @@ class SisParams(ctypes.Structure):
         ("sweeps_per_exchange", ctypes.c_int), ("replicas_per_group", ctypes.c_int),
         ("polish_iters", ctypes.c_int), ("init_mode", ctypes.c_int), ("flags", ctypes.c_int),
         ("rng_seed", ctypes.c_uint64), ("collective_moves", ctypes.c_int), ("kick_every", ctypes.c_int),
-        ("sweeps_per_shift", ctypes.c_int),
+        ("sweeps_per_shift", ctypes.c_int), ("improve", ctypes.c_int), ("cool_period", ctypes.c_int),
     ]
 
 
@@ -64,6 +64,7 @@ class SisResult(ctypes.Structure):
         ("status", ctypes.c_int), ("seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
         ("fp64_rejects", ctypes.c_int), ("kicks", ctypes.c_int),
         ("collective_tried", ctypes.c_uint64), ("collective_accepted", ctypes.c_uint64),
+        ("feasible_checkpoints", ctypes.c_uint64),
     ]
 
 
@@ -73,6 +74,32 @@ class SisServerCfg(ctypes.Structure):
         ("intervals_per_round", ctypes.c_int), ("certifier_threads", ctypes.c_int),
         ("certify", ctypes.c_int),
     ]
+
+
+# Per-replica and per-slot records of the engine (sis_rep_state, sis_slot_state).
+REP_DTYPE = np.dtype([("E", "<f4"), ("slot", "<i4"), ("acc", "<u4"), ("pad", "<u4"), ("Lc", "<f8"),
+                      ("lam", "<f8"), ("minx", "<f8"), ("miny", "<f8")])
+SLOT_DTYPE = np.dtype([("inv_t_log2e", "<f4"), ("step_xy", "<f4"), ("step_th", "<f4"), ("L", "<f4"),
+                       ("T", "<f4"), ("base_xy", "<f4"), ("base_th", "<f4"), ("pad", "<f4")])
+
+
+class SisLadderDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("R", ctypes.c_int), ("ladder", ctypes.c_int), ("improve", ctypes.c_int),
+        ("cool_period", ctypes.c_int), ("spe", ctypes.c_int),
+        ("sweeps_total", ctypes.c_int64), ("sweeps_done", ctypes.c_int64),
+        ("key0", ctypes.c_uint32), ("key1", ctypes.c_uint32), ("interval", ctypes.c_uint32),
+        ("shrink", ctypes.c_float), ("target_acc", ctypes.c_float), ("scale", ctypes.c_float),
+        ("Lf", ctypes.c_float),
+        ("L", ctypes.c_double), ("L_arch", ctypes.c_double), ("L_box_arch", ctypes.c_double),
+        ("arch_valid", ctypes.c_int), ("shrinks", ctypes.c_int), ("feasible", ctypes.c_int),
+        ("fp64_rejects", ctypes.c_int),
+        ("ex_tried", ctypes.c_uint64), ("ex_acc", ctypes.c_uint64), ("certs", ctypes.c_uint64),
+    ]
+
+
+def desc_dict(d) -> dict:
+    return {name: getattr(d, name) for name, _ in d._fields_}
 
 
 _lib = None
@@ -128,6 +155,14 @@ def lib() -> ctypes.CDLL:
                                 fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, fp, ip, fp, u64p]
     L.sis_dev_polish.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+    vp = ctypes.c_void_p
+    L.sis_dev_ladder.argtypes = [ctypes.POINTER(SisLadderDesc), vp, vp, ip, fp, fp]
+    L.sis_dev_batch_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SisLadderDesc), vp, vp, ip,
+                                      fp, fp]
+    L.sis_dev_launch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, fp, ctypes.c_float, ctypes.c_int,
+                                 fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, fp, ip,
+                                 fp, u64p, vp]
     L.sis_dev_zero64.argtypes = [ctypes.c_int, ctypes.c_double, fp, ip, dp]
     _lib = L
     return L
@@ -298,6 +333,25 @@ class BatchSession:
         if rc:
             _err(rc)
         return [_result(res[j], outs[j]) for j in range(self.k)]
+
+    def state(self, k: int = 0) -> dict:
+        """Device state of instance k: descriptor fields, replica and slot records,
+        slot owners, replica states (R, n, 4) and archive (n, 4)."""
+        d = SisLadderDesc()
+        rc = lib().sis_dev_batch_state(self.h, k, ctypes.byref(d), None, None, None, None, None)
+        if rc:
+            _err(rc)
+        R, n = d.R, d.n
+        reps = np.zeros(R, REP_DTYPE)
+        slots = np.zeros(R, SLOT_DTYPE)
+        ros = np.zeros(R, np.int32)
+        st = np.zeros((R, n, 4), np.float32)
+        arch = np.zeros((n, 4), np.float32)
+        rc = lib().sis_dev_batch_state(self.h, k, ctypes.byref(d), reps.ctypes.data, slots.ctypes.data,
+                                       _ptr(ros, ctypes.c_int), _ptr(st, ctypes.c_float), _ptr(arch, ctypes.c_float))
+        if rc:
+            _err(rc)
+        return dict(desc=desc_dict(d), reps=reps, slots=slots, ros=ros, state=st, arch=arch)
 
     def close(self):
         if self.h:
@@ -474,3 +528,45 @@ def dev_polish(q: np.ndarray, L: float, iters: int):
     if rc:
         _err(rc)
     return dict(e_before=eb.value, e_after=ea.value, lam=lam.value, L_cert=lc.value, q=out)
+
+
+def dev_ladder(desc: dict, reps: np.ndarray, slots: np.ndarray, ros: np.ndarray, xycs: np.ndarray,
+               arch: np.ndarray):
+    """One exchange step on the device (k_ladder) on copies of the given records;
+    returns (desc, reps, slots, ros, xycs, arch) after it."""
+    d = SisLadderDesc(**desc)
+    reps = np.ascontiguousarray(reps, REP_DTYPE).copy()
+    slots = np.ascontiguousarray(slots, SLOT_DTYPE).copy()
+    ros = np.ascontiguousarray(ros, np.int32).copy()
+    st = np.ascontiguousarray(xycs, np.float32).copy()
+    ar = np.ascontiguousarray(arch, np.float32).copy()
+    rc = lib().sis_dev_ladder(ctypes.byref(d), reps.ctypes.data, slots.ctypes.data, _ptr(ros, ctypes.c_int),
+                              _ptr(st, ctypes.c_float), _ptr(ar, ctypes.c_float))
+    if rc:
+        _err(rc)
+    return desc_dict(d), reps, slots, ros, st, ar
+
+
+def dev_launch(xycs: np.ndarray, Lgrid: float, box, cell: float, inv_t_log2e, step_xy, step_th, key0: int,
+               key1: int, sweep0: int, sweeps: int, group: int = 1, collective: int = 0, kick_every: int = 0,
+               interval: int = 0, shift_every: int = 4, certify: bool = False):
+    """One sweep launch with a box per replica; with certify, every replica's FP64
+    certified side at the end.  Returns (state, accepted, energy, counts[6], reps)."""
+    R, n, _ = xycs.shape
+    st = np.ascontiguousarray(xycs, dtype=np.float32).copy()
+    bx = np.ascontiguousarray(np.broadcast_to(np.asarray(box, np.float32), (R,)))
+    ik = np.ascontiguousarray(np.broadcast_to(np.asarray(inv_t_log2e, np.float32), (R,)))
+    sx = np.ascontiguousarray(np.broadcast_to(np.asarray(step_xy, np.float32), (R,)))
+    sth = np.ascontiguousarray(np.broadcast_to(np.asarray(step_th, np.float32), (R,)))
+    acc = np.zeros(R, dtype=np.int32)
+    en = np.zeros(R, dtype=np.float32)
+    counts = np.zeros(6, dtype=np.uint64)
+    reps = np.zeros(R, REP_DTYPE)
+    rc = lib().sis_dev_launch(n, R, Lgrid, _ptr(bx, ctypes.c_float), cell, group, _ptr(ik, ctypes.c_float),
+                              _ptr(sx, ctypes.c_float), _ptr(sth, ctypes.c_float), key0, key1, sweep0, sweeps,
+                              shift_every, collective, kick_every, interval, int(certify),
+                              _ptr(st, ctypes.c_float), _ptr(acc, ctypes.c_int), _ptr(en, ctypes.c_float),
+                              _ptr(counts, ctypes.c_uint64), reps.ctypes.data)
+    if rc:
+        _err(rc)
+    return st, acc, en, counts, reps

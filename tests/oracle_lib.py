@@ -31,7 +31,24 @@ class PtsaOut(ctypes.Structure):
 class SweepCfg(ctypes.Structure):
     _fields_ = [("n", i32), ("L", f32), ("w", f32), ("step_xy", f32), ("step_th", f32),
                 ("inv_t_log2e", f32), ("key0", u32), ("key1", u32), ("replica", u32), ("sweep", u32),
-                ("sweeps_per_shift", u32)]
+                ("sweeps_per_shift", u32), ("dE32", P(f32)), ("dE64", P(f64)), ("depth64", P(f64))]
+
+
+# o_rep_state / o_slot_state / o_ladder_desc of sis_oracle.h
+REP_DTYPE = np.dtype([("E", "<f4"), ("slot", "<i4"), ("acc", "<u4"), ("pad", "<u4"), ("Lc", "<f8"),
+                      ("lam", "<f8"), ("minx", "<f8"), ("miny", "<f8")])
+SLOT_DTYPE = np.dtype([("inv_t_log2e", "<f4"), ("step_xy", "<f4"), ("step_th", "<f4"), ("L", "<f4"),
+                       ("T", "<f4"), ("base_xy", "<f4"), ("base_th", "<f4"), ("pad", "<f4")])
+
+
+class LadderDesc(ctypes.Structure):
+    _fields_ = [("n", i32), ("R", i32), ("ladder", i32), ("improve", i32), ("cool_period", i32), ("spe", i32),
+                ("sweeps_total", ctypes.c_int64), ("sweeps_done", ctypes.c_int64),
+                ("key0", u32), ("key1", u32), ("interval", u32),
+                ("shrink", f32), ("target_acc", f32), ("scale", f32), ("Lf", f32),
+                ("L", f64), ("L_arch", f64), ("L_box_arch", f64),
+                ("arch_valid", i32), ("shrinks", i32), ("feasible", i32), ("fp64_rejects", i32),
+                ("ex_tried", u64), ("ex_acc", u64), ("certs", u64)]
 
 
 def ptr(a, t):
@@ -45,6 +62,8 @@ class Oracle:
         L.o_philox.argtypes = [P(u32), P(u32), P(u32)]
         L.o_u01f.argtypes = [u32]
         L.o_u01f.restype = f32
+        L.o_u01d.argtypes = [u32, u32]
+        L.o_u01d.restype = f64
         L.o_ref_cell.argtypes = [f32, f32]
         L.o_ref_depth32.argtypes = [P(f32), P(f32)]
         L.o_ref_depth32.restype = f32
@@ -84,6 +103,11 @@ class Oracle:
         L.o_collective32_model.argtypes = [P(SweepCfg), i32, P(f32), P(f32), P(f32), P(f32), P(f32)]
         L.o_kick32_model.argtypes = [P(SweepCfg), u32, i32, P(f32), P(f32), P(f32), P(f32), P(f32)]
         L.o_grid_packing.argtypes = [i32, P(f64)]
+        L.o_certify_xycs.argtypes = [i32, P(f32), P(f64)]
+        L.o_ladder_model.argtypes = [P(LadderDesc), ctypes.c_void_p, ctypes.c_void_p, P(i32), P(f32), P(f32)]
+        L.o_exp2_fixed64.argtypes = [f64]
+        L.o_exp2_fixed64.restype = f64
+        L.o_cool_factor.argtypes = [u32, i32, P(f64), P(f64)]
 
     # -- thin wrappers -------------------------------------------------------
     def philox(self, ctr, key):
@@ -117,12 +141,48 @@ class Oracle:
         return q, e.value, it
 
     def sweep_model(self, x, y, c, s, L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep,
-                    shift_every=1):
+                    shift_every=1, per_move=False):
+        """One sweep in place.  With per_move, also returns the FP32 energy change of
+        every square's proposal, its FP64 recompute over all pairs and the summed
+        overlap depths (NaN where the proposal was rejected before the stencil)."""
         cfg = SweepCfg(len(x), L, w, step_xy, step_th, inv_t_log2e, key0, key1, replica, sweep, shift_every)
+        moves = None
+        if per_move:
+            n = len(x)
+            moves = (np.zeros(n, np.float32), np.zeros(n, np.float64), np.zeros(n, np.float64))
+            cfg.dE32, cfg.dE64, cfg.depth64 = ptr(moves[0], f32), ptr(moves[1], f64), ptr(moves[2], f64)
         counts = np.zeros(3, dtype=np.uint64)
         acc = self.lib.o_sweep32_model(ctypes.byref(cfg), ptr(x, f32), ptr(y, f32), ptr(c, f32), ptr(s, f32),
                                        ptr(counts, u64))
+        if per_move:
+            return acc, counts, moves
         return acc, counts
+
+    def certify_xycs(self, q):
+        """FP64 certified side of an (n, 4) float32 state: (side, lam, minx, miny)."""
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros(4)
+        self.lib.o_certify_xycs(q.shape[0], ptr(q, f32), ptr(out, f64))
+        return tuple(float(v) for v in out)
+
+    def ladder(self, desc: dict, reps, slots, ros, xycs, arch):
+        """One exchange step (o_ladder_model) on copies; returns the updated
+        (desc, reps, slots, ros, xycs, arch) and whether it improved."""
+        d = LadderDesc(**desc)
+        reps = np.ascontiguousarray(reps, REP_DTYPE).copy()
+        slots = np.ascontiguousarray(slots, SLOT_DTYPE).copy()
+        ros = np.ascontiguousarray(ros, np.int32).copy()
+        st = np.ascontiguousarray(xycs, np.float32).copy()
+        ar = np.ascontiguousarray(arch, np.float32).copy()
+        imp = self.lib.o_ladder_model(ctypes.byref(d), reps.ctypes.data, slots.ctypes.data, ptr(ros, i32),
+                                      ptr(st, f32), ptr(ar, f32))
+        out = {name: getattr(d, name) for name, _ in LadderDesc._fields_}
+        return out, reps, slots, ros, st, ar, bool(imp)
+
+    def cool_factor(self, k, P):
+        c, r = ctypes.c_double(), ctypes.c_double()
+        self.lib.o_cool_factor(k, P, ctypes.byref(c), ctypes.byref(r))
+        return c.value, r.value
 
     def energy32(self, x, y, c, s, L, w):
         return self.lib.o_energy32_model(len(x), L, w, ptr(x, f32), ptr(y, f32), ptr(c, f32), ptr(s, f32))
