@@ -159,6 +159,17 @@ int sis_portfolio(const sis_instance* inst, const sis_params* params, int jitter
  * (cuda/packing.cu:823-853) driven once per n by cuda/cpp_loop.py:47-80. */
 int sis_solve(const sis_instance* inst, const sis_params* params, sis_result* out);
 
+/* The plain form (sis_solve, same results bit for bit) without any device
+ * allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every device buffer comes from one
+ * 32 MiB array per device that the library holds until the process ends;
+ * requests whose state needs more (about n x replicas > 2 million squares)
+ * are rejected with SIS_EINVAL.  Calls on one device share the array: the
+ * library serialises them.  Work runs on the null stream with synchronous
+ * copies; device_seconds is host time around the launches. */
+int sis_solve_noalloc(const sis_instance* inst, const sis_params* params, sis_result* out);
+
 /* Batched form: `count` instances of any n, all replica tiles in one launch per
  * interval, the whole schedule captured into a HIP graph.  Replaces the
  * reference's one-process-per-n pool (cuda/cpp_loop.py:82-106). */

@@ -124,6 +124,8 @@ struct DevBuf {
     operator T*() const { return p; }
 };
 int ensure_device(int device);
+// Address of the allocation-free solve's device buffer on `device` (current).
+int noalloc_setup(int device);
 const DeviceInfo& device_info();
 void default_params(sis_params* p);
 int validate(const sis_instance* in, const sis_params* p);

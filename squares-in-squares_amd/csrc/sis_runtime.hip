@@ -57,6 +57,8 @@ int ensure_device(int device) {
     for (int threads : {kSweepThreads, kSweepThreadsBig})
         HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)prop.sharedMemPerBlock));
+    // the allocation-free solve's device buffer: its address on this device
+    if (int rc = noalloc_setup(id)) return rc;
     g_dev.cus = prop.multiProcessorCount;
     g_dev.lds_per_block = (size_t)prop.sharedMemPerBlock;
     g_dev.name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;

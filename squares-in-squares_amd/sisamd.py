@@ -123,6 +123,8 @@ def lib() -> ctypes.CDLL:
     L.sis_default_params.restype = None
     L.sis_portfolio.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.c_int, dp]
     L.sis_solve.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), ctypes.POINTER(SisResult)]
+    L.sis_solve_noalloc.argtypes = [ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams),
+                                    ctypes.POINTER(SisResult)]
     L.sis_solve_batch.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams),
                                   ctypes.POINTER(SisResult)]
     L.sis_autotune.argtypes = [ctypes.POINTER(SisInstance), ctypes.c_int, ctypes.POINTER(SisParams), ctypes.c_int,
@@ -243,6 +245,20 @@ def solve(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Re
     r = SisResult()
     r.q_out = _ptr(out, ctypes.c_double)
     rc = lib().sis_solve(ctypes.byref(inst), ctypes.byref(p), ctypes.byref(r))
+    if rc:
+        _err(rc)
+    del keep
+    return _result(r, out)
+
+
+def solve_noalloc(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Result:
+    """The plain form without device allocation (sis_solve_noalloc)."""
+    p = params if params is not None else default_params(**kw)
+    inst, keep = _instance(n, L, q)
+    out = np.zeros((n, 3), dtype=np.float64)
+    r = SisResult()
+    r.q_out = _ptr(out, ctypes.c_double)
+    rc = lib().sis_solve_noalloc(ctypes.byref(inst), ctypes.byref(p), ctypes.byref(r))
     if rc:
         _err(rc)
     del keep
