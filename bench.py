@@ -5,8 +5,9 @@ Workload (the plain GPU PT-SA configuration): n = 50 unit squares, 256 replicas
 on one geometric temperature ladder 1e-5..5e-2, translation and rotation step
 sigma 0.05 adapted to 30 % acceptance, Philox seed 7, centres uniform in
 [0.5, L0-0.5]^2 with L0 = ceil(sqrt(50)) + 1 = 9, angles uniform, replica
-exchange every 10 sweeps, box shrink 1e-4 on zero overlap, final FP64 polish
-and certified-side export.
+exchange every 10 sweeps, the reference's improvement path (best certified
+side, per-temperature targets best - 1e-4 * mults, ladder heads restarted
+from the best configuration), final FP64 polish and certified-side export.
 
 One step = 1000 sweeps of every replica (100 exchange intervals of 10 sweeps,
 run in the plain form: one launch sequence, host synchronisation after every
@@ -40,7 +41,21 @@ PEAK_FP32_TFLOPS = 157.3
 def c2_params(sis, sweeps_total):
     return sis.default_params(tmin=1e-5, tmax=5e-2, move_sigma=0.05, rot_sigma=0.05, target_accept=0.3,
                               shrink=1e-4, replicas=REPLICAS, ladder=0, sweeps=sweeps_total,
-                              sweeps_per_exchange=SPE, polish_iters=200, rng_seed=7, init_mode=1)
+                              sweeps_per_exchange=SPE, polish_iters=200, rng_seed=7, init_mode=1, improve=1)
+
+
+def host_cpu():
+    """Host cores visible to this process and the CPU model."""
+    model = "unknown"
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": avail, "cpu_count": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(target_s: float = 12.0):
@@ -50,7 +65,7 @@ def cpu_baseline(target_s: float = 12.0):
     import oracle_lib
 
     o = oracle_lib.load()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(16, host_cpu()["nproc"]))
     probe = 20
     r = o.ptsa(n=N, L0=L0, replicas=REPLICAS, tmin=1e-5, tmax=5e-2, sweeps=probe, swap_every=SPE, shrink=1e-4,
                seed=7, threads=threads)
@@ -60,7 +75,57 @@ def cpu_baseline(target_s: float = 12.0):
                seed=7, threads=threads)
     return {"value": r["proposals"] / r["seconds"], "unit": "proposals/s", "cores": threads, "kind": "port",
             "sample": f"n=50, 256 replicas, {sweeps} sweeps ({r['proposals']} proposals) of the FP64 C++ "
-                      f"PT-SA in {r['seconds']:.2f} s on {threads} threads"}
+                      f"PT-SA in {r['seconds']:.2f} s on {threads} threads", "host": host_cpu()}
+
+
+def reference_certifier_check(packings):
+    """Every served packing through the reference's own interval certifier
+    (built from its source into oracle/_ref), outside any timed region, as the
+    checker: the export gate of cuda/demo.py:25-40 (centres dilated by
+    1.000002, 1.00001, 1.0001 until accepted).  One CSV holds one puzzle per n,
+    so the packings go in rounds of distinct n.  Returns counts, or None when
+    the certifier was not built."""
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "oracle" / "_ref" / "certifier"
+    if not exe.exists():
+        return None
+    pending = list(range(len(packings)))
+    passed = {}
+    threads = str(max(1, min(16, host_cpu()["nproc"])))
+    t0 = time.perf_counter()
+    for dil in (1.000002, 1.00001, 1.0001):
+        left = pending
+        while left:
+            seen, batch, rest = set(), [], []
+            for k in left:
+                n = packings[k].shape[0]
+                (rest if n in seen else batch).append(k)
+                seen.add(n)
+            with tempfile.TemporaryDirectory() as d:
+                csv, det = Path(d) / "p.csv", Path(d) / "d.json"
+                lines, row = ["id,n,square_id,x,y,angle"], 0
+                for k in batch:
+                    q = packings[k]
+                    for i, (x, y, a) in enumerate(q):
+                        lines.append(f"{row},{q.shape[0]},{i},{x * dil:.17g},{y * dil:.17g},{a:.17g}")
+                        row += 1
+                csv.write_text("\n".join(lines) + "\n")
+                maxn = max(packings[k].shape[0] for k in batch)
+                subprocess.run([str(exe), str(csv), "--max-n", str(maxn), "--threads", threads, "--details",
+                                str(det)], check=True, capture_output=True, timeout=600)
+                res = json.loads(det.read_text())["results"]
+                for k in batch:
+                    r = res[packings[k].shape[0] - 1]
+                    if r["L"] is not None:
+                        passed[k] = (dil, r["L"])
+            left = rest
+        pending = [k for k in pending if k not in passed]
+        if not pending:
+            break
+    return {"checked": len(packings), "passed": len(passed), "seconds": time.perf_counter() - t0,
+            "dilations": {str(d): sum(1 for v in passed.values() if v[0] == d) for d in (1.000002, 1.00001, 1.0001)}}
 
 
 def traffic_per_launch():
@@ -152,7 +217,8 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
     ns = rng.integers(20, 201, n_inst)
     insts = [(int(n), float(math.ceil(math.sqrt(n)) + 1), None) for n in ns]
     params = [sis.default_params(tmin=1e-5, tmax=5e-2, replicas=64, sweeps=sweeps, sweeps_per_exchange=SPE,
-                                 rng_seed=1000 + k, init_mode=1, polish_iters=200) for k in range(n_inst)]
+                                 rng_seed=1000 + k, init_mode=1, polish_iters=200, shrink=1e-4, improve=1)
+              for k in range(n_inst)]
     tune = None
     if autotune:
         # autotuner on a 64-instance slice of the stream, 20 probe sweeps per candidate
@@ -176,8 +242,9 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
     c_per = sum(r.candidates for r in res) / props
     p_per = sum(r.near_pairs for r in res) / props
     feas = sum(r.feasible for r in res)
-    # plain form on a sample of the same stream
-    plain_props, plain_wall, sweep_ms = 0, 0.0, 0.0
+    # plain form on a sample of the same stream, same sweeps: same results
+    plain_props, plain_wall, sweep_ms, same = 0, 0.0, 0.0, 0
+    plain_L = []
     for k in range(plain_sample):
         s = sis.BatchSession([insts[k]], params[k])
         t1 = time.perf_counter()
@@ -185,15 +252,25 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
         plain_wall += time.perf_counter() - t1
         plain_props += insts[k][0] * 64 * sweeps
         sweep_ms += sm
+        pr = s.finalize()[0]
         s.close()
+        plain_L.append(pr.L_cert)
+        same += bool(np.array_equal(pr.q, res[k].q) and pr.L_cert == res[k].L_cert)
     plain = plain_props / plain_wall if plain_wall > 0 else None
+    gain = [1.0 - r.L_cert / L for r, (_, L, _) in zip(res, insts) if r.L_cert < 1e8]
     return {
         "workload": f"{n_inst} instances, n ~ U{{20..200}} (seed 3), 64 replicas each, {sweeps} sweeps each, "
-                    "exchange every 10 sweeps; batched = one HIP-graph schedule over all instances",
+                    "exchange every 10 sweeps (autotuned below), random starts in a box of side ceil(sqrt n) + 1, "
+                    "the reference's improvement path; batched = one HIP-graph schedule over all instances",
         "batched_proposals_per_s": batched, "plain_proposals_per_s": plain,
         "ratio": batched / plain if plain else None,
         "plain_sample_instances": plain_sample, "timed_proposals": timed, "wall_s": wall,
         "device_ms": dev_ms, "feasible_instances": feas, "autotuned": tune,
+        "quality": {"instances_improved": feas, "mean_relative_side_reduction": float(np.mean(gain)) if gain else None,
+                    "mean_L_cert_sample_batched": float(np.mean([r.L_cert for r in res[:plain_sample]]))
+                    if plain_sample else None,
+                    "mean_L_cert_sample_plain": float(np.mean(plain_L)) if plain_L else None,
+                    "plain_sample_identical": same},
         "flop_per_proposal": 12.0 * c_per + 28.0 * p_per + 45.0,
         "achieved_tflops_wall": timed * (12.0 * c_per + 28.0 * p_per + 45.0) / wall / 1e12,
     }
@@ -213,15 +290,19 @@ def grid_seed(n: int):
     return float(w * 1.00002), q
 
 
+SIZE_CLASSES = ((10, 63), (64, 255), (256, 1000))
+
+
 def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seconds: float, seed: int = 5):
     """Serving stream: Poisson arrivals at `rate`/s for `duration` s; n
     log-uniform in 10..1000; sweeps U(5e4, 5e5) x sweep_scale; replicas
     U{32, 64, ..., 256}; half the requests with a 200-iteration FP64 polish,
-    half without; every request starts from its certified grid seed and every
-    result is certified exactly.  Reports p50/p99 time to a certified L and
-    aggregate proposals/s, and the plain form (requests one at a time, host
-    sync per interval) on the same requests in arrival order for about
-    `plain_seconds`."""
+    half without; every request starts from its certified grid seed, runs the
+    reference's improvement path and is certified exactly.  Reports p50/p99
+    time to a certified L and aggregate proposals/s, and the plain form
+    (requests one at a time, host sync per interval) on the same requests,
+    about `plain_seconds` split over three size classes (n < 64, 64-255,
+    >= 256), each class in arrival order.  Returns the line and the packings."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -236,17 +317,21 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         R = int(rng.integers(1, 9)) * 32
         polish = 200 if rng.random() < 0.5 else 0
         reqs.append((t, n, sweeps, R, polish, 100 + len(reqs)))
-    cpu = max(1, min(16, os.cpu_count() or 1))
+    cpu = max(1, min(16, host_cpu()["nproc"]))
     srv = sis.Server(max_instances=1024, intervals_per_round=5, certifier_threads=cpu, certify=True,
                      hbm_budget_bytes=200e9)
+
+    def params(n, sweeps, R, polish, s):
+        return sis.default_params(replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=0,
+                                  tmin=1e-5, tmax=5e-2, shrink=1e-4, improve=1)
+
     tickets = []
     t0 = time.perf_counter()
     for (ta, n, sweeps, R, polish, s) in reqs:
         while time.perf_counter() - t0 < ta:
             time.sleep(0.0005)
         L0, q0 = grid_seed(n)
-        tickets.append(srv.submit(n, L0, q0, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s,
-                                  init_mode=0, tmin=1e-5, tmax=5e-2))
+        tickets.append(srv.submit(n, L0, q0, params=params(n, sweeps, R, polish, s)))
     results = {}
     while len(results) < len(tickets):
         for tk in tickets:
@@ -256,12 +341,12 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
                     results[tk] = r
         time.sleep(0.002)
     wall = time.perf_counter() - t0
-    lat, cert_ok, Ls = [], 0, {}
-    for tk in tickets:
-        _, _, tc, L50 = srv.times(tk)
+    lat, cert_ok, done_at = [], 0, []
+    for tk, (ta, *_rest) in zip(tickets, reqs):
+        _, _, tc, _L50 = srv.times(tk)
         lat.append(tc)
+        done_at.append(ta + tc)
         cert_ok += results[tk].status == 0
-        Ls[tk] = L50
         srv.release(tk)
     srv.close()
     props = sum(results[tk].proposals for tk in tickets)
@@ -269,33 +354,113 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
                 for tk in tickets)
     offered = sum(n * R * sweeps for (_, n, sweeps, R, _, _) in reqs)
     lat = np.array(lat)
-    # plain form on the same requests in arrival order; identical results
-    pp, pw, k, same = 0, 0.0, 0, 0
-    while k < len(reqs) and pw < plain_seconds:
-        ta, n, sweeps, R, polish, s = reqs[k]
-        L0, q0 = grid_seed(n)
-        t1 = time.perf_counter()
-        r = sis.solve(n, L0, q0, replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=0,
-                      tmin=1e-5, tmax=5e-2)
-        pw += time.perf_counter() - t1
-        pp += r.proposals
-        same += bool(np.array_equal(r.q, results[tickets[k]].q) and r.L_cert == results[tickets[k]].L_cert)
-        k += 1
     served = props / wall
-    return {
+    # plain form on the same requests: about plain_seconds / 3 per size class,
+    # in arrival order; results identical to the served ones
+    classes = []
+    pp_all, pw_all, same_all, k_all = 0, 0.0, 0, 0
+    for lo, hi in SIZE_CLASSES:
+        idx = [k for k, r in enumerate(reqs) if lo <= r[1] <= hi]
+        pp, pw, same, done = 0, 0.0, 0, 0
+        for k in idx:
+            if pw >= plain_seconds / len(SIZE_CLASSES):
+                break
+            ta, n, sweeps, R, polish, s = reqs[k]
+            L0, q0 = grid_seed(n)
+            t1 = time.perf_counter()
+            r = sis.solve(n, L0, q0, params=params(n, sweeps, R, polish, s))
+            pw += time.perf_counter() - t1
+            pp += r.proposals
+            same += bool(np.array_equal(r.q, results[tickets[k]].q) and r.L_cert == results[tickets[k]].L_cert)
+            done += 1
+        cprops = sum(results[tickets[k]].proposals for k in idx)
+        span = (max(done_at[k] for k in idx) - min(reqs[k][0] for k in idx)) if idx else 0.0
+        c_served = cprops / span if span > 0 else None
+        c_plain = pp / pw if pw > 0 else None
+        classes.append({"n": f"{lo}-{hi}", "requests": len(idx), "proposals": cprops,
+                        "served_proposals_per_s": c_served, "plain_proposals_per_s": c_plain,
+                        "ratio_served_vs_plain": c_served / c_plain if c_served and c_plain else None,
+                        "p50_time_to_certified_L_s": float(np.percentile(lat[idx], 50)) if idx else None,
+                        "plain_requests": done, "plain_results_identical": same})
+        pp_all += pp
+        pw_all += pw
+        same_all += same
+        k_all += done
+    # the stream's own mix, at the plain rates of its classes
+    plain_mix = None
+    if all(c["plain_proposals_per_s"] for c in classes if c["requests"]):
+        plain_mix = props / sum(c["proposals"] / c["plain_proposals_per_s"] for c in classes if c["requests"])
+    line = {
         "workload": f"Poisson arrivals {rate}/s for {duration} s ({len(reqs)} requests), n log-uniform 10..1000, "
                     f"sweeps U(5e4,5e5) x {sweep_scale}, replicas U{{32..256}}, polish 0/200 mixed, grid seeds, "
-                    "every result certified",
+                    "the reference's improvement path, every result certified",
         "requests": len(reqs), "certified": cert_ok,
         "offered_proposals_per_s": offered / duration, "served_proposals_per_s": served,
         "p50_time_to_certified_L_s": float(np.percentile(lat, 50)),
         "p99_time_to_certified_L_s": float(np.percentile(lat, 99)),
-        "makespan_s": wall, "plain_proposals_per_s": pp / pw if pw else None,
-        "ratio_served_vs_plain": served / (pp / pw) if pw else None,
-        "plain_requests": k, "plain_results_identical": same,
+        "makespan_s": wall, "plain_proposals_per_s": pp_all / pw_all if pw_all else None,
+        "ratio_served_vs_plain": served / (pp_all / pw_all) if pw_all else None,
+        "plain_proposals_per_s_stream_mix": plain_mix,
+        "ratio_served_vs_plain_stream_mix": served / plain_mix if plain_mix else None,
+        "plain_requests": k_all, "plain_results_identical": same_all,
+        "size_classes": classes,
         "served_tflops": flops / wall / 1e12,
         "fraction_of_fp32_peak": flops / wall / (PEAK_FP32_TFLOPS * 1e12),
     }
+    return line, [results[tk].q for tk in tickets]
+
+
+def tilted_five_rows():
+    """The tilted five of the reference's demo seed for n = 5
+    (src/squares/optimizer.py:13-36): four corner squares and a central one at
+    45 degrees in a box of side 2 + 1/sqrt(2), centres dilated by 1.000000001."""
+    side = 2 + 1 / math.sqrt(2)
+    f = 1.000000001
+    pts = [(0.5, 0.5, 0.0), (side - 0.5, 0.5, 0.0), (0.5, side - 0.5, 0.0), (side - 0.5, side - 0.5, 0.0),
+           (side / 2, side / 2, math.pi / 4)]
+    return [(x * f, y * f, a) for x, y, a in pts]
+
+
+def reference_seed_leg(sis):
+    """The drop-in `packing` binary on the reference's demo workload
+    (cuda/demo.py:11-23, 72-86: grid seeds, the tilted five for n = 5,
+    coordinates and side x 1.00002, `packing seed 20000 32 256 42`): certified
+    L (this engine's exact certifier) beside the recorded target_L of
+    cuda/results/raw{n}.json, and with `steps` counted as sweeps."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    exe = ROOT / "squares-in-squares_amd" / "bin" / "packing"
+    rows_out = []
+    with tempfile.TemporaryDirectory() as d:
+        for n in (5, 10, 11, 17, 50):
+            if n == 5:
+                q = np.asarray(tilted_five_rows())
+            else:
+                w = math.isqrt(n - 1) + 1
+                q = np.asarray([((i % w) + 0.5, (i // w) + 0.5, 0.0) for i in range(n)])
+            side = float(sis.certify_strings([tuple(format(float(v), ".17g") for v in r) for r in q]))
+            seed = Path(d) / f"seed{n}.txt"
+            seed.write_text(f"{n} {side * 1.00002:.12g}\n" + "".join(
+                f"{x * 1.00002:.12g} {y * 1.00002:.12g} {a:.12g}\n" for x, y, a in q))
+            rec = json.loads((ROOT / "tests" / "golden" / "raw" / f"raw{n}.json").read_text())
+            row = {"n": n, "seed_L": side * 1.00002, "recorded_target_L": rec["target_L"]}
+            for name, env in (("reference_argv", {}), ("steps_as_sweeps", {"PACK_SWEEP_STEPS": "1"})):
+                out = Path(d) / f"raw{n}_{name}.json"
+                t0 = time.perf_counter()
+                subprocess.run([str(exe), str(seed), "20000", "32", "256", "42", str(out)], check=True,
+                               env={**os.environ, **env}, capture_output=True, timeout=600)
+                wall = time.perf_counter() - t0
+                raw = json.loads(out.read_text())
+                try:
+                    dil, L = sis.export_certify(np.asarray(raw["q"]))
+                except Exception as e:  # reported, not hidden
+                    dil, L = None, f"not certified: {e}"
+                row[name] = {"certified_L": L, "dilation": dil, "seconds": wall, "proposals": raw["proposals"]}
+            rows_out.append(row)
+    return rows_out
 
 
 def main():
@@ -307,7 +472,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-instances", type=int, default=512)
-    ap.add_argument("--batched-sweeps", type=int, default=400)
+    ap.add_argument("--batched-sweeps", type=int, default=20000)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-served", action="store_true", help="skip the serving leg")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
@@ -318,6 +483,7 @@ def main():
     ap.add_argument("--plain-seconds", type=float, default=8.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the n=500/1000 precision leg")
     ap.add_argument("--precision-sweeps", type=int, default=10000)
+    ap.add_argument("--no-reference-seeds", action="store_true", help="skip the demo-seed quality leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (requests shard across GPUs "
@@ -390,9 +556,10 @@ def main():
             "device": dev, "compute_units": cus,
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_sweep<64>",
-            "roof": "FP32 compute peak; the sweep runs on the FP32 vector ALUs (no dense contraction on this "
-                    "path, no matrix-core work); HBM traffic is far below its roof",
+            "bound": "valu", "kernel": "k_sweep<64>",
+            "roof": "FP32 vector ALU peak (157.3 TFLOP/s); no dense contraction on this path, so no matrix-core "
+                    "work; HBM traffic is far below its roof; at this load (one wavefront per compute unit) the "
+                    "launch is latency-bound",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
             "traffic": traffic_per_launch(),
@@ -408,13 +575,19 @@ def main():
     }
     if not a.no_batched:
         line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
+    packings = []
     if not a.no_served:
-        line["served"] = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale, a.plain_seconds)
+        line["served"], packings = served_leg(sis, a.serve_rate, a.serve_seconds, a.serve_sweep_scale,
+                                              a.plain_seconds)
     if not a.no_precision:
         line["precision"] = precision_leg(sis, a.precision_sweeps)
+    if not a.no_reference_seeds:
+        line["reference_seeds"] = reference_seed_leg(sis)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         line["cpu_baseline"]["c1_exact"] = cpu_c1(sis)
+        if packings:
+            line["cpu_baseline"]["served_reference_certifier"] = reference_certifier_check(packings)
     print(json.dumps(line), flush=True)
 
 

@@ -82,6 +82,10 @@ typedef struct sis_params {
     int cool_period;            /* reheating cycle of cuda/packing.cu:516-517,756-758 in exchange
                                    intervals: temperatures times 0.002^((k % P) / (P - 1)) and step
                                    sizes times its square root in interval k; 0 = none (P >= 2) */
+    int certify_every;          /* improve = 1: the certified sides are computed at the end of every
+                                   k-th sweep launch (interval k - 1, 2k - 1, ...) and of the last one,
+                                   1..1e6, default 1; the reference computes them once per epoch
+                                   (cuda/packing.cu:506, 689) */
 } sis_params;
 
 /* Result of one instance.  Replaces raw{n}.json (cuda/packing.cu:859-865). */
@@ -90,7 +94,8 @@ typedef struct sis_result {
     double target_L;            /* box side at which the exported configuration had zero overlap */
     double L_cert;              /* bounding side of q_out (FP64 certified-side test) */
     double lambda;              /* centre dilation applied by the certified-side test */
-    double e32;                 /* FP32 energy of the source configuration */
+    double e32;                 /* FP32 energy of the source configuration at the end of the sweep
+                                   launch it came from (0 for a seed that was never improved on) */
     double e64_before;          /* FP64 energy of the same configuration before polish */
     double e64_after;           /* FP64 energy after polish */
     double e_gap;               /* |e32 - e64_before| */
@@ -138,6 +143,15 @@ int sis_last_error_code(void);
 /* Compute units and marketing name of the device in use (the reference prints
  * prop.multiProcessorCount and prop.name into raw.json, cuda/packing.cu:862-863). */
 int sis_device_info(int* compute_units, char* name, int name_len);
+
+/* Sweep workgroups of this instance that one compute unit holds at once, by
+ * the HIP occupancy query for the tile shape the engine picks when the
+ * instance runs alone (replicas per workgroup, LDS), and those replicas per
+ * workgroup.  Replaces cudaOccupancyMaxActiveBlocksPerMultiprocessor for the
+ * search kernel (cuda/packing.cu:790-794), whose result the reference reports
+ * as blocks_per_sm and uses to size its chains. */
+int sis_occupancy(const sis_instance* inst, const sis_params* params, int* blocks_per_cu,
+                  int* replicas_per_block);
 
 /* Defaults of cuda/packing.cu:777-786 mapped to this engine. */
 void sis_default_params(sis_params* p);

@@ -12,10 +12,19 @@
 // certified side on the GPU).
 //
 // Mapping of the reference arguments onto this engine:
-//   epochs  exchange rounds of the search (0: 50/40/30 by n; >120 capped to 80)
-//   steps   sweeps per epoch, one sweep = n proposals (0: 64/48/32 by n)
-//   chains  replicas, multiple of 8, in ladders of 8 temperatures
-//           (0: enough to give every compute unit two; PACK_AUTOTUNE raises it)
+//   epochs  epochs of the search (0: 50/40/30 by n; >120 capped to 80); an
+//           epoch is one exchange step per min(sweeps per epoch, 10) sweeps
+//   steps   single-square proposals per chain per epoch, as the reference
+//           counts them (0: 64/48/32 by n): ceil(steps / n) sweeps of n
+//           proposals; with PACK_SWEEP_STEPS=1 (seeded: SEED_SWEEP_STEPS=1)
+//           steps counts sweeps instead
+//   chains  replicas, multiple of 8, in ladders of 8 temperatures (0: one
+//           wave of the device, compute units x resident sweep workgroups x
+//           replicas per workgroup, by the occupancy query; PACK_AUTOTUNE
+//           raises it)
+// The search follows the reference's improvement path (best certified side,
+// per-chain targets best - shrink * mults[b % 8], ladder heads restarted from
+// the best configuration) and its 200-epoch reheating cycle.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -80,6 +89,7 @@ int main(int argc, char** argv) {
     const int structured = (int)envf("SEED_STRUCTURED", 1, 0, 1);
     const int autotune = (int)envf("SEED_AUTOTUNE", 1, 0, 1);
     const int max_chains = (int)envf("SEED_MAX_CHAINS", 768, 8, 4096);
+    const int sweep_steps = (int)envf("SEED_SWEEP_STEPS", 0, 0, 1);
     const double tmin = 1e-11, rot = .12;  // cuda/seeded.cu:494: ladder from 1e-11 to tmax
 #else
     const double tmin = envf("PACK_TMIN", 1e-11, 1e-12, 1.0);
@@ -90,6 +100,7 @@ int main(int argc, char** argv) {
     const double shrink = envf("PACK_SHRINK", .001, .000001, .1);
     const int autotune = (int)envf("PACK_AUTOTUNE", 1, 0, 1);
     const int max_chains = (int)envf("PACK_MAX_CHAINS", 768, 8, 4096);
+    const int sweep_steps = (int)envf("PACK_SWEEP_STEPS", 0, 0, 1);
     const int collective = 1, polish = 200, structured = 1;
 #endif
     const unsigned long long rng_seed = std::strtoull(argv[5], nullptr, 10);
@@ -101,17 +112,26 @@ int main(int argc, char** argv) {
     int cus = 0;
     char name[256] = {0};
     sis_device_info(&cus, name, sizeof(name));
-    const int wave_chains = std::max(8, ((2 * cus) / 8) * 8);
+    if (epochs == 0) epochs = n <= 64 ? 50 : n <= 256 ? 40 : 30;
+    if (steps == 0) steps = n <= 64 ? 64 : n <= 256 ? 48 : 32;
+    const int eff_epochs = epochs > 120 ? std::min(epochs, 80) : epochs;
+    const int epoch_sweeps = sweep_steps ? std::max(1, steps) : std::max(1, (steps + n - 1) / n);
+
+    sis_params p;
+    sis_default_params(&p);
+    sis_instance inst{n, L, seed.data()};
+    // occupancy of the sweep kernel for this instance (cuda/packing.cu:788-795)
+    p.replicas = max_chains;
+    int blocks_per_cu = 1, per_block = 1;
+    if (sis_occupancy(&inst, &p, &blocks_per_cu, &per_block) != SIS_OK) {
+        std::cerr << sis_last_error() << "\n";
+        return 2;
+    }
+    const int wave_chains = std::max(8, ((cus * std::max(1, blocks_per_cu) * per_block) / 8) * 8);
     if (chains == 0 || (autotune && chains >= 64)) {
         const int target = chains == 0 ? wave_chains : std::max(chains, std::min(wave_chains, max_chains));
         chains = ((std::min(target, max_chains) + 7) / 8) * 8;
     }
-    if (epochs == 0) epochs = n <= 64 ? 50 : n <= 256 ? 40 : 30;
-    if (steps == 0) steps = n <= 64 ? 64 : n <= 256 ? 48 : 32;
-    const int eff_epochs = epochs > 120 ? std::min(epochs, 80) : epochs;
-
-    sis_params p;
-    sis_default_params(&p);
     p.tmin = tmin;
     p.tmax = tmax;
     p.move_sigma = move;
@@ -120,19 +140,24 @@ int main(int argc, char** argv) {
     p.shrink = shrink;
     p.replicas = chains;
     p.ladder = 8;
-    p.sweeps = (int64_t)eff_epochs * steps;
-    p.sweeps_per_exchange = std::max(1, std::min(steps, 10));
+    p.sweeps = (int64_t)eff_epochs * epoch_sweeps;
+    p.sweeps_per_exchange = std::max(1, std::min(epoch_sweeps, 10));
+    const int epoch_intervals = (epoch_sweeps + p.sweeps_per_exchange - 1) / p.sweeps_per_exchange;
     // the reference's epoch mixes collective moves into its steps
     // (cuda/packing.cu:564-616) and, when it polishes, kicks once per epoch
     // (:696-745)
     p.collective_moves = collective ? 4 : 0;
-    p.kick_every = polish > 0 ? std::max(1, steps / p.sweeps_per_exchange) : 0;
+    p.kick_every = polish > 0 ? epoch_intervals : 0;
+    // improvement path (cuda/packing.cu:503-504, 831-851) and the reheating
+    // cycle of 200 epochs (:516)
+    p.improve = 1;
+    p.cool_period = 200 * epoch_intervals;
+    p.certify_every = 1;  // every interval: at least once per epoch (cuda/packing.cu:506, 689)
     p.polish_iters = polish;
     p.rng_seed = rng_seed;
     // portfolio starts + structured jitter (cuda/packing.cu:806,821), or the
     // seed in every chain
     p.init_mode = structured ? 2 : 0;
-    sis_instance inst{n, L, seed.data()};
     std::vector<double> q(3 * (size_t)n);
     sis_result r{};
     r.q_out = q.data();
@@ -154,7 +179,7 @@ int main(int argc, char** argv) {
         << ",\"shrinks\":" << (improved ? r.shrinks : 0) << ",\"epochs\":" << eff_epochs << ",\"steps\":" << steps
         << ",\"chains\":" << chains << ",\"device\":\"";
     json_escape(out, name);
-    out << "\",\"sm_count\":" << cus << ",\"blocks_per_sm\":" << 2 << ",\"wave_chains\":" << wave_chains
+    out << "\",\"sm_count\":" << cus << ",\"blocks_per_sm\":" << blocks_per_cu << ",\"wave_chains\":" << wave_chains
         << ",\"collective\":" << collective << ",\"polish\":" << polish << ",\"structured\":" << structured
         << ",\"proposals\":" << r.proposals + r.collective_tried
         << ",\"accepted\":" << r.accepted + r.collective_accepted
@@ -163,9 +188,9 @@ int main(int argc, char** argv) {
 #else
     out << "{\"n\":" << n << ",\"target_L\":" << bestL << ",\"seconds\":" << r.seconds
         << ",\"chain_steps\":" << (long long)chains * eff_epochs * steps << ",\"proposals\":" << r.proposals
-        << ",\"feasible_checkpoints\":" << std::max(1, r.feasible ? r.shrinks + 1 : 0)
+        << ",\"feasible_checkpoints\":" << std::max<unsigned long long>(1, r.feasible_checkpoints)
         << ",\"shrinks\":" << (improved ? r.shrinks : 0) << ",\"chains\":" << chains
-        << ",\"sm_count\":" << cus << ",\"blocks_per_sm\":" << 2 << ",\"wave_chains\":" << wave_chains
+        << ",\"sm_count\":" << cus << ",\"blocks_per_sm\":" << blocks_per_cu << ",\"wave_chains\":" << wave_chains
         << ",\"device\":\"";
     json_escape(out, name);
     out << "\",\"q\":[";

@@ -140,7 +140,7 @@ int plan_instance(const InstSpec& s, int batch_replicas, Plan& pl);
 void instance_records(const InstSpec& s, const Plan& pl, InstDesc& D, std::vector<RepState>& reps,
                       std::vector<SlotState>& slots, std::vector<int>& ros);
 double slot_box(const InstSpec& s, int ladder, int slot, double Lb);
-void finalize_source(const InstDesc& D, double& box, double& target_L);
+bool finalize_source(const InstDesc& D, double& box, double& target_L);
 // A launch set of this many replicas keeps every compute unit busy (eight per
 // compute unit): below it tiles hold one replica each and stage every size.
 bool device_loaded(long long replicas);

@@ -54,8 +54,10 @@ struct InstDesc {
     int kicks;                  // kicks applied (counted once per replica)
     int improve;                // 0: shrink on a zero-overlap replica; 1: certified-side improvement path
     int cool_period;            // reheating cycle in intervals (0: none)
-    int pad1;
+    int certify_every;          // improve = 1: launches between certified-side passes
     double L_box_arch;          // box side the archived configuration was in
+    int seeded;                 // replica 0 started from the caller's configuration (the initial archive)
+    float E_arch;               // FP32 energy of the archived configuration when it was archived
     double Lmin;                // sqrt(n): no box goes below it
     unsigned long long proposals, accepted, cand, near, ex_tried, ex_acc;
     unsigned long long coll_tried, coll_acc;
@@ -182,7 +184,8 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G, bool wide
     b = (b + 15) & ~(size_t)15;
     b += (size_t)G * 8;                         // per-replica occupied-group mask
     b += (size_t)G * sizeof(CollWork);          // collective-move / kick scratch
-    b += (size_t)G * 10 * 4;                    // shift x/y (this and next sweep), steps, 1/T, box, accepts, energy
+    b += (size_t)G * 11 * 4;                    // shift x/y (this and next sweep), steps, 1/T, box, accepts, energy,
+                                                // largest overlap
     b += 2 * (kGroups + 2) * 4;                 // level counts / cursors, level starts
     return b;
 }

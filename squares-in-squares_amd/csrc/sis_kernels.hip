@@ -256,6 +256,7 @@ struct TileLds {
     float* ik;
     float* Lr;        // box side of each replica (its temperature slot's)
     float* eout;      // canonical energy per replica at the end of the launch
+    unsigned* maxd;   // per replica: the largest pair depth^2 of the last energy pass (float bits)
     CollWork* cw;     // per replica: collective-move / kick scratch
     int* acc;
     int* gcnt;        // kGroups + 2: level counts, then scatter cursors
@@ -321,7 +322,8 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool w
     t.Lr = f + 7 * G;
     t.eout = f + 8 * G;
     t.acc = reinterpret_cast<int*>(f + 9 * G);
-    t.gcnt = t.acc + G;
+    t.maxd = reinterpret_cast<unsigned*>(f + 10 * G);
+    t.gcnt = reinterpret_cast<int*>(t.maxd + G);
     t.gstart = t.gcnt + (kGroups + 2);
     return t;
 }
@@ -720,11 +722,17 @@ __device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int 
 // Canonical FP32 energy of every tile replica: per-square wall + pairs with
 // larger index inside the 5x5 stencil, then a fixed pairwise tree over the
 // square index, so the bits do not depend on the tile shape.
+// With kMaxDepth (uniform), also the largest pair depth^2 of every replica
+// (t.maxd).
 template <int NT>
 __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float inv_w,
-                            float* out /* G floats, LDS or global */) {
+                            float* out /* G floats, LDS or global */, bool kMaxDepth = false) {
     const int tid = threadIdx.x;
-    for (int g = tid; g < G; g += NT) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
+    for (int g = tid; g < G; g += NT) {
+        t.shx[g] = 0.0f;
+        t.shy[g] = 0.0f;
+        if (kMaxDepth) t.maxd[g] = 0u;
+    }
     tile_sync<NT>();
     bin_tile<NT>(t, n, G, M, m, inv_w);
     float* e = reinterpret_cast<float*>(t.rank);   // rank + items: 4 bytes per square
@@ -737,15 +745,19 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const Stencil sc(t.A + g * gw, M, kx, ky);
         const float4* pg = t.pos + g * n;
         float ei = wall32(p.x, p.y, p.z, p.w, t.Lr[g]);
+        float md = 0.0f;
         for (int f = 0; f < sc.total; ++f) {
             const int k = sc.at(f);
             const int j = t.sorted[k];
             // with the cell-ordered copy the position does not wait for the index
             const float4 v = t.spos ? t.spos[k] : pg[j];
             if (j <= i) continue;
-            ei += depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
+            const float d2 = depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
+            ei += d2;
+            if (kMaxDepth) md = fmaxf(md, d2);
         }
         e[q] = ei;
+        if (kMaxDepth && md > 0.0f) atomicMax(&t.maxd[g], __float_as_uint(md));
     }
     tile_sync<NT>();
     if (NT == 64 && G * n <= 64) {
@@ -998,8 +1010,34 @@ __device__ __forceinline__ double cert_need(double dx, double dy, double ci, dou
     const double vmax = fmax(fmax(v1, v2), fmax(v3, v4));
     return vmax > 1e-12 ? (r + kCertTau) / vmax : 1e9;
 }
-// rs: the tile's first replica record.  Writes Lc (1e9 when the dilation
-// exceeds 1.008), lam, minx, miny per replica; certs counts the certified.
+// Bounds that keep the FP64 work to the pairs that can matter, both exact:
+//  - a replica with an FP32 pair depth above kCertSkipDepth2 (depth 0.00975)
+//    needs a dilation above 1 + 0.00975 / R >= 1 + 0.00975 / 1.2072 > 1.008
+//    (an FP32 depth is within 1e-6 of the FP64 one), so it has no certified
+//    side;
+//  - a pair whose FP32 gap is below -1e-5 is separated by more than
+//    1e-5 - 1e-6 on its best axis, so its dilation (R + tau) / vmax is below 1.
+constexpr float kCertSkipDepth2 = 9.5e-5f;
+constexpr float kCertPairGap = -1e-5f;
+
+// Order-preserving keys of a running min / max, flushed to LDS atomics when
+// the thread's next square belongs to another replica.
+struct SpanAcc {
+    int g = -1;
+    double lox = 1e300, hix = -1e300, loy = 1e300, hiy = -1e300;
+    __device__ __forceinline__ void flush(unsigned long long* acc, int stride) {
+        if (g < 0) return;
+        unsigned long long* a = acc + stride * g;
+        atomicMin(a + 1, dkey(lox));
+        atomicMax(a + 2, dkey(hix));
+        atomicMin(a + 3, dkey(loy));
+        atomicMax(a + 4, dkey(hiy));
+    }
+};
+
+// rs: the tile's first replica record.  Needs the cell lists (shift 0) and
+// t.maxd of the final positions.  Writes Lc (1e9 when the dilation exceeds
+// 1.008), and for certified replicas lam, minx, miny; certs counts them.
 template <int NT>
 __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs, unsigned& certs) {
     static_assert(sizeof(CollWork) >= 5 * 8, "the per-replica scratch holds five 64-bit keys");
@@ -1019,21 +1057,27 @@ __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs
     tile_sync<NT>();
     for (int q = tid; q < G * n; q += NT) {
         const int g = rep_of(t, q), i = q - g * n;
+        if (__uint_as_float(t.maxd[g]) > kCertSkipDepth2) continue;
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
         const Stencil sc(t.A + g * gw, M, ck & 255, ck >> 8);
         const float4* pg = t.pos + (size_t)g * n;
-        const double xi = p.x, yi = p.y;
-        double ci, si;
-        unit64(p.z, p.w, ci, si);
-        double lam = 1.0;
+        double ci = 0.0, si = 0.0, lam = 1.0;
+        bool have = false;
         for (int f = 0; f < sc.total; ++f) {
             const int k = sc.at(f);
             const int j = t.sorted[k];
             if (j <= i) continue;
             const float4 v = t.spos ? t.spos[k] : pg[j];
-            const double dx = xi - (double)v.x, dy = yi - (double)v.y;
+            const float fx = v.x - p.x, fy = v.y - p.y;
+            if (!(__builtin_fmaf(fx, fx, fy * fy) < 2.06f)) continue;
+            if (!(sat_gap(fx, fy, p.z, p.w, v.z, v.w) > kCertPairGap)) continue;
+            const double dx = (double)p.x - (double)v.x, dy = (double)p.y - (double)v.y;
             if (dx * dx + dy * dy >= kCertNear2) continue;
+            if (!have) {
+                unit64(p.z, p.w, ci, si);
+                have = true;
+            }
             double cj, sj;
             unit64(v.z, v.w, cj, sj);
             lam = fmax(lam, cert_need(dx, dy, ci, si, cj, sj));
@@ -1041,32 +1085,40 @@ __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs
         if (lam > 1.0) atomicMax(acc + kStride * g, dkey(lam));
     }
     tile_sync<NT>();
+    SpanAcc sp;
     for (int q = tid; q < G * n; q += NT) {
         const int g = rep_of(t, q);
-        unsigned long long* a = acc + kStride * g;
-        const double lam = dval(a[0]);
+        if (__uint_as_float(t.maxd[g]) > kCertSkipDepth2) continue;
+        const double lam = dval(acc[kStride * g]);
+        if (lam > kCertMaxLam) continue;
+        if (g != sp.g) {
+            sp.flush(acc, kStride);
+            sp = SpanAcc{};
+            sp.g = g;
+        }
         const float4 p = t.pos[q];
         double ci, si;
         unit64(p.z, p.w, ci, si);
         const double r = 0.5 * (fabs(ci) + fabs(si));
         const double x = (double)p.x * lam, y = (double)p.y * lam;
-        atomicMin(a + 1, dkey(x - r));
-        atomicMax(a + 2, dkey(x + r));
-        atomicMin(a + 3, dkey(y - r));
-        atomicMax(a + 4, dkey(y + r));
+        sp.lox = fmin(sp.lox, x - r);
+        sp.hix = fmax(sp.hix, x + r);
+        sp.loy = fmin(sp.loy, y - r);
+        sp.hiy = fmax(sp.hiy, y + r);
     }
+    sp.flush(acc, kStride);
     tile_sync<NT>();
     for (int g = tid; g < G; g += NT) {
         const unsigned long long* a = acc + kStride * g;
-        const double lam = dval(a[0]), minx = dval(a[1]), maxx = dval(a[2]), miny = dval(a[3]),
-                     maxy = dval(a[4]);
-        const double Lc = lam > kCertMaxLam ? kCertNone : fmax(maxx - minx, maxy - miny);
+        const bool skip = __uint_as_float(t.maxd[g]) > kCertSkipDepth2;
+        const double lam = skip ? kCertNone : dval(a[0]);
+        const bool ok = lam <= kCertMaxLam;
         RepState* r = rs + g;
-        r->Lc = Lc;
+        r->Lc = ok ? fmax(dval(a[2]) - dval(a[1]), dval(a[4]) - dval(a[3])) : kCertNone;
         r->lam = lam;
-        r->minx = minx;
-        r->miny = miny;
-        certs += Lc < kCertNone ? 1u : 0u;
+        r->minx = ok ? dval(a[1]) : 0.0;
+        r->miny = ok ? dval(a[3]) : 0.0;
+        certs += ok ? 1u : 0u;
     }
     tile_sync<NT>();
 }
@@ -1225,6 +1277,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     TileLds t = carve(smem, n, M, G, tl.wide != 0);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
+    // the state and the replica records load together; the clamp into each
+    // replica's box follows in LDS
+    for (int q = tid; q < G * n; q += NT) t.pos[q] = gstate[q];
     for (int g = tid; g < G; g += NT) {
         const int slot = reps[D->rep_off + tl.r0 + g].slot;
         const float4 ss = *reinterpret_cast<const float4*>(slots + D->rep_off + slot);
@@ -1241,11 +1296,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         t.shy[g] = w * u01(o.y);
     }
     tile_sync<NT>();
-    // stage the state: centres rescaled (after a box shrink) and clamped into
-    // the replica's box
+    // staged state: centres rescaled (after a box shrink) and clamped into the
+    // replica's box
     for (int q = tid; q < G * n; q += NT) {
         const float L = t.Lr[rep_of(t, q)];
-        float4 v = gstate[q];
+        float4 v = t.pos[q];
         v.x = fminf(fmaxf(v.x * scale, 0.0f), L);
         v.y = fminf(fmaxf(v.y * scale, 0.0f), L);
         t.pos[q] = v;
@@ -1377,7 +1432,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     }
 
-    tile_energy<NT>(t, n, G, M, m, inv_w, t.eout);
+    const bool improve = D->improve != 0;
+    // improve = 1: certified sides every certify_every launches and at the
+    // last one; the other launches mark them absent
+    const bool certify_now =
+        improve && ((interval + 1u) % (uint32_t)max(1, D->certify_every) == 0u || (long long)nsw >= left);
+    tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, certify_now);
     unsigned ctried = 0, caccd = 0, kicked = 0;
     for (int c = 0; c < ncoll; ++c)
         collective_step<NT>(t, n, G, M, m, inv_w, tl.r0, sweep0 + (uint32_t)nsw, (uint32_t)c, k0, k1, gstate, ctried,
@@ -1387,15 +1447,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         kick<NT>(t, n, G, M, m, inv_w, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u, k0, k1,
                  kicked);
     unsigned certs = 0;
-    if (D->improve) {
-        // the cell lists must hold the final positions: the last binning is
-        // the energy's (after the sweeps, or the kick's), unless rejected
-        // collective moves restored positions after it
-        if (ncoll > 0 && !kick_now) {
-            for (int g = tid; g < G; g += NT) { t.shx[g] = 0.0f; t.shy[g] = 0.0f; }
-            tile_sync<NT>();
-            bin_tile<NT>(t, n, G, M, m, inv_w);
-        }
+    if (certify_now) {
+        // the cell lists and largest overlaps must be those of the final
+        // positions: collective moves and kicks change them after the first
+        // energy pass (the canonical energy it writes again is the same)
+        if (ncoll > 0 || kick_now) tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, true);
         tile_certify<NT>(t, n, G, M, reps + D->rep_off + tl.r0, certs);
     }
     for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
@@ -1404,6 +1460,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         RepState* rs = reps + D->rep_off + tl.r0 + g;
         rs->E = t.eout[g];
         rs->acc = (unsigned)t.acc[g];
+        if (improve && !certify_now) rs->Lc = kCertNone;
         accsum += (unsigned)t.acc[g];
     }
 #pragma unroll
@@ -1724,6 +1781,9 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     __syncthreads();
 
     // 1. step adaptation per temperature slot, from the replica that held it
+    //    (and the smallest certified side: per thread, per wavefront, then one
+    //    LDS atomic per wavefront)
+    unsigned long long my_lc = ~0ull;
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
         const float rate = (float)rs.acc / (float)((long long)n * ran);
@@ -1736,7 +1796,12 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         eS[rs.slot] = rs.E;
         kS[rs.slot] = ss.inv_t_log2e;
         rS[rs.slot] = r;
-        if (improve) atomicMin(&best_lc, (unsigned long long)__double_as_longlong(rs.Lc));
+        if (improve) my_lc = min(my_lc, (unsigned long long)__double_as_longlong(rs.Lc));
+    }
+    if (improve) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) my_lc = min(my_lc, (unsigned long long)__shfl_xor(my_lc, off, 64));
+        if ((tid & 63) == 0) atomicMin(&best_lc, my_lc);
     }
     __syncthreads();
 
@@ -1754,6 +1819,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             const int rb = best_r;
             const RepState rs = reps[base + rb];
             const double Lb = rs.Lc, lam = rs.lam, mx = rs.minx, my = rs.miny;
+            const float E_b = rs.E;
             const float box_b = slots[base + rs.slot].L;
             const float4* src = state + D->state_off + (size_t)rb * n;
             for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
@@ -1781,6 +1847,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                 D->Lf = (float)Lb;
                 D->L_arch = Lb;
                 D->L_box_arch = (double)box_b;
+                D->E_arch = E_b;
                 D->arch_valid = 1;
                 D->feasible += 1;
                 D->shrinks += 1;
@@ -1872,6 +1939,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             D->arch_valid = 1;
             D->L_arch = L0;
             D->L_box_arch = L0;
+            D->E_arch = 0.0f;
             D->feasible += 1;
         }
         if (Ln < L0) {

@@ -149,10 +149,9 @@ int solve_noalloc(const sis_instance* inst, const sis_params* params, sis_result
     job.n = n;
     job.iters = spec.p.polish_iters;
     job.out_off = 0;
-    finalize_source(D, job.L, rec.target_L);
-    if (D.arch_valid) {
+    if (finalize_source(D, job.L, rec.target_L)) {
         job.src32 = d_arch;
-        rec.e32 = 0.0;
+        rec.e32 = D.E_arch;
     } else {
         int best = 0;
         for (int r = 1; r < R; ++r)

@@ -117,6 +117,7 @@ int sis_dev_launch(int n, int R, float Lgrid, const float* box, float cell, int 
     D.sweeps_total = (long long)sweep0 + sweeps; D.sweeps_done = sweep0;
     D.L = Lgrid; D.L_arch = Lgrid; D.Lf = Lgrid; D.scale = 1.0f; D.w = cell; D.key0 = key0; D.key1 = key1;
     D.collective = collective; D.kick_every = kick_every; D.interval = interval; D.improve = certify;
+    D.certify_every = 1;
     D.Lmin = std::sqrt((double)n);
     std::vector<Tile> tiles;
     for (int r0 = 0; r0 < R; r0 += G) tiles.push_back(Tile{0, r0, std::min(G, R - r0), wide ? 1 : 0});

@@ -94,6 +94,7 @@ void default_params(sis_params* p) {
     p->rng_seed = 7;
     p->improve = 0;
     p->cool_period = 0;
+    p->certify_every = 1;
 }
 
 int validate(const sis_instance* in, const sis_params* p) {
@@ -127,6 +128,7 @@ int validate(const sis_instance* in, const sis_params* p) {
     if (p->improve < 0 || p->improve > 1) return fail(SIS_EINVAL, "improve must be 0 or 1");
     if (p->cool_period < 0 || p->cool_period == 1 || p->cool_period > 1000000)
         return fail(SIS_EINVAL, "cool_period must be 0 or 2..1e6");
+    if (p->certify_every < 1 || p->certify_every > 1000000) return fail(SIS_EINVAL, "certify_every must be 1..1e6");
     const float w = (float)p->cell;
     const int M = grid_cells((float)in->L, w) + 4;
     if (M > 255) return fail(SIS_EINVAL, "cell grid wider than 255 cells: raise the cell width");
@@ -251,7 +253,10 @@ void instance_records(const InstSpec& s, const Plan& pl, InstDesc& D, std::vecto
     D.w = (float)s.p.cell; D.shrink = (float)s.p.shrink; D.target_acc = (float)s.p.target_accept;
     D.key0 = (uint32_t)s.p.rng_seed; D.key1 = (uint32_t)(s.p.rng_seed >> 32);
     D.collective = s.p.collective_moves; D.kick_every = s.p.kick_every;
-    D.improve = s.p.improve; D.cool_period = s.p.cool_period;
+    D.improve = s.p.improve; D.cool_period = s.p.cool_period; D.certify_every = s.p.certify_every;
+    // replica 0 starts from the seed exactly (init_mode 0 without jitter, or
+    // the portfolio's unjittered class 0), and the archive starts as replica 0
+    D.seeded = !s.q.empty() && ((s.p.init_mode == 0 && s.p.init_jitter == 0.0) || s.p.init_mode == 2);
     reps.resize(R);
     slots.resize(R);
     ros.resize(R);
@@ -274,16 +279,20 @@ void instance_records(const InstSpec& s, const Plan& pl, InstDesc& D, std::vecto
 }
 
 // The FP64 finalisation of an instance: the archive when there is one
-// (polished in the box it was found in), else the best replica by FP32
-// energy in the instance's box.
-void finalize_source(const InstDesc& D, double& box, double& target_L) {
+// (polished in the box it was found in); under the improvement path without
+// an improvement, the seed itself, which the archive holds from the start
+// (the reference keeps the seed and its side then, cuda/packing.cu:820,
+// 859-865); else the best replica by FP32 energy in the instance's box.
+// Returns whether the archive is the source.
+bool finalize_source(const InstDesc& D, double& box, double& target_L) {
     if (D.arch_valid) {
         box = D.improve ? D.L_box_arch : D.L_arch;
         target_L = D.L_arch;
-    } else {
-        box = D.L;
-        target_L = D.L;
+        return true;
     }
+    box = D.L;
+    target_L = D.L;
+    return D.improve && D.seeded;
 }
 
 // ---------------------------------------------------------------------------
@@ -675,10 +684,9 @@ int Batch::finalize(std::vector<FinalRecord>& rec) {
         int best = 0;
         for (int r = 1; r < D.R; ++r)
             if (h_reps[D.rep_off + r].E < h_reps[D.rep_off + best].E) best = r;
-        finalize_source(D, j.L, rec[k].target_L);
-        if (D.arch_valid) {
+        if (finalize_source(D, j.L, rec[k].target_L)) {
             j.src32 = d_arch + D.arch_off;
-            rec[k].e32 = 0.0;
+            rec[k].e32 = D.E_arch;
         } else {
             j.src32 = d_state + D.state_off + (size_t)best * D.n;
             rec[k].e32 = h_reps[D.rep_off + best].E;
@@ -789,6 +797,22 @@ int sis_portfolio(const sis_instance* inst, const sis_params* params, int jitter
     std::vector<double> q;
     portfolio_state(make_spec(inst, params), jitter != 0, q);
     std::copy(q.begin(), q.end(), out);
+    return SIS_OK;
+}
+
+int sis_occupancy(const sis_instance* inst, const sis_params* params, int* blocks_per_cu,
+                  int* replicas_per_block) {
+    int rc = validate(inst, params);
+    if (rc) return rc;
+    if ((rc = ensure_device(-1))) return rc;
+    const InstSpec s = make_spec(inst, params);
+    Plan pl;
+    if ((rc = plan_instance(s, s.p.replicas, pl))) return rc;
+    const int threads = sweep_threads(s.n);
+    int nb = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_kernel_ptr(threads), threads, pl.lds));
+    if (blocks_per_cu) *blocks_per_cu = nb;
+    if (replicas_per_block) *replicas_per_block = pl.G;
     return SIS_OK;
 }
 

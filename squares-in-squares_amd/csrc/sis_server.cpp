@@ -400,8 +400,7 @@ int Server::finalize_start(std::vector<Req*>& done) {
             job.iters = r->spec.p.polish_iters;
             job.out_off = 3ll * kMaxN * j;
             double target_L;
-            finalize_source(D, job.L, target_L);
-            if (D.arch_valid) {
+            if (finalize_source(D, job.L, target_L)) {
                 job.src32 = d_arch + D.arch_off;
             } else {
                 // the best replica, chosen on the device
@@ -434,7 +433,7 @@ int Server::finalize_done(FinBlock& fb) {
         finalize_source(D, box, rec.target_L);
         rec.L_cert = fo.L_cert;
         rec.lambda = fo.lambda;
-        rec.e32 = D.arch_valid ? 0.0 : fo.e32;
+        rec.e32 = (D.arch_valid || (D.improve && D.seeded)) ? (double)D.E_arch : fo.e32;
         rec.e64_before = fo.e64_before;
         rec.e64_after = fo.e64_after;
         rec.desc = D;

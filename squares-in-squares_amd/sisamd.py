@@ -48,6 +48,7 @@ class SisParams(ctypes.Structure):
         ("polish_iters", ctypes.c_int), ("init_mode", ctypes.c_int), ("flags", ctypes.c_int),
         ("rng_seed", ctypes.c_uint64), ("collective_moves", ctypes.c_int), ("kick_every", ctypes.c_int),
         ("sweeps_per_shift", ctypes.c_int), ("improve", ctypes.c_int), ("cool_period", ctypes.c_int),
+        ("certify_every", ctypes.c_int),
     ]
 
 
@@ -445,6 +446,18 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def certify_strings(rows) -> str:
+    """Exact certificate of decimal-string rows [(x, y, angle)]: the 50-decimal
+    L, or SisError (code 2) with the rejection reason."""
+    n = len(rows)
+    cols = [(ctypes.c_char_p * n)(*[str(r[k]).encode() for r in rows]) for k in range(3)]
+    buf = ctypes.create_string_buffer(64)
+    rc = lib().sis_certify_strings(n, cols[0], cols[1], cols[2], buf, 64)
+    if rc:
+        _err(rc)
+    return buf.value.decode()
 
 
 def export_certify(q: np.ndarray):

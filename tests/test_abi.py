@@ -58,6 +58,7 @@ def test_default_params(sis):
     assert p.tmin == 1e-5 and p.tmax == 5e-2
     assert p.move_sigma == 0.05 and p.rot_sigma == 0.05 and p.target_accept == 0.3
     assert p.shrink == 1e-4 and p.sweeps_per_exchange == 10 and p.polish_iters == 200
+    assert p.improve == 0 and p.cool_period == 0 and p.certify_every == 1
     assert abs(p.cell - 0.7075) < 1e-12
 
 
@@ -131,7 +132,8 @@ def test_cli_seed_file_validation(tmp_path, text):
 
 @pytest.mark.parametrize("name,value", [("PACK_TMIN", "0"), ("PACK_TMAX", "2"), ("PACK_MOVE", "0.5"),
                                         ("PACK_ROTATION", "abc"), ("PACK_SHRINK", "1"),
-                                        ("PACK_AUTOTUNE", "3"), ("PACK_MAX_CHAINS", "4")])
+                                        ("PACK_AUTOTUNE", "3"), ("PACK_MAX_CHAINS", "4"),
+                                        ("PACK_SWEEP_STEPS", "2")])
 def test_cli_env_knobs_range_checked(tmp_path, name, value):
     seed = tmp_path / "seed.txt"
     seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")
@@ -167,7 +169,7 @@ def test_seeded_usage_exit_2():
 @pytest.mark.parametrize("name,value", [("SEED_SHRINK", "0"), ("SEED_TMAX", "0.5"), ("SEED_MOVE", "1"),
                                         ("SEED_COLLECTIVE", "2"), ("SEED_POLISH", "5000"),
                                         ("SEED_STRUCTURED", "-1"), ("SEED_AUTOTUNE", "x"),
-                                        ("SEED_MAX_CHAINS", "7")])
+                                        ("SEED_MAX_CHAINS", "7"), ("SEED_SWEEP_STEPS", "-1")])
 def test_seeded_env_knobs_range_checked(tmp_path, name, value):
     seed = tmp_path / "seed.txt"
     seed.write_text("2 3\n0.5 0.5 0\n1.5 0.5 0\n")

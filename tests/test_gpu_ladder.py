@@ -124,7 +124,8 @@ def test_ladder_kernel_matches_model(sis, oracle, n, R, lad, improve, cool, L):
 def test_launch_certified_side_matches_oracle(sis, oracle, n, L, R, group):
     """Per-replica boxes below the grid side: the sweeps equal the model run in
     each replica's own box, and the FP64 certified side of every replica at the
-    end of the launch equals the oracle's all-pairs one (cuda/packing.cu:254-297)."""
+    end of the launch equals the oracle's all-pairs one (cuda/packing.cu:254-297);
+    its dilation and corner are defined where it certifies."""
     rng = np.random.default_rng(n)
     k = int(math.isqrt(n - 1)) + 1
     pitch = (L - 0.1) / k
@@ -163,9 +164,9 @@ def test_launch_certified_side_matches_oracle(sis, oracle, n, L, R, group):
         assert np.array_equal(dst[r], m), r
         assert den[r] == oracle.energy32(x, y, c, s, float(box[r]), 0.7075)
         side, lam, mx, my = oracle.certify_xycs(dst[r])
-        assert dreps["lam"][r] == pytest.approx(lam, rel=1e-14, abs=0)
         if side < 1e8:
             certified += 1
+            assert dreps["lam"][r] == pytest.approx(lam, rel=1e-14, abs=0)
             assert dreps["Lc"][r] == pytest.approx(side, rel=1e-14)
             assert dreps["minx"][r] == pytest.approx(mx, rel=1e-13, abs=1e-13)
             assert dreps["miny"][r] == pytest.approx(my, rel=1e-13, abs=1e-13)
@@ -174,14 +175,17 @@ def test_launch_certified_side_matches_oracle(sis, oracle, n, L, R, group):
     assert certified >= 1 and (R == 1 or dreps["Lc"][R - 1] == 1e9)
 
 
-def _model_interval(oracle, s, w, spsh):
+def _model_interval(oracle, s, w, spsh, certify_every):
     """One interval of the oracle: staging (scale, clamp to the slot box),
-    sweeps, canonical energy, certified side (improve = 1), exchange step."""
+    sweeps, canonical energy, certified side (improve = 1, every
+    certify_every-th launch and the last; absent, 1e9, otherwise), exchange
+    step."""
     d = s["desc"]
     R, n = d["R"], d["n"]
     left = d["sweeps_total"] - d["sweeps_done"]
     nsw = min(d["spe"], left)
     sweep0 = d["sweeps_done"]
+    certify = d["improve"] and ((d["interval"] + 1) % certify_every == 0 or nsw >= left)
     st, reps, slots = s["state"].copy(), s["reps"].copy(), s["slots"]
     scale = np.float32(d["scale"])
     for r in range(R):
@@ -198,9 +202,11 @@ def _model_interval(oracle, s, w, spsh):
         st[r] = np.stack([x, y, c, sn], -1)
         reps["E"][r] = oracle.energy32(x, y, c, sn, float(Lr), w)
         reps["acc"][r] = acc
-        if d["improve"]:
+        if certify:
             side, lam, mx, my = oracle.certify_xycs(st[r])
             reps["Lc"][r], reps["lam"][r], reps["minx"][r], reps["miny"][r] = side, lam, mx, my
+        elif d["improve"]:
+            reps["Lc"][r] = 1e9
     return st, reps
 
 
@@ -215,7 +221,8 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
     if case == "improve":
         n, L, R = 24, 7.0, 16
         p = sis.default_params(replicas=R, ladder=8, sweeps=5 * 40, sweeps_per_exchange=5, tmin=1e-4, tmax=1e-2,
-                               shrink=2e-3, improve=1, cool_period=20, init_mode=1, rng_seed=31, polish_iters=0)
+                               shrink=2e-3, improve=1, cool_period=20, certify_every=3, init_mode=1, rng_seed=31,
+                               polish_iters=0)
     else:
         n, L, R = 16, 6.0, 12
         p = sis.default_params(replicas=R, ladder=12, sweeps=4 * 40, sweeps_per_exchange=4, tmin=1e-5, tmax=1e-3,
@@ -228,7 +235,7 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
         say("trajectory", case, iv)
         b.run(1, 0)
         dev = b.state(0)
-        st, reps = _model_interval(oracle, s, w, spsh)
+        st, reps = _model_interval(oracle, s, w, spsh, p.certify_every)
         # energies and accept counts come from the sweeps; the states are
         # compared after the exchange step, which may restart replicas
         assert np.array_equal(dev["reps"]["E"], reps["E"]) and np.array_equal(dev["reps"]["acc"], reps["acc"]), iv
@@ -236,7 +243,7 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
             fin = reps["Lc"] < 1e8
             assert np.array_equal(dev["reps"]["Lc"] < 1e8, fin), iv
             for f in ("Lc", "lam", "minx", "miny"):
-                a_, b_ = dev["reps"][f][fin], reps[f][fin]
+                a_, b_ = dev["reps"][f][fin], reps[f][fin]  # defined where certified
                 assert np.all(np.abs(a_ - b_) <= 1e-14 * np.maximum(1, np.abs(b_))), (iv, f)
                 reps[f] = dev["reps"][f]
         d, r2, sl2, ros2, st2, ar2, imp = oracle.ladder(s["desc"], reps, s["slots"], s["ros"], st, s["arch"])
@@ -254,3 +261,23 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
     else:
         assert s["desc"]["shrinks"] >= 3 and s["desc"]["L"] < L
     b.close()
+
+
+def test_seed_kept_when_nothing_improves(sis, ref_certifier):
+    """The improvement path when no replica certifies below the seed's side
+    (the 2 x 2 grid of four squares, side 2.00004, searched at T = 1, where
+    every replica overlaps far beyond a dilation of 1.008): the result is the
+    seed itself, at the seed's side, and it certifies (the reference writes
+    the seed and its L when no chain improves on it, cuda/packing.cu:820,
+    859-865)."""
+    from refcert import certify_q
+    f = 1.00002
+    q = np.array([[0.5 * f, 0.5 * f, 0.0], [1.5 * f, 0.5 * f, 0.0], [0.5 * f, 1.5 * f, 0.0], [1.5 * f, 1.5 * f, 0.0]])
+    say("seed kept")
+    for mode in (0, 2):
+        r = sis.solve(4, 2 * f, q, replicas=16, ladder=8, sweeps=200, improve=1, certify_every=1, init_mode=mode,
+                      tmin=1.0, tmax=1.0, shrink=1e-3, polish_iters=20, rng_seed=3)
+        assert r.feasible == 0 and r.target_L == 2 * f
+        assert r.L_cert <= 2 * f + 1e-12
+        dil, L50 = certify_q(ref_certifier, r.q)
+        assert dil is not None and float(L50) <= 2 * f + 1e-6

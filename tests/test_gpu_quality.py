@@ -1,0 +1,87 @@
+"""Certified L of the drop-in `packing` binary on the reference's own demo
+workload (cuda/demo.py:11-23, 72-86: grid seeds, the tilted five for n = 5,
+coordinates and side scaled by 1.00002; `packing seed 20000 32 256 42`),
+beside the reference's recorded outputs cuda/results/raw{5,10,11,17,50}.json.
+Every output passes the reference's interval certifier (oracle/_ref).
+
+Margins: n = 5, 10, 17 and 50 reach or beat the recorded target_L (the
+engine starts the chains from Goebel-strip and tilted-grid topologies, the
+reference's portfolio, and polishes in FP64); n = 11 with the reference's
+argv runs ceil(32 / 11) = 3 sweeps in each of 80 epochs and keeps the seed,
+while `steps` counted as sweeps (PACK_SWEEP_STEPS=1) reaches below 3.93
+(recorded: 3.890)."""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, str(ROOT / "tests" / "golden"))
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _demo_seed(tmp_path, ref_certifier, n):
+    import analytic
+    from refcert import certify_rows
+    rows = analytic.tilted_five() if n == 5 else analytic.grid(n)
+    ok, side, _ = certify_rows(ref_certifier, n, rows)
+    assert ok
+    side = float(side)
+    text = f"{n} {side * 1.00002:.12g}\n" + "".join(
+        " ".join(format(float(r[k]) * (1.00002 if k < 2 else 1), ".12g") for k in range(3)) + "\n" for r in rows)
+    path = tmp_path / f"seed{n}.txt"
+    path.write_text(text)
+    return path, side * 1.00002
+
+
+def _run(tmp_path, seed, n, env_extra=None):
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    out = tmp_path / f"raw{n}.json"
+    r = subprocess.run([str(PKG / "bin" / "packing"), str(seed), "20000", "32", "256", "42", str(out)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    return json.loads(out.read_text())
+
+
+@pytest.mark.parametrize("n,margin", [(5, 0.0), (10, 0.0), (17, 0.0), (50, 0.0), (11, None)])
+def test_packing_on_the_reference_demo_seeds(tmp_path, ref_certifier, n, margin):
+    from refcert import certify_q
+    seed, seed_L = _demo_seed(tmp_path, ref_certifier, n)
+    say("demo", n)
+    raw = _run(tmp_path, seed, n)
+    ref = json.loads((ROOT / "tests/golden/raw" / f"raw{n}.json").read_text())
+    dil, L50 = certify_q(ref_certifier, np.asarray(raw["q"]))
+    assert dil is not None, "the reference certifier rejected the output"
+    L = float(L50)
+    say("demo", n, "certified", L50, "recorded", ref["target_L"], "seed", seed_L)
+    # the steps argument counts proposals per chain and epoch, as the reference's
+    epoch_sweeps = -(-32 // n)
+    assert raw["proposals"] == raw["chains"] * 80 * epoch_sweeps * n
+    assert raw["feasible_checkpoints"] >= 1 and raw["blocks_per_sm"] >= 1
+    assert L <= seed_L + 1e-9
+    if margin is not None:
+        assert L <= ref["target_L"] + margin, (L, ref["target_L"])
+
+
+def test_packing_steps_as_sweeps_n11(tmp_path, ref_certifier):
+    from refcert import certify_q
+    seed, seed_L = _demo_seed(tmp_path, ref_certifier, 11)
+    say("demo sweeps", 11)
+    raw = _run(tmp_path, seed, 11, {"PACK_SWEEP_STEPS": "1"})
+    assert raw["proposals"] == raw["chains"] * 80 * 32 * 11
+    dil, L50 = certify_q(ref_certifier, np.asarray(raw["q"]))
+    assert dil is not None
+    say("demo sweeps", 11, "certified", L50)
+    assert float(L50) < 3.93
