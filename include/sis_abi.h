@@ -309,6 +309,14 @@ typedef struct sis_rep_state {
                                    the last launch (1e9 when the dilation would exceed 1.008) */
     double lam;                 /* its centre dilation */
     double minx, miny;          /* lower corner of the dilated configuration's bounding box */
+    /* Counters of the last sweep launch's workgroup whose first replica
+     * this is (zero in the other replicas), moved into the instance's totals
+     * and cleared by the exchange step that follows every launch. */
+    uint64_t cand;              /* stencil candidates examined */
+    uint64_t near_pairs;        /* candidate pairs with centres closer than sqrt(2), before and after */
+    uint32_t coll_tried, coll_acc;  /* collective proposals tried / accepted */
+    uint32_t kicks;             /* replicas kicked */
+    uint32_t pad2;
 } sis_rep_state;
 
 /* Per temperature slot.  The sweep reads the first four fields. */

@@ -1167,6 +1167,12 @@ int o_ladder_model(o_ladder_desc* d, o_rep_state* reps, o_slot_state* slots, int
         uint64_t bits;
         memcpy(&bits, &rs.Lc, 8);
         if (d->improve && bits < best_lc) best_lc = bits;
+        /* replicas with a certified side count as checkpoints; the sweep
+           workgroups' counters move into the instance's totals */
+        if (d->improve && rs.Lc < 1e9) d->certs += 1;
+        reps[r].cand = 0;
+        reps[r].near_pairs = 0;
+        reps[r].coll_tried = reps[r].coll_acc = reps[r].kicks = 0;
     }
     const double L0 = d->L, lmin = sqrt((double)n) + 1e-6;
     bool improved = false;

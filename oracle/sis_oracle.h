@@ -129,6 +129,8 @@ void o_certify_xycs(int n, const float* xycs, double out[4]);
 typedef struct o_rep_state {
     float E; int slot; unsigned acc; unsigned pad;
     double Lc, lam, minx, miny;
+    uint64_t cand, near_pairs;           /* sweep-workgroup counters, cleared by the exchange step */
+    uint32_t coll_tried, coll_acc, kicks, pad2;
 } o_rep_state;
 typedef struct o_slot_state {
     float inv_t_log2e, step_xy, step_th, L, T, base_xy, base_th, pad;

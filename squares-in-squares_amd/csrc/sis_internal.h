@@ -81,7 +81,7 @@ struct Tile {
 // ABI's exchange-step probe, used as they are.
 using RepState = sis_rep_state;
 using SlotState = sis_slot_state;
-static_assert(sizeof(RepState) == 48 && sizeof(SlotState) == 32, "record layouts");
+static_assert(sizeof(RepState) == 80 && sizeof(SlotState) == 32, "record layouts");
 
 // The reference's retarget multipliers by temperature class (cuda/packing.cu:503, 841).
 constexpr double kTargetMults[8] = {0.02, 0.1, 0.4, 1.0, 2.5, 5.0, 10.0, 20.0};

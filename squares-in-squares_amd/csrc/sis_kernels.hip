@@ -719,6 +719,155 @@ __device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int 
                     __uint_as_float(w2.z), n, lpi, sub, cC, cP);
 }
 
+// Lanes per mover of a level of cnt movers in one wavefront: 2^lsh, the most
+// that fit (at most 16, the lanes of one DPP row).
+__device__ __forceinline__ int level_lsh(int cnt) {
+    return cnt > 32 ? 0 : cnt > 16 ? 1 : cnt > 8 ? 2 : cnt > 4 ? 3 : 4;
+}
+
+struct MoverRec {
+    uint4 w0, w1, w2, w3;
+    uint2 w4;
+};
+__device__ __forceinline__ MoverRec load_rec(const TileLds& t, int j) {
+    const uint4* w = t.rec + (size_t)j * kRecWords;
+    MoverRec r;
+    r.w0 = w[0];
+    r.w1 = w[1];
+    r.w2 = w[2];
+    r.w3 = w[3];
+    r.w4 = *reinterpret_cast<const uint2*>(w + 4);
+    return r;
+}
+
+// One level of a one-wavefront tile with mover records when all its movers
+// fit the wavefront at once: lane l serves record a + (l >> lsh) as its
+// (l & (lpi - 1))-th lane.  The arithmetic, the order of the four partial
+// sums and the Metropolis test are those of evaluate(); a mover rejected
+// before the stencil (or a lane past the level's end) runs the same
+// instructions with its contributions masked, so the level has no divergent
+// exit.  Counts are 32-bit per lane.
+__device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, bool act, int lpi, int sub,
+                                          unsigned& cC, unsigned& cP) {
+    const bool ok = act && __uint_as_float(r.w1.z) != kNoMove;
+    const float px = __uint_as_float(r.w0.x), py = __uint_as_float(r.w0.y);
+    const float pc = __uint_as_float(r.w0.z), ps = __uint_as_float(r.w0.w);
+    const float nx = __uint_as_float(r.w1.x), ny = __uint_as_float(r.w1.y);
+    const float nc = __uint_as_float(r.w1.z), ns = __uint_as_float(r.w1.w);
+    const int self = (int)(r.w2.w >> 16);
+    const Stencil sc = Stencil::unpack(r.w3, r.w4.x);
+    const int last = ok ? sc.total - 1 : -1;  // the whole group of a skipped mover runs no candidate
+    unsigned near = 0;
+    auto delta = [&](const float4& v, int k, bool valid) -> float {
+        const float dxn = v.x - nx, dyn = v.y - ny;
+        const float dxo = v.x - px, dyo = v.y - py;
+        const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
+        const float eo = depth2(dxo, dyo, pc, ps, v.z, v.w);
+        const bool other = valid && k != self;
+        near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
+        near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        return other ? en - eo : 0.0f;
+    };
+    float sum;
+    if (lpi >= 8) {
+        // rounds of lpi candidates, f = round + sub; partial k (f = k mod 4)
+        // folds lanes k, k + 4 (k + 8, k + 12) of each round into lane k
+        float sa = 0.0f;
+        for (int base = 0; base <= last; base += lpi) {
+            const int f = base + sub;
+            const int k = sc.at(min(f, last));
+            const float d = delta(t.spos[k], k, f <= last);
+            const float d4 = row_up<4>(d);
+            if (lpi == 8) {
+                sa = (sa + d) + d4;
+            } else {
+                const float d8 = row_up<8>(d), d12 = row_up<12>(d);
+                sa = (((sa + d) + d4) + d8) + d12;
+            }
+        }
+        const float h = sa + quad_xor1(sa);
+        sum = h + quad_xor2(h);
+    } else {
+        // f = sub + lpi j; partial f mod 4 of step j: all four by j (lpi 1),
+        // sub or sub + 2 by the parity of j (lpi 2), always sub (lpi 4)
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        int j = 0;
+        for (int f = sub; f <= last; f += lpi, ++j) {
+            const int k = sc.at(f);
+            const float d = delta(t.spos[k], k, true);
+            const int slot = lpi == 1 ? (j & 3) : lpi == 2 ? 2 * (j & 1) : 0;
+            if (slot == 0)
+                s0 += d;
+            else if (slot == 1)
+                s1 += d;
+            else if (slot == 2)
+                s2 += d;
+            else
+                s3 += d;
+        }
+        if (lpi == 1) {
+            sum = (s0 + s1) + (s2 + s3);
+        } else if (lpi == 2) {
+            sum = (s0 + quad_xor1(s0)) + (s2 + quad_xor1(s2));
+        } else {
+            const float h = s0 + quad_xor1(s0);
+            sum = h + quad_xor2(h);
+        }
+    }
+    const bool lead = ok && sub == 0;
+    cC += lead ? (unsigned)last : 0u;
+    cP += near;
+    const float dE = __uint_as_float(r.w2.x) + sum;
+    if (lead && (dE <= 0.0f || __uint_as_float(r.w2.y) < accept_exp2(-dE * __uint_as_float(r.w2.z)))) {
+        const float4 np = make_float4(nx, ny, nc, ns);
+        t.pos[r.w2.w & 0xFFFFu] = np;
+        t.spos[self] = np;
+        atomicAdd(&t.acc[r.w4.y], 1);
+    }
+}
+
+// The levels of one sweep of a one-wavefront tile with mover records (gs:
+// lane l holds the start of level l, lane kGroups the end of the last).
+// Records do not change during a sweep (a square moves once, and a move never
+// leaves its cell), so the next level's records are read while this level
+// computes; only the candidates' positions wait for the level before.
+// Levels of more than 64 movers (tiles of more than 64 squares) take rounds
+// of the general pass.
+__device__ __forceinline__ void rec_levels(const TileLds& t, int n, int gs, unsigned long long& cC,
+                                           unsigned long long& cP) {
+    const int lane = threadIdx.x;
+    unsigned c32 = 0, p32 = 0;
+    int a = __builtin_amdgcn_readlane(gs, 0), b = __builtin_amdgcn_readlane(gs, 1);
+    int lsh = level_lsh(b - a);
+    MoverRec cur;
+    if (a < b && b - a <= kSweepThreads) cur = load_rec(t, min(a + (lane >> lsh), b - 1));
+    for (int lv = 0; lv < kGroups && a < b; ++lv) {
+        const int cnt = b - a;
+        const int lpi = 1 << lsh, sub = lane & (lpi - 1);
+        const bool act = a + (lane >> lsh) < b;
+        // the next level's bounds and records
+        const int a2 = b, b2 = lv + 1 < kGroups ? __builtin_amdgcn_readlane(gs, lv + 2) : b;
+        const int lsh2 = level_lsh(b2 - a2);
+        if (cnt <= kSweepThreads) {
+            MoverRec nxt = cur;
+            if (a2 < b2 && b2 - a2 <= kSweepThreads) nxt = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
+            rec_level(t, cur, act, lpi, sub, c32, p32);
+            cur = nxt;
+        } else {
+            const int per = kSweepThreads >> lsh;
+            for (int base = a; base < b; base += per)
+                if (base + (lane >> lsh) < b) propose_rec(t, base + (lane >> lsh), n, lpi, sub, cC, cP);
+            if (a2 < b2 && b2 - a2 <= kSweepThreads) cur = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
+        }
+        wave_sync();
+        a = a2;
+        b = b2;
+        lsh = lsh2;
+    }
+    cC += c32;
+    cP += p32;
+}
+
 // Canonical FP32 energy of every tile replica: per-square wall + pairs with
 // larger index inside the 5x5 stencil, then a fixed pairwise tree over the
 // square index, so the bits do not depend on the tile shape.
@@ -1255,13 +1404,13 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
 // residency more than the kernel needs registers).
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(const Tile* __restrict__ tiles,
-                                              InstDesc* __restrict__ insts,
+                                              const InstDesc* __restrict__ insts,
                                               RepState* __restrict__ reps,
                                               const SlotState* __restrict__ slots,
                                               float4* __restrict__ state) {
     extern __shared__ __align__(16) unsigned char smem[];
     const Tile tl = tiles[blockIdx.x];
-    InstDesc* D = insts + tl.inst;
+    const InstDesc* D = insts + tl.inst;
     const long long left = D->sweeps_total - D->sweeps_done;
     const int nsw = (int)min((long long)D->spe, left);
     if (nsw <= 0) return;
@@ -1274,6 +1423,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int ncoll = D->collective, kick_every = D->kick_every;
     const uint32_t interval = D->interval;
     const uint32_t spsh = (uint32_t)max(1, D->spsh);
+    // every descriptor field the launch uses is read here: the sweep never
+    // writes the descriptor
+    const bool improve = D->improve != 0;
+    const uint32_t certify_every = (uint32_t)max(1, D->certify_every);
+    const int rep_base = D->rep_off + tl.r0;
     TileLds t = carve(smem, n, M, G, tl.wide != 0);
     float4* gstate = state + D->state_off + (size_t)tl.r0 * n;
 
@@ -1281,7 +1435,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // replica's box follows in LDS
     for (int q = tid; q < G * n; q += NT) t.pos[q] = gstate[q];
     for (int g = tid; g < G; g += NT) {
-        const int slot = reps[D->rep_off + tl.r0 + g].slot;
+        const int slot = reps[rep_base + g].slot;
         const float4 ss = *reinterpret_cast<const float4*>(slots + D->rep_off + slot);
         t.ik[g] = ss.x;
         t.sxy[g] = ss.y;
@@ -1391,7 +1545,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             tile_sync<NT>();
         }
-        for (int lv = 0; lv < kGroups; ++lv) {
+        if constexpr (NT == 64) {
+            if (t.rec) rec_levels(t, n, gs, cC, cP);
+        }
+        for (int lv = 0; lv < kGroups && !(NT == 64 && t.rec); ++lv) {
             const int a = __builtin_amdgcn_readlane(gs, lv);
             const int b = __builtin_amdgcn_readlane(gs, lv + 1);
             if (a == b) break;  // levels are dense: the first empty one is the last
@@ -1432,11 +1589,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     }
 
-    const bool improve = D->improve != 0;
     // improve = 1: certified sides every certify_every launches and at the
     // last one; the other launches mark them absent
-    const bool certify_now =
-        improve && ((interval + 1u) % (uint32_t)max(1, D->certify_every) == 0u || (long long)nsw >= left);
+    const bool certify_now = improve && ((interval + 1u) % certify_every == 0u || (long long)nsw >= left);
     tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, certify_now);
     unsigned ctried = 0, caccd = 0, kicked = 0;
     for (int c = 0; c < ncoll; ++c)
@@ -1446,44 +1601,61 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (kick_now)
         kick<NT>(t, n, G, M, m, inv_w, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u, k0, k1,
                  kicked);
-    unsigned certs = 0;
     if (certify_now) {
         // the cell lists and largest overlaps must be those of the final
         // positions: collective moves and kicks change them after the first
         // energy pass (the canonical energy it writes again is the same)
         if (ncoll > 0 || kick_now) tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, true);
-        tile_certify<NT>(t, n, G, M, reps + D->rep_off + tl.r0, certs);
+        unsigned certs = 0;
+        tile_certify<NT>(t, n, G, M, reps + rep_base, certs);
     }
     for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
-    unsigned long long accsum = 0;
     for (int g = tid; g < G; g += NT) {
-        RepState* rs = reps + D->rep_off + tl.r0 + g;
+        RepState* rs = reps + rep_base + g;
         rs->E = t.eout[g];
         rs->acc = (unsigned)t.acc[g];
         if (improve && !certify_now) rs->Lc = kCertNone;
-        accsum += (unsigned)t.acc[g];
     }
+    // The tile's counters go to its first replica's record, stored (not
+    // added: the exchange step after every launch moves them into the
+    // instance's totals) by one thread, so that the launch makes no atomics
+    // on the shared descriptor and waits on no global load at its end.
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         cC += __shfl_xor(cC, off, 64);
         cP += __shfl_xor(cP, off, 64);
-        accsum += __shfl_xor(accsum, off, 64);
         ctried += __shfl_xor(ctried, off, 64);
-        certs += __shfl_xor(certs, off, 64);
         caccd += __shfl_xor(caccd, off, 64);
         kicked += __shfl_xor(kicked, off, 64);
     }
-    if (lane == 0) {  // one set of counter atomics per wavefront
-        atomicAdd(&D->cand, cC);
-        atomicAdd(&D->near, cP);
-        atomicAdd(&D->accepted, accsum);
-        if (ctried) {
-            atomicAdd(&D->coll_tried, (unsigned long long)ctried);
-            atomicAdd(&D->coll_acc, (unsigned long long)caccd);
+    if constexpr (NT > 64) {
+        // wavefront totals through the collective-move scratch (free now)
+        unsigned long long* red = reinterpret_cast<unsigned long long*>(t.cw);
+        static_assert(sizeof(CollWork) >= 5 * 8, "five 64-bit sums fit one replica's scratch");
+        tile_sync<NT>();
+        if (tid < 5) red[tid] = 0ull;
+        tile_sync<NT>();
+        if (lane == 0) {
+            atomicAdd(red + 0, cC);
+            atomicAdd(red + 1, cP);
+            atomicAdd(red + 2, (unsigned long long)ctried);
+            atomicAdd(red + 3, (unsigned long long)caccd);
+            atomicAdd(red + 4, (unsigned long long)kicked);
         }
-        if (kicked) atomicAdd(&D->kicks, (int)kicked);
-        if (certs) atomicAdd(&D->certs, (unsigned long long)certs);
-        if (tid == 0) atomicAdd(&D->proposals, (unsigned long long)G * n * nsw);
+        tile_sync<NT>();
+        cC = red[0];
+        cP = red[1];
+        ctried = (unsigned)red[2];
+        caccd = (unsigned)red[3];
+        kicked = (unsigned)red[4];
+    }
+    if (tid == 0) {
+        RepState* r0s = reps + rep_base;
+        r0s->cand = cC;
+        r0s->near_pairs = cP;
+        r0s->coll_tried = ctried;
+        r0s->coll_acc = caccd;
+        r0s->kicks = kicked;
     }
 }
 
@@ -1758,6 +1930,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     __shared__ int zero_slot;
     __shared__ int best_r;
     __shared__ unsigned long long best_lc;
+    __shared__ unsigned long long tot[7];  // accepted, certified, candidates, near pairs, collective tried / accepted, kicks
     InstDesc* D = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
     const long long left = D->sweeps_total - D->sweeps_done;
     if (left <= 0) return;
@@ -1778,14 +1951,32 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         best_r = R;
         best_lc = ~0ull;
     }
+    if (tid < 7) tot[tid] = 0ull;
     __syncthreads();
 
     // 1. step adaptation per temperature slot, from the replica that held it
     //    (and the smallest certified side: per thread, per wavefront, then one
-    //    LDS atomic per wavefront)
+    //    LDS atomic per wavefront); the interval's counters of the replicas and
+    //    of the sweep workgroups move into the instance's totals
     unsigned long long my_lc = ~0ull;
+    unsigned long long c_acc = 0, c_cert = 0, c_cand = 0, c_near = 0, c_ct = 0, c_ca = 0, c_kick = 0;
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
+        c_acc += rs.acc;
+        c_cert += (improve && rs.Lc < kCertNone) ? 1u : 0u;
+        if (rs.cand | rs.near_pairs | rs.coll_tried | rs.coll_acc | rs.kicks) {
+            c_cand += rs.cand;
+            c_near += rs.near_pairs;
+            c_ct += rs.coll_tried;
+            c_ca += rs.coll_acc;
+            c_kick += rs.kicks;
+            RepState* w = reps + base + r;
+            w->cand = 0ull;
+            w->near_pairs = 0ull;
+            w->coll_tried = 0u;
+            w->coll_acc = 0u;
+            w->kicks = 0u;
+        }
         const float rate = (float)rs.acc / (float)((long long)n * ran);
         // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
         const float f = accept_exp2((rate - target) * 0.72134752f);
@@ -1802,6 +1993,25 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) my_lc = min(my_lc, (unsigned long long)__shfl_xor(my_lc, off, 64));
         if ((tid & 63) == 0) atomicMin(&best_lc, my_lc);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c_acc += __shfl_xor(c_acc, off, 64);
+        c_cert += __shfl_xor(c_cert, off, 64);
+        c_cand += __shfl_xor(c_cand, off, 64);
+        c_near += __shfl_xor(c_near, off, 64);
+        c_ct += __shfl_xor(c_ct, off, 64);
+        c_ca += __shfl_xor(c_ca, off, 64);
+        c_kick += __shfl_xor(c_kick, off, 64);
+    }
+    if ((tid & 63) == 0 && (c_acc | c_cert | c_cand | c_near | c_ct | c_ca | c_kick)) {
+        atomicAdd(tot + 0, c_acc);
+        atomicAdd(tot + 1, c_cert);
+        atomicAdd(tot + 2, c_cand);
+        atomicAdd(tot + 3, c_near);
+        atomicAdd(tot + 4, c_ct);
+        atomicAdd(tot + 5, c_ca);
+        atomicAdd(tot + 6, c_kick);
     }
     __syncthreads();
 
@@ -1950,6 +2160,14 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         D->scale = (float)scale;
         D->sweeps_done += ran;
         D->interval += 1u;
+        D->proposals += (unsigned long long)n * (unsigned long long)R * (unsigned long long)ran;
+        D->accepted += tot[0];
+        D->certs += tot[1];
+        D->cand += tot[2];
+        D->near += tot[3];
+        D->coll_tried += tot[4];
+        D->coll_acc += tot[5];
+        D->kicks += (int)tot[6];
     }
 }
 

@@ -159,12 +159,17 @@ int sis_dev_launch(int n, int R, float Lgrid, const float* box, float cell, int 
         if (reps_out) reps_out[r] = reps[r];
     }
     if (counts) {
-        counts[0] = D.proposals;
-        counts[1] = D.cand;
-        counts[2] = D.near;
-        counts[3] = D.coll_tried;
-        counts[4] = D.coll_acc;
-        counts[5] = (uint64_t)D.kicks;
+        // a launch without the exchange step leaves its counters in the
+        // replica records (each tile's in its first replica)
+        counts[0] = (uint64_t)n * (uint64_t)R * (uint64_t)sweeps;
+        for (int k = 1; k < 6; ++k) counts[k] = 0;
+        for (int r = 0; r < R; ++r) {
+            counts[1] += reps[r].cand;
+            counts[2] += reps[r].near_pairs;
+            counts[3] += reps[r].coll_tried;
+            counts[4] += reps[r].coll_acc;
+            counts[5] += reps[r].kicks;
+        }
     }
     return SIS_OK;
 }

@@ -36,7 +36,9 @@ class SweepCfg(ctypes.Structure):
 
 # o_rep_state / o_slot_state / o_ladder_desc of sis_oracle.h
 REP_DTYPE = np.dtype([("E", "<f4"), ("slot", "<i4"), ("acc", "<u4"), ("pad", "<u4"), ("Lc", "<f8"),
-                      ("lam", "<f8"), ("minx", "<f8"), ("miny", "<f8")])
+                      ("lam", "<f8"), ("minx", "<f8"), ("miny", "<f8"),
+                      ("cand", "<u8"), ("near_pairs", "<u8"), ("coll_tried", "<u4"), ("coll_acc", "<u4"),
+                      ("kicks", "<u4"), ("pad2", "<u4")])
 SLOT_DTYPE = np.dtype([("inv_t_log2e", "<f4"), ("step_xy", "<f4"), ("step_th", "<f4"), ("L", "<f4"),
                        ("T", "<f4"), ("base_xy", "<f4"), ("base_th", "<f4"), ("pad", "<f4")])
 

@@ -53,6 +53,41 @@ def test_ctypes_layouts_match_header(sis, tmp_path):
         assert got == int(val), (cname, field, got, val)
 
 
+def test_record_dtypes_match_header(sis, tmp_path):
+    """The numpy records the probes exchange (replica, temperature slot) and
+    the exchange-step descriptor have the header's size and field offsets,
+    and the oracle's mirrors have the same layout."""
+    import oracle_lib
+    recs = {"sis_rep_state": sis.REP_DTYPE, "sis_slot_state": sis.SLOT_DTYPE}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, dt in recs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname in dt.names:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append('printf("sis_ladder_desc size %zu\\n", sizeof(sis_ladder_desc));')
+    for fname, _ in sis.SisLadderDesc._fields_:
+        lines.append(f'printf("sis_ladder_desc {fname} %zu\\n", offsetof(sis_ladder_desc, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "records.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "records"
+    subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for ln in filter(None, out):
+        cname, field, val = ln.split()
+        if cname == "sis_ladder_desc":
+            py = sis.SisLadderDesc
+            got = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        else:
+            dt = recs[cname]
+            got = dt.itemsize if field == "size" else dt.fields[field][1]
+        assert got == int(val), (cname, field, got, val)
+        seen += 1
+    assert seen > 30
+    assert oracle_lib.REP_DTYPE == sis.REP_DTYPE and oracle_lib.SLOT_DTYPE == sis.SLOT_DTYPE
+
+
 def test_default_params(sis):
     p = sis.default_params()
     assert p.tmin == 1e-5 and p.tmax == 5e-2

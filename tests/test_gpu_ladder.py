@@ -252,7 +252,6 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
         assert np.array_equal(dev["slots"].view(np.uint32), sl2.view(np.uint32)), iv
         assert np.array_equal(dev["ros"], ros2) and np.array_equal(dev["arch"], ar2), iv
         assert np.array_equal(dev["state"], st2), iv
-        d["certs"] += int(np.sum(reps["Lc"] < 1e8)) if p.improve else 0
         for k, v in d.items():
             assert dev["desc"][k] == v, (iv, k, dev["desc"][k], v)
         s = dev
