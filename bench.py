@@ -128,6 +128,21 @@ def reference_certifier_check(packings):
             "dilations": {str(d): sum(1 for v in passed.values() if v[0] == d) for d in (1.000002, 1.00001, 1.0001)}}
 
 
+def near_pair_replay(sis, insts, params, intervals_runs=None):
+    """Near pairs per proposal of a workload from an untimed replay with
+    near-pair counting on (FLAG_NEAR_PAIRS).  Counting changes no arithmetic,
+    so the replay makes the same proposals as the timed run, which counts only
+    the candidates.  intervals_runs: the run() calls of the timed session
+    (default: the whole schedule at once).  Returns (P per proposal, results)."""
+    sess = sis.BatchSession(insts, [sis.with_near_pairs(p) for p in params])
+    for iv, mode in intervals_runs or [(0, 1)]:
+        sess.run(iv, mode=mode)
+    res = sess.finalize()
+    sess.close()
+    props = sum(r.proposals for r in res)
+    return sum(r.near_pairs for r in res) / max(1, props), res
+
+
 def traffic_per_launch():
     """HBM bytes per sweep launch of the plain n=50 workload, from the
     FETCH_SIZE and WRITE_SIZE counter runs summarised in
@@ -240,7 +255,7 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
     batched = timed / wall
     props = sum(r.proposals for r in res)
     c_per = sum(r.candidates for r in res) / props
-    p_per = sum(r.near_pairs for r in res) / props
+    p_per, _ = near_pair_replay(sis, insts, params)
     feas = sum(r.feasible for r in res)
     # plain form on a sample of the same stream, same sweeps: same results
     plain_props, plain_wall, sweep_ms, same = 0, 0.0, 0.0, 0
@@ -350,8 +365,13 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         srv.release(tk)
     srv.close()
     props = sum(results[tk].proposals for tk in tickets)
-    flops = sum(12.0 * results[tk].candidates + 28.0 * results[tk].near_pairs + 45.0 * results[tk].proposals
-                for tk in tickets)
+    # near pairs per proposal from an untimed replay, with near-pair
+    # counting, of every k-th request (at most 32) as one batch
+    step = max(1, -(-len(reqs) // 32))
+    sample = reqs[::step]
+    p_per, _ = near_pair_replay(sis, [(n,) + grid_seed(n) for (_, n, *_r) in sample],
+                                [params(n, sweeps, R, polish, s_) for (_, n, sweeps, R, polish, s_) in sample])
+    flops = sum(12.0 * results[tk].candidates + (28.0 * p_per + 45.0) * results[tk].proposals for tk in tickets)
     offered = sum(n * R * sweeps for (_, n, sweeps, R, _, _) in reqs)
     lat = np.array(lat)
     served = props / wall
@@ -404,6 +424,7 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         "ratio_served_vs_plain_stream_mix": served / plain_mix if plain_mix else None,
         "plain_requests": k_all, "plain_results_identical": same_all,
         "size_classes": classes,
+        "P_per_proposal": p_per, "P_source": f"untimed replay of {len(sample)} requests with near-pair counting",
         "served_tflops": flops / wall / 1e12,
         "fraction_of_fp32_peak": flops / wall / (PEAK_FP32_TFLOPS * 1e12),
     }
@@ -502,6 +523,7 @@ def main():
         print(json.dumps({"copy_bytes": x.numel() * 4}), flush=True)
         return
 
+    import numpy as np
     import sisamd as sis
 
     cus, dev = sis.device_info()
@@ -528,9 +550,12 @@ def main():
 
     timed_props = a.steps * SWEEPS_PER_STEP * N * REPLICAS
     value = timed_props / wall
-    # algorithmic FLOP per proposal from the device counters (whole run)
+    # algorithmic FLOP per proposal from the device counters (whole run); the
+    # near pairs (old + new position) from an untimed replay of the same run
     c_per = res.candidates / res.proposals
-    p_per = res.near_pairs / res.proposals  # old + new position
+    p_per, rep = near_pair_replay(sis, [(N, L0, None)], [p], [(intervals_per_step, 0)] * total_steps)
+    replay_same = bool(rep[0].L_cert == res.L_cert and rep[0].accepted == res.accepted
+                       and np.array_equal(rep[0].q, res.q))
     flop_per_prop = 12.0 * c_per + 28.0 * p_per + 45.0
     achieved = timed_props * flop_per_prop / (sweep_ms * 1e-3) / 1e12 if sweep_ms > 0 else None
     line = {
@@ -564,6 +589,7 @@ def main():
             "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
             "traffic": traffic_per_launch(),
             "flop_per_proposal": flop_per_prop, "C_per_proposal": c_per, "P_per_proposal": p_per,
+            "P_source": "untimed replay of the same run with near-pair counting (identical: %s)" % replay_same,
             "sweep_kernel_ms_avg": sweep_ms / max(1, launches), "sweep_launches": launches,
         },
         "quality": {

@@ -29,6 +29,12 @@ extern "C" {
 #define SIS_EINVAL 2
 #define SIS_EDEVICE 3
 
+/* sis_params.flags: count the near pairs (centres closer than sqrt(2), before
+ * and after the move) of every proposal into sis_result.near_pairs, the P of
+ * the roofline's flop model.  A diagnostic: it costs about 3 % of sweep
+ * throughput, and without it near_pairs is 0. */
+#define SIS_FLAG_NEAR_PAIRS 1
+
 /* One packing instance: n unit squares in a box of side L.
  * Replaces the seed file `n L` + n lines `x y angle` (cuda/packing.cu:772-775). */
 typedef struct sis_instance {
@@ -55,7 +61,7 @@ typedef struct sis_params {
     int polish_iters;           /* FP64 conjugate-gradient polish iterations at the end */
     int init_mode;              /* 0: seed configuration (+ init_jitter) in every replica;
                                    1: uniform random; 2: portfolio (sis_portfolio) */
-    int flags;                  /* reserved, 0 */
+    int flags;                  /* 0 or SIS_FLAG_NEAR_PAIRS */
     uint64_t rng_seed;          /* Philox key */
     int collective_moves;       /* collective proposals per replica after each interval's sweeps
                                    (row / column / cluster / orientation / diagonal / tilted-split

@@ -61,6 +61,7 @@ public:
     long long max_intervals = 0, intervals_done = 0;
     double device_ms = 0, last_ms = 0, sweep_ms = 0;
     long long sweep_launches = 0;
+    bool near_pairs = false;  // some instance asks for near-pair counts (SIS_FLAG_NEAR_PAIRS)
 
     hipStream_t stream = nullptr, stream2 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;

@@ -476,8 +476,9 @@ __device__ __forceinline__ bool make_move(const TileLds& t, int g, const float4&
 // lane is `sub`).  The stencil energy change is four partial sums over the
 // flat candidate index f (f mod 4), each in increasing f, combined as
 // (s0 + s1) + (s2 + s3): the same bits whatever the number of lanes sharing
-// the work.
-template <bool kQuads>
+// the work.  kNear: also count the near pairs (centres closer than sqrt 2)
+// before and after the move.
+template <bool kQuads, bool kNear>
 __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int self, const float4& p,
                                          const Stencil& sc, const Move& mv, float u, float ik, int n,
                                          int lpi, int sub, unsigned long long& cC,
@@ -496,8 +497,10 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
         const bool other = valid && k != self;
         // near pairs: two compare masks combined with `other` on the scalar
         // unit, two conditional increments
-        near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
-        near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        if constexpr (kNear) {
+            near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
+            near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        }
         return other ? en - eo : 0.0f;
     };
     // Four candidates at once: their positions are read from the cell-list
@@ -628,7 +631,7 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
 // One Metropolis proposal for square q of its tile replica: the move comes
 // from t.prop (staged for the whole sweep before the first level) or is drawn
 // here.
-template <bool kQuads>
+template <bool kQuads, bool kNear>
 __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, int m, float inv_w,
                                         int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
                                         int lpi, int sub, unsigned long long& cC,
@@ -659,7 +662,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
         if (!make_move(t, g, p, r, kx, ky, m, inv_w, mv)) return;
         u = r.w;
     }
-    evaluate<kQuads>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
+    evaluate<kQuads, kNear>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
 }
 
 // Mover record j of a level (one-wavefront tiles): square q (replica g,
@@ -700,6 +703,7 @@ __device__ __forceinline__ void write_record(const TileLds& t, int j, int q, int
 
 // The proposal of mover record j: one LDS round trip for everything but the
 // candidates.
+template <bool kNear>
 __device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int lpi, int sub,
                                             unsigned long long& cC, unsigned long long& cP) {
     const uint4* w = t.rec + (size_t)j * kRecWords;
@@ -715,7 +719,7 @@ __device__ __forceinline__ void propose_rec(const TileLds& t, int j, int n, int 
     const Move mv{__uint_as_float(w1.x), __uint_as_float(w1.y), __uint_as_float(w1.z), __uint_as_float(w1.w),
                   __uint_as_float(w2.x)};
     const Stencil sc = Stencil::unpack(w3, w4.x);
-    evaluate<false>(t, (int)(w2.w & 0xFFFFu), (int)w4.y, (int)(w2.w >> 16), p, sc, mv, __uint_as_float(w2.y),
+    evaluate<false, kNear>(t, (int)(w2.w & 0xFFFFu), (int)w4.y, (int)(w2.w >> 16), p, sc, mv, __uint_as_float(w2.y),
                     __uint_as_float(w2.z), n, lpi, sub, cC, cP);
 }
 
@@ -747,6 +751,7 @@ __device__ __forceinline__ MoverRec load_rec(const TileLds& t, int j) {
 // before the stencil (or a lane past the level's end) runs the same
 // instructions with its contributions masked, so the level has no divergent
 // exit.  Counts are 32-bit per lane.
+template <bool kNear>
 __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, bool act, int lpi, int sub,
                                           unsigned& cC, unsigned& cP) {
     const bool ok = act && __uint_as_float(r.w1.z) != kNoMove;
@@ -764,8 +769,10 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
         const float en = depth2(dxn, dyn, nc, ns, v.z, v.w);
         const float eo = depth2(dxo, dyo, pc, ps, v.z, v.w);
         const bool other = valid && k != self;
-        near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
-        near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        if constexpr (kNear) {
+            near += (unsigned)(other && __builtin_fmaf(dxn, dxn, dyn * dyn) < 2.0f);
+            near += (unsigned)(other && __builtin_fmaf(dxo, dxo, dyo * dyo) < 2.0f);
+        }
         return other ? en - eo : 0.0f;
     };
     float sum;
@@ -833,6 +840,7 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
 // computes; only the candidates' positions wait for the level before.
 // Levels of more than 64 movers (tiles of more than 64 squares) take rounds
 // of the general pass.
+template <bool kNear>
 __device__ __forceinline__ void rec_levels(const TileLds& t, int n, int gs, unsigned long long& cC,
                                            unsigned long long& cP) {
     const int lane = threadIdx.x;
@@ -851,12 +859,12 @@ __device__ __forceinline__ void rec_levels(const TileLds& t, int n, int gs, unsi
         if (cnt <= kSweepThreads) {
             MoverRec nxt = cur;
             if (a2 < b2 && b2 - a2 <= kSweepThreads) nxt = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
-            rec_level(t, cur, act, lpi, sub, c32, p32);
+            rec_level<kNear>(t, cur, act, lpi, sub, c32, p32);
             cur = nxt;
         } else {
             const int per = kSweepThreads >> lsh;
             for (int base = a; base < b; base += per)
-                if (base + (lane >> lsh) < b) propose_rec(t, base + (lane >> lsh), n, lpi, sub, cC, cP);
+                if (base + (lane >> lsh) < b) propose_rec<kNear>(t, base + (lane >> lsh), n, lpi, sub, cC, cP);
             if (a2 < b2 && b2 - a2 <= kSweepThreads) cur = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
         }
         wave_sync();
@@ -1402,7 +1410,7 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
 
 // Four waves per SIMD: at most 128 VGPRs (the tiles under load need the
 // residency more than the kernel needs registers).
-template <int NT>
+template <int NT, bool kNear>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(const Tile* __restrict__ tiles,
                                               const InstDesc* __restrict__ insts,
                                               RepState* __restrict__ reps,
@@ -1546,7 +1554,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             tile_sync<NT>();
         }
         if constexpr (NT == 64) {
-            if (t.rec) rec_levels(t, n, gs, cC, cP);
+            if (t.rec) rec_levels<kNear>(t, n, gs, cC, cP);
         }
         for (int lv = 0; lv < kGroups && !(NT == 64 && t.rec); ++lv) {
             const int a = __builtin_amdgcn_readlane(gs, lv);
@@ -1563,12 +1571,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (base + slot >= b) continue;
                 if constexpr (NT == 64) {
                     if (t.rec) {
-                        propose_rec(t, base + slot, n, lpi, sub, cC, cP);
+                        propose_rec<kNear>(t, base + slot, n, lpi, sub, cC, cP);
                         continue;
                     }
                 }
-                propose<(NT > 64)>(t, t.items[base + slot], n, M, m, inv_w, tl.r0, sweep, k0, k1, lpi, sub, cC,
-                                   cP);
+                propose<(NT > 64), kNear>(t, t.items[base + slot], n, M, m, inv_w, tl.r0, sweep, k0, k1, lpi, sub,
+                                          cC, cP);
             }
             tile_sync<NT>();
         }
@@ -1576,7 +1584,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (tid == 0)
                 for (int q = 0; q < G * n; ++q)
                     if (t.rank[q] == kGroups)
-                        propose<(NT > 64)>(t, q, n, M, m, inv_w, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
+                        propose<(NT > 64), kNear>(t, q, n, M, m, inv_w, tl.r0, sweep, k0, k1, 1, 0, cC, cP);
             tile_sync<NT>();
         }
         if ((sweep + 1u) % spsh == 0u) {  // the next sweep starts a block: its shift becomes current
@@ -2466,15 +2474,24 @@ __global__ void k_cellcoord(int count, const float* __restrict__ x, float shift,
 // ---------------------------------------------------------------------------
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
                         InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
-                        int threads) {
-    if (threads == kSweepThreadsBig)
-        hipLaunchKernelGGL(k_sweep<kSweepThreadsBig>, dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
-                           tiles, insts, reps, slots, state);
-    else if (threads == kSweepThreads)
-        hipLaunchKernelGGL(k_sweep<kSweepThreads>, dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
-                           insts, reps, slots, state);
-    else
+                        int threads, bool near_pairs) {
+    if (threads == kSweepThreadsBig) {
+        if (near_pairs)
+            hipLaunchKernelGGL((k_sweep<kSweepThreadsBig, true>), dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
+                               tiles, insts, reps, slots, state);
+        else
+            hipLaunchKernelGGL((k_sweep<kSweepThreadsBig, false>), dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
+                               tiles, insts, reps, slots, state);
+    } else if (threads == kSweepThreads) {
+        if (near_pairs)
+            hipLaunchKernelGGL((k_sweep<kSweepThreads, true>), dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
+                               insts, reps, slots, state);
+        else
+            hipLaunchKernelGGL((k_sweep<kSweepThreads, false>), dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
+                               insts, reps, slots, state);
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -2527,8 +2544,9 @@ hipError_t launch_accept_exp2(hipStream_t st, int count, const float* x, float* 
 }
 
 const void* sweep_kernel_ptr(int threads) {
-    return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig>)
-                                       : reinterpret_cast<const void*>(k_sweep<kSweepThreads>);
+    // the production form (no near-pair counts); both forms use the same LDS
+    return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, false>)
+                                       : reinterpret_cast<const void*>(k_sweep<kSweepThreads, false>);
 }
 
 }  // namespace sis

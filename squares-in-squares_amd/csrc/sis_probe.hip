@@ -148,7 +148,7 @@ int sis_dev_launch(int n, int R, float Lgrid, const float* box, float cell, int 
     SIS_HIPCHECK(hipMemcpy(d_slots, slots.data(), sizeof(SlotState) * R, hipMemcpyHostToDevice));
     SIS_HIPCHECK(hipMemcpy(d_state, xycs, sizeof(float4) * (size_t)n * R, hipMemcpyHostToDevice));
     SIS_HIPCHECK(launch_sweep(nullptr, (int)tiles.size(), lds, d_tiles, d_inst, d_reps, d_slots, d_state,
-                              sweep_threads(n)));
+                              sweep_threads(n), true));
     SIS_HIPCHECK(hipDeviceSynchronize());
     SIS_HIPCHECK(hipMemcpy(xycs, d_state, sizeof(float4) * (size_t)n * R, hipMemcpyDeviceToHost));
     SIS_HIPCHECK(hipMemcpy(reps.data(), d_reps, sizeof(RepState) * R, hipMemcpyDeviceToHost));

@@ -123,6 +123,7 @@ int validate(const sis_instance* in, const sis_params* p) {
     if (p->polish_iters < 0 || p->polish_iters > 100000) return fail(SIS_EINVAL, "polish_iters out of range");
     if (!(p->init_jitter >= 0 && p->init_jitter < 1)) return fail(SIS_EINVAL, "init_jitter in [0,1)");
     if (p->init_mode < 0 || p->init_mode > 2) return fail(SIS_EINVAL, "init_mode must be 0, 1 or 2");
+    if (p->flags & ~SIS_FLAG_NEAR_PAIRS) return fail(SIS_EINVAL, "unknown flags");
     if (p->collective_moves < 0 || p->collective_moves > 64) return fail(SIS_EINVAL, "collective_moves in 0..64");
     if (p->kick_every < 0 || p->kick_every > 1000000) return fail(SIS_EINVAL, "kick_every in 0..1e6");
     if (p->improve < 0 || p->improve > 1) return fail(SIS_EINVAL, "improve must be 0 or 1");
@@ -391,8 +392,10 @@ int Batch::build(std::vector<InstSpec>&& in) {
     size_t nsq = 0, nrep = 0, narch = 0;
     lds_big = lds_small = lds_ladder = 0;
     max_intervals = 0;
+    near_pairs = false;
     for (int k = 0; k < ni; ++k) {
         const InstSpec& s = specs[k];
+        near_pairs = near_pairs || (s.p.flags & SIS_FLAG_NEAR_PAIRS) != 0;
         Plan& pl = plans[k];
         int rc = plan_instance(s, (int)std::min<long long>(total_reps, 1 << 30), pl);
         if (rc) return rc;
@@ -519,10 +522,11 @@ int Batch::enqueue_chain(int ci) {
     Chain& c = chains[ci];
     const Tile* tiles = d_ctiles + c.tile_off;
     if (c.n_big > 0)
-        HIPCHECK(launch_sweep(c.st, c.n_big, c.lds_big, tiles, d_inst, d_reps, d_slots, d_state, kSweepThreadsBig));
+        HIPCHECK(launch_sweep(c.st, c.n_big, c.lds_big, tiles, d_inst, d_reps, d_slots, d_state, kSweepThreadsBig,
+                              near_pairs));
     if (c.ntiles > c.n_big)
         HIPCHECK(launch_sweep(c.st, c.ntiles - c.n_big, c.lds_small, tiles + c.n_big, d_inst, d_reps, d_slots,
-                              d_state, kSweepThreads));
+                              d_state, kSweepThreads, near_pairs));
     HIPCHECK(launch_ladder(c.st, c.ninst, c.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
                            d_clist + c.list_off));
     return SIS_OK;
@@ -542,10 +546,10 @@ int Batch::launch_tiles() {
     }
     if (n_big > 0)
         HIPCHECK(launch_sweep(stream, n_big, lds_big, d_tiles, d_inst, d_reps, d_slots, d_state,
-                              kSweepThreadsBig));
+                              kSweepThreadsBig, near_pairs));
     if (nt > n_big)
         HIPCHECK(launch_sweep(both ? stream2 : stream, nt - n_big, lds_small, d_tiles + n_big, d_inst, d_reps,
-                              d_slots, d_state, kSweepThreads));
+                              d_slots, d_state, kSweepThreads, near_pairs));
     if (both) {
         HIPCHECK(hipEventRecord(ev_join, stream2));
         HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));

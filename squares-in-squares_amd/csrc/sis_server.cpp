@@ -125,6 +125,7 @@ struct Lane {
     bool in_flight = false;
     bool dirty = true;
     bool planned_loaded = false;      // device load the tiles were planned for
+    bool near_pairs = false;          // some active request counts near pairs (SIS_FLAG_NEAR_PAIRS)
     std::vector<Req*> active;         // resident, in list order
     std::vector<Tile> tiles;          // four-wavefront tiles first (n_big of them)
     std::vector<int> list;            // descriptor slots of `active`
@@ -312,7 +313,9 @@ int Server::launch_round(Lane& ln) {
         ln.tiles.clear();
         ln.list.clear();
         ln.lds_big = ln.lds_small = ln.lds_ladder = 0;
+        ln.near_pairs = false;
         for (Req* r : ln.active) {
+            ln.near_pairs = ln.near_pairs || (r->spec.p.flags & SIS_FLAG_NEAR_PAIRS) != 0;
             Plan pl;
             if (plan_instance(r->spec, (int)std::min<long long>(total_reps, 1 << 30), pl)) {
                 loop_err = "tile planning failed";
@@ -340,10 +343,10 @@ int Server::launch_round(Lane& ln) {
     for (int k = 0; k < ipr; ++k) {
         if (ln.n_big > 0)
             SRV_CHECK(launch_sweep(ln.stream, ln.n_big, ln.lds_big, ln.d_tiles, d_inst, d_reps, d_slots, d_state,
-                                   kSweepThreadsBig));
+                                   kSweepThreadsBig, ln.near_pairs));
         if (nt > ln.n_big)
             SRV_CHECK(launch_sweep(ln.stream, nt - ln.n_big, ln.lds_small, ln.d_tiles + ln.n_big, d_inst, d_reps,
-                                   d_slots, d_state, kSweepThreads));
+                                   d_slots, d_state, kSweepThreads, ln.near_pairs));
         SRV_CHECK(launch_ladder(ln.stream, (int)ln.list.size(), ln.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
                                 ln.d_list));
     }

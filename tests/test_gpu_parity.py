@@ -466,6 +466,33 @@ def test_batch_equals_alone(sis):
             assert a.fields[k] == m.fields[k], k
 
 
+def test_near_pair_counting_changes_nothing_else(sis):
+    """SIS_FLAG_NEAR_PAIRS only adds counting: the same packing, counters and
+    certified side with and without it (plain, batched with a four-wavefront
+    instance, and allocation-free forms); near pairs are counted only with it.
+    Each near pair is a candidate seen from one of two positions, so with it
+    0 < near_pairs <= 2 candidates."""
+    insts = [(37, 8.0, None), (120, 13.0, None)]
+    ps = [sis.default_params(replicas=r, sweeps=s, rng_seed=seed, init_mode=1, tmin=1e-4, tmax=3e-2,
+                             collective_moves=2, kick_every=3, improve=1)
+          for r, s, seed in ((16, 200, 5), (8, 60, 6))]
+    say("near pairs: batch")
+    off = sis.solve_batch(insts, ps)
+    on = sis.solve_batch(insts, [sis.with_near_pairs(p) for p in ps])
+    say("near pairs: alone / no-alloc")
+    alone = sis.solve(*insts[0], params=sis.with_near_pairs(ps[0]))
+    noalloc = sis.solve_noalloc(*insts[0], params=sis.with_near_pairs(ps[0]))
+    for a, b in list(zip(off, on)) + [(off[0], alone), (off[0], noalloc)]:
+        assert np.array_equal(a.q, b.q)
+        for k in ("proposals", "accepted", "candidates", "collective_tried", "collective_accepted", "kicks",
+                  "exchanges_accepted", "shrinks", "L_cert", "target_L", "feasible_checkpoints"):
+            assert a.fields[k] == b.fields[k], k
+        assert a.near_pairs == 0 and 0 < b.near_pairs <= 2 * b.candidates
+    assert alone.near_pairs == on[0].near_pairs == noalloc.near_pairs
+    with pytest.raises(sis.SisError):
+        sis.solve(*insts[0], params=sis.default_params(flags=2, sweeps=10))
+
+
 def test_autotune_returns_a_candidate_that_runs(sis):
     """The autotuner tries every (tile size, sweeps per exchange, cell width)
     candidate on the batch and returns the fastest; its choice is one of the
