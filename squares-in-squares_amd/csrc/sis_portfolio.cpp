@@ -9,10 +9,17 @@
 //     capacity is n..n+4, sorted by box side;
 //   - wall-hugging grids with a column and a row of tilted squares;
 //   - four corner squares and a tilted grid.
-// This file builds the same families for a request (square order and
-// positions within the families are this build's own) and assigns them to
-// replicas by ladder and temperature position.  Everything is FP64 and
-// deterministic; the jitter draws from the request's Philox stream.
+// This file builds the same families for a request and assigns them to
+// replicas by ladder and temperature position.  The families follow
+// cuda/packing.cu:304-452 step for step, because they define the starting
+// chains the `packing` contract produces: the Goebel-strip enumeration (stack
+// sizes, band widths, box-side formula, capacity window, square order, the 64
+// smallest), the wall grid and the corner-tilted grid (square positions and
+// order), and the assignment of families to the eight temperature classes.
+// What differs: the assignment is written for ladders of any length (class =
+// position x 8 / ladder), ties in the sort keep enumeration order, and the
+// jitter draws from the request's Philox stream instead of the reference's
+// generator.  Everything is FP64 and deterministic.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
