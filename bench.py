@@ -333,8 +333,9 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         polish = 200 if rng.random() < 0.5 else 0
         reqs.append((t, n, sweeps, R, polish, 100 + len(reqs)))
     cpu = max(1, min(16, host_cpu()["nproc"]))
+    # 16 GB of device arena: the stream holds at most about 4 GB of replica state
     srv = sis.Server(max_instances=1024, intervals_per_round=5, certifier_threads=cpu, certify=True,
-                     hbm_budget_bytes=200e9)
+                     hbm_budget_bytes=16e9)
 
     def params(n, sweeps, R, polish, s):
         return sis.default_params(replicas=R, sweeps=sweeps, polish_iters=polish, rng_seed=s, init_mode=0,

@@ -28,6 +28,12 @@ extern "C" {
 #define SIS_PENDING 1
 #define SIS_EINVAL 2
 #define SIS_EDEVICE 3
+/* A served request that reached its deadline (sis_submit_timeout,
+ * sis_server_cfg.request_timeout_s; the reference gives each case's `packing`
+ * process 900 s, cuda/cpp_loop.py:63, 78).  As sis_result.status: stopped at
+ * a round boundary, the result is the best configuration found so far.  As
+ * sis_poll's return: the deadline passed before admission, no result. */
+#define SIS_ETIMEOUT 4
 
 /* sis_params.flags: count the near pairs (centres closer than sqrt(2), before
  * and after the move) of every proposal into sis_result.near_pairs, the P of
@@ -133,6 +139,7 @@ typedef struct sis_server_cfg {
     int intervals_per_round;    /* exchange intervals per device round */
     int certifier_threads;      /* host threads that certify finished packings */
     int certify;                /* 1: certify every result (dilation ladder + exact certifier) */
+    double request_timeout_s;   /* deadline of every request, seconds after sis_submit; 0: none */
 } sis_server_cfg;
 
 typedef struct sis_server sis_server;
@@ -232,8 +239,18 @@ int sis_autotune(const sis_instance* insts, int count, const sis_params* params,
 sis_server* sis_server_create(const sis_server_cfg* cfg);
 int sis_submit(sis_server* s, const sis_instance* inst, const sis_params* params,
                uint64_t* ticket);
-/* SIS_OK when finished (out filled; out->q_out must hold n*3 doubles),
- * SIS_PENDING while queued or running. */
+/* sis_submit with a deadline of its own, timeout_s seconds after the call
+ * (0: none), in place of the configuration's request_timeout_s
+ * (cuda/cpp_loop.py:63: `timeout=900` on each case). */
+int sis_submit_timeout(sis_server* s, const sis_instance* inst, const sis_params* params, double timeout_s,
+                       uint64_t* ticket);
+/* SIS_OK when finished (out filled; out->q_out must hold n*3 doubles;
+ * out->status SIS_OK, SIS_EINVAL when the certifier rejected it, SIS_ETIMEOUT
+ * when it stopped at its deadline), SIS_PENDING while queued or running,
+ * SIS_ETIMEOUT when its deadline passed before admission and SIS_EDEVICE when
+ * a device round that held it failed (the message in sis_last_error; requests
+ * in other lanes are not affected).  The reference records such cases as
+ * "rejected" with the exception text (cuda/cpp_loop.py:77-78). */
 int sis_poll(sis_server* s, uint64_t ticket, sis_result* out);
 /* Seconds from submit to a certified result, and the certified L (50 decimals). */
 int sis_ticket_times(sis_server* s, uint64_t ticket, double* queued_s, double* done_s,
@@ -244,6 +261,17 @@ int sis_ticket_times(sis_server* s, uint64_t ticket, double* queued_s, double* d
  * whose request loop ends with its process pool. */
 int sis_ticket_release(sis_server* s, uint64_t ticket);
 int sis_server_destroy(sis_server* s);
+/* Admission state: the device arena's capacity in bytes (hbm_budget_bytes,
+ * at most 90 % of the device memory free at creation), the bytes of replica
+ * state resident now and at most so far, and the requests resident now.  The
+ * reference's admission reads free VRAM and RAM (cuda/cpp_loop.py:35-45). */
+int sis_server_stats(sis_server* s, double* arena_bytes, double* resident_bytes, double* peak_resident_bytes,
+                     int* resident_requests);
+/* Test hook, no counterpart in the reference: the next round of lane `lane`
+ * (0: n < 64, 1: n < 256, 2 and 3: the rest) fails at its first sweep launch,
+ * before anything is enqueued, as a failed launch would.  For tests of the
+ * lane-failure path, which need no faulting kernel. */
+int sis_server_inject_lane_fault(sis_server* s, int lane);
 
 /* Exact certificate of one packing: decimal strings in, 50-decimal L out.
  * Replaces the certifier process boundary (cuda/certifier.cpp:933-1077,

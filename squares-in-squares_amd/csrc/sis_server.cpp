@@ -97,6 +97,10 @@ struct Req {
     std::string L50, cert_reason;
     double dilation = 0;
     bool certified = false;
+    double deadline = 0;        // now_s() value after which the request stops; 0: none
+    bool timed_out = false;     // stopped at a round boundary past its deadline
+    int fail_code = 0;          // SIS_EDEVICE (its lane's round failed) or SIS_ETIMEOUT (never admitted)
+    std::string fail_msg;
 };
 
 size_t request_bytes(int n, int R) {
@@ -171,6 +175,10 @@ struct Server {
     std::thread loop;
     std::vector<std::thread> certifiers;
     std::string error;       // under mu once the loop runs
+    int fault_lane = -1;     // test hook: the lane whose next round fails (under mu)
+    size_t arena_bytes = 0;
+    size_t peak_resident = 0;  // under mu
+    int resident_reqs = 0;     // under mu
 
     // loop thread only
     std::string loop_err;
@@ -196,6 +204,8 @@ struct Server {
     int admit_one(Req* r, Lane& ln);
     int launch_round(Lane& ln);
     int round_done(Lane& ln);
+    bool fail_lane(Lane& ln, const std::string& why);
+    void release_req(Req* r);
     int finalize_start(std::vector<Req*>& done);
     int finalize_done(FinBlock& fb);
     bool work_left();
@@ -214,10 +224,27 @@ struct Server {
 int Server::init() {
     if (ensure_device(-1)) return SIS_EDEVICE;
     cap_inst = std::max(1, cfg.max_instances);
-    // arena: enough squares for the admission cap, bounded to 4 GiB of state
+    // arena: the admission budget (4 GiB when none is given), at most 90 % of
+    // the device memory free now less the per-instance buffers; 90 % of it for
+    // replica states, 10 % for the per-replica records (replica, slot, owner
+    // and the lanes' tiles: 180 bytes a replica, 7 % of a request at n = 150,
+    // half of one at n = 10, where the instance cap binds first)
+    size_t free_b = 0, total_b = 0;
+    SRV_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const double fixed = (double)cap_inst * (kMaxN * sizeof(float4) + kLanes * (sizeof(int) + 2 * sizeof(InstDesc)) +
+                                             sizeof(InstDesc)) +
+                         (double)kFinBlocks * kFinJobs * (sizeof(FinalizeJob) + sizeof(FinalizeOut) + 3.0 * kMaxN * 8);
     const double budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : 4.0 * (1u << 30);
-    cap_sq = (size_t)std::min(budget, 4.0 * (1u << 30)) / sizeof(float4);
-    cap_rep = std::min<size_t>(cap_sq / 10, (size_t)cap_inst * 4096);
+    const double usable = 0.9 * (double)free_b - fixed;
+    if (!(usable > 0)) {
+        loop_err = "not enough free device memory for the server";
+        return SIS_EDEVICE;
+    }
+    const double arena = std::min(budget, usable);
+    const double per_rep = sizeof(RepState) + sizeof(SlotState) + sizeof(int) + kLanes * sizeof(Tile);
+    cap_sq = (size_t)(0.9 * arena / sizeof(float4));
+    cap_rep = std::min<size_t>((size_t)(0.1 * arena / per_rep), (size_t)cap_inst * 4096);
+    arena_bytes = (size_t)((double)cap_sq * sizeof(float4) + (double)cap_rep * per_rep);
     sq_alloc.reset(cap_sq);
     rep_alloc.reset(cap_rep);
     arch_alloc.reset((size_t)cap_inst * kMaxN);
@@ -340,6 +367,14 @@ int Server::launch_round(Lane& ln) {
     }
     const int ipr = std::max(1, cfg.intervals_per_round);
     const int nt = (int)ln.tiles.size();
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (fault_lane == (int)(&ln - lanes)) {
+            fault_lane = -1;
+            loop_err = "injected launch failure";
+            return SIS_EDEVICE;
+        }
+    }
     for (int k = 0; k < ipr; ++k) {
         if (ln.n_big > 0)
             SRV_CHECK(launch_sweep(ln.stream, ln.n_big, ln.lds_big, ln.d_tiles, d_inst, d_reps, d_slots, d_state,
@@ -363,11 +398,15 @@ int Server::launch_round(Lane& ln) {
 int Server::round_done(Lane& ln) {
     ln.in_flight = false;
     std::vector<Req*> done, keep;
+    const double t = now_s();
     for (size_t k = 0; k < ln.active.size(); ++k) {
         Req* r = ln.active[k];
         h_inst[r->slot] = ln.h_desc[k];
         const InstDesc& D = h_inst[r->slot];
-        (D.sweeps_done >= D.sweeps_total ? done : keep).push_back(r);
+        const bool finished = D.sweeps_done >= D.sweeps_total;
+        // past its deadline: finalized from where it is, at this round boundary
+        if (!finished && r->deadline > 0 && t >= r->deadline) r->timed_out = true;
+        (finished || r->timed_out ? done : keep).push_back(r);
     }
     if (done.empty()) return SIS_OK;
     ln.active.swap(keep);
@@ -446,14 +485,10 @@ int Server::finalize_done(FinBlock& fb) {
         Batch none;
         fill_result(rec, none, 0, t - r->t_admit, &res);
         res.device_seconds = 0;
+        if (r->timed_out) res.status = SIS_ETIMEOUT;
         r->res = res;
         r->t_done = t;
-        const int n = r->spec.n, R = r->spec.p.replicas;
-        sq_alloc.release(r->sq_off, (size_t)n * R);
-        rep_alloc.release(r->rep_off, R);
-        arch_alloc.release(r->arch_off, n);
-        free_slots.push_back(r->slot);
-        resident_bytes -= r->bytes;
+        release_req(r);
         if (cfg.certify)
             cert_queue.push_back(r);
         else
@@ -463,6 +498,44 @@ int Server::finalize_done(FinBlock& fb) {
     fb.busy = false;
     cert_cv.notify_all();
     return SIS_OK;
+}
+
+// Arena ranges, slot and budget of a resident request back (under mu).
+void Server::release_req(Req* r) {
+    const int n = r->spec.n, R = r->spec.p.replicas;
+    sq_alloc.release(r->sq_off, (size_t)n * R);
+    rep_alloc.release(r->rep_off, R);
+    arch_alloc.release(r->arch_off, n);
+    free_slots.push_back(r->slot);
+    resident_bytes -= r->bytes;
+    resident_reqs -= 1;
+    r->slot = -1;
+}
+
+// A round of this lane failed.  Its stream is drained first (launches the
+// round enqueued before the failure still run, on the lane's own arena
+// ranges); if that fails too the device itself is in error and the loop stops
+// (returns false).  Otherwise every request of the lane fails with
+// SIS_EDEVICE, its memory is released, and the other lanes go on.
+bool Server::fail_lane(Lane& ln, const std::string& why) {
+    const hipError_t e = hipStreamSynchronize(ln.stream);
+    if (e != hipSuccess) {
+        loop_err = why + "; then " + hipGetErrorString(e);
+        return false;
+    }
+    (void)hipGetLastError();  // the failed launch's error is not sticky
+    std::lock_guard<std::mutex> lk(mu);
+    for (Req* r : ln.active) {
+        release_req(r);
+        r->fail_code = SIS_EDEVICE;
+        r->fail_msg = "device round failed: " + why;
+        r->t_done = now_s();
+        r->done = true;
+    }
+    ln.active.clear();
+    ln.in_flight = false;
+    ln.dirty = true;
+    return true;
 }
 
 bool Server::work_left() {
@@ -506,8 +579,12 @@ void Server::run() {
                 const hipError_t q = hipEventQuery(ln.done_ev);
                 if (q == hipErrorNotReady) continue;
                 if (q != hipSuccess) {
-                    die(std::string("device round failed: ") + hipGetErrorString(q));
-                    return;
+                    if (!fail_lane(ln, hipGetErrorString(q))) {
+                        die(std::string("device round failed: ") + loop_err);
+                        return;
+                    }
+                    progressed = true;
+                    continue;
                 }
                 if (round_done(ln) != SIS_OK) {
                     die(loop_err);
@@ -521,6 +598,20 @@ void Server::run() {
             {
                 std::lock_guard<std::mutex> lk(mu);
                 std::deque<Req*>& qu = queue[li];
+                // requests whose deadline passed while queued are never admitted
+                const double tq = now_s();
+                for (auto it = qu.begin(); it != qu.end();) {
+                    Req* r = *it;
+                    if (r->deadline > 0 && tq >= r->deadline) {
+                        r->fail_code = SIS_ETIMEOUT;
+                        r->fail_msg = "deadline passed before admission";
+                        r->t_done = tq;
+                        r->done = true;
+                        it = qu.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
                 while (!qu.empty() && !free_slots.empty()) {
                     Req* r = qu.front();
                     const double budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : 1e300;
@@ -536,21 +627,29 @@ void Server::run() {
                     free_slots.pop_back();
                     r->t_admit = now_s();
                     resident_bytes += r->bytes;
+                    resident_reqs += 1;
+                    peak_resident = std::max(peak_resident, resident_bytes);
                     admitted.push_back(r);
                 }
             }
-            for (Req* r : admitted) {
+            // every admitted request joins the lane before its state is
+            // copied in, so that a failed copy fails all of them
+            for (Req* r : admitted) ln.active.push_back(r);
+            if (!admitted.empty()) ln.dirty = true;
+            for (Req* r : admitted)
                 if (admit_one(r, ln) != SIS_OK) {
-                    die(loop_err);
-                    return;
+                    if (!fail_lane(ln, loop_err)) {
+                        die(loop_err);
+                        return;
+                    }
+                    break;
                 }
-                ln.active.push_back(r);
-                ln.dirty = true;
-            }
             if (!ln.active.empty()) {
                 if (launch_round(ln) != SIS_OK) {
-                    die(loop_err);
-                    return;
+                    if (!fail_lane(ln, loop_err)) {
+                        die(loop_err);
+                        return;
+                    }
                 }
                 progressed = true;
             }
@@ -605,7 +704,8 @@ extern "C" {
 
 sis_server* sis_server_create(const sis_server_cfg* cfg) {
     if (!cfg || cfg->max_instances < 1 || cfg->max_instances > 100000 || cfg->intervals_per_round < 1 ||
-        cfg->certifier_threads < 0 || cfg->certifier_threads > 256) {
+        cfg->certifier_threads < 0 || cfg->certifier_threads > 256 || !(cfg->hbm_budget_bytes >= 0) ||
+        !(cfg->request_timeout_s >= 0 && cfg->request_timeout_s < 1e9)) {
         fail(SIS_EINVAL, "bad server configuration");
         return nullptr;
     }
@@ -624,7 +724,14 @@ sis_server* sis_server_create(const sis_server_cfg* cfg) {
 }
 
 int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params, uint64_t* ticket) {
+    if (!h) return fail(SIS_EINVAL, "null server");
+    return sis_submit_timeout(h, inst, params, h->s.cfg.request_timeout_s, ticket);
+}
+
+int sis_submit_timeout(sis_server* h, const sis_instance* inst, const sis_params* params, double timeout_s,
+                       uint64_t* ticket) {
     if (!h || !ticket) return fail(SIS_EINVAL, "null server or ticket");
+    if (!(timeout_s >= 0 && timeout_s < 1e9)) return fail(SIS_EINVAL, "timeout_s must be in [0, 1e9)");
     int rc = validate(inst, params);
     if (rc) return rc;
     auto* r = new Req;
@@ -640,6 +747,7 @@ int sis_submit(sis_server* h, const sis_instance* inst, const sis_params* params
     r->bytes = request_bytes(inst->n, params->replicas);
     initial_state(r->spec, r->init);  // in the caller's thread, off the device loop
     r->t_submit = now_s();
+    r->deadline = timeout_s > 0 ? r->t_submit + timeout_s : 0.0;
     {
         std::lock_guard<std::mutex> lk(h->s.mu);
         if (h->s.stop) {
@@ -667,11 +775,33 @@ int sis_poll(sis_server* h, uint64_t ticket, sis_result* out) {
         if (h->s.stop && !h->s.error.empty()) return fail(SIS_EDEVICE, h->s.error);
         return SIS_PENDING;
     }
+    if (r->fail_code) {
+        out->status = r->fail_code;
+        return fail(r->fail_code, r->fail_msg);
+    }
     double* q = out->q_out;
     *out = r->res;
     out->q_out = q;
     if (q) std::copy(r->q.begin(), r->q.end(), q);
     if (h->s.cfg.certify && !r->certified) out->status = SIS_EINVAL;
+    return SIS_OK;
+}
+
+int sis_server_stats(sis_server* h, double* arena_bytes, double* resident_bytes, double* peak_resident_bytes,
+                     int* resident_requests) {
+    if (!h) return fail(SIS_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(h->s.mu);
+    if (arena_bytes) *arena_bytes = (double)h->s.arena_bytes;
+    if (resident_bytes) *resident_bytes = (double)h->s.resident_bytes;
+    if (peak_resident_bytes) *peak_resident_bytes = (double)h->s.peak_resident;
+    if (resident_requests) *resident_requests = h->s.resident_reqs;
+    return SIS_OK;
+}
+
+int sis_server_inject_lane_fault(sis_server* h, int lane) {
+    if (!h || lane < 0 || lane >= kLanes) return fail(SIS_EINVAL, "bad server or lane");
+    std::lock_guard<std::mutex> lk(h->s.mu);
+    h->s.fault_lane = lane;
     return SIS_OK;
 }
 

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libsisamd.so")
 PACKING_BIN = os.path.join(HERE, "bin", "packing")
 
-SIS_OK, SIS_PENDING, SIS_EINVAL, SIS_EDEVICE = 0, 1, 2, 3
+SIS_OK, SIS_PENDING, SIS_EINVAL, SIS_EDEVICE, SIS_ETIMEOUT = 0, 1, 2, 3, 4
 # sis_params.flags: count every proposal's near pairs (a diagnostic, about 3 %
 # of sweep throughput; without it Result.near_pairs is 0)
 FLAG_NEAR_PAIRS = 1
@@ -76,7 +76,7 @@ class SisServerCfg(ctypes.Structure):
     _fields_ = [
         ("hbm_budget_bytes", ctypes.c_double), ("max_instances", ctypes.c_int),
         ("intervals_per_round", ctypes.c_int), ("certifier_threads", ctypes.c_int),
-        ("certify", ctypes.c_int),
+        ("certify", ctypes.c_int), ("request_timeout_s", ctypes.c_double),
     ]
 
 
@@ -144,6 +144,10 @@ def lib() -> ctypes.CDLL:
     L.sis_server_create.argtypes = [ctypes.POINTER(SisServerCfg)]
     L.sis_server_create.restype = ctypes.c_void_p
     L.sis_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams), u64p]
+    L.sis_submit_timeout.argtypes = [ctypes.c_void_p, ctypes.POINTER(SisInstance), ctypes.POINTER(SisParams),
+                                     ctypes.c_double, u64p]
+    L.sis_server_stats.argtypes = [ctypes.c_void_p, dp, dp, dp, ip]
+    L.sis_server_inject_lane_fault.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.sis_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(SisResult)]
     L.sis_ticket_times.argtypes = [ctypes.c_void_p, ctypes.c_uint64, dp, dp, dp, ctypes.c_char_p, ctypes.c_int]
     L.sis_ticket_release.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
@@ -399,18 +403,25 @@ class Server:
     """Continuous-batching request server (sis_server_*)."""
 
     def __init__(self, max_instances=256, intervals_per_round=10, certifier_threads=8, certify=True,
-                 hbm_budget_bytes=200e9):
-        cfg = SisServerCfg(hbm_budget_bytes, max_instances, intervals_per_round, certifier_threads, int(certify))
+                 hbm_budget_bytes=4 * 2**30, request_timeout_s=0.0):
+        cfg = SisServerCfg(hbm_budget_bytes, max_instances, intervals_per_round, certifier_threads, int(certify),
+                           request_timeout_s)
         self.h = lib().sis_server_create(ctypes.byref(cfg))
         if not self.h:
             _err(lib().sis_last_error_code())
         self.ns = {}
 
-    def submit(self, n, L, q=None, params: SisParams | None = None, **kw) -> int:
+    def submit(self, n, L, q=None, params: SisParams | None = None, timeout_s=None, **kw) -> int:
+        """Ticket of a new request; timeout_s: its own deadline in seconds
+        (0: none) instead of the server's request_timeout_s."""
         p = params if params is not None else default_params(**kw)
         inst, keep = _instance(n, L, q)
         t = ctypes.c_uint64()
-        rc = lib().sis_submit(self.h, ctypes.byref(inst), ctypes.byref(p), ctypes.byref(t))
+        if timeout_s is None:
+            rc = lib().sis_submit(self.h, ctypes.byref(inst), ctypes.byref(p), ctypes.byref(t))
+        else:
+            rc = lib().sis_submit_timeout(self.h, ctypes.byref(inst), ctypes.byref(p), float(timeout_s),
+                                          ctypes.byref(t))
         if rc:
             _err(rc)
         del keep
@@ -429,6 +440,22 @@ class Server:
         if rc:
             _err(rc)
         return _result(r, out)
+
+    def stats(self) -> dict:
+        """Arena capacity and resident replica state (bytes), resident requests."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        k = ctypes.c_int()
+        rc = lib().sis_server_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(k))
+        if rc:
+            _err(rc)
+        return {"arena_bytes": a.value, "resident_bytes": b.value, "peak_resident_bytes": c.value,
+                "resident_requests": k.value}
+
+    def inject_lane_fault(self, lane: int):
+        """Test hook: the next round of `lane` fails at its first launch."""
+        rc = lib().sis_server_inject_lane_fault(self.h, lane)
+        if rc:
+            _err(rc)
 
     def times(self, ticket: int):
         """(queued_s, device_done_s, certified_s, L50 or rejection reason)."""

@@ -239,6 +239,10 @@ def test_handle_constructors_report_the_failure_code(sis):
     with pytest.raises(sis.SisError) as e:
         sis.Server(intervals_per_round=0)
     assert e.value.code == sis.SIS_EINVAL
+    for bad in (dict(request_timeout_s=-1.0), dict(request_timeout_s=float("nan")), dict(hbm_budget_bytes=-1.0)):
+        with pytest.raises(sis.SisError) as e:
+            sis.Server(**bad)
+        assert e.value.code == sis.SIS_EINVAL, bad
     p = sis.default_params(replicas=8)
     with pytest.raises(sis.SisError) as e:
         sis.BatchSession([(0, 5.0, None)], p)
