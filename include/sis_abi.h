@@ -110,7 +110,7 @@ typedef struct sis_result {
                                    launch it came from (0 for a seed that was never improved on) */
     double e64_before;          /* FP64 energy of the same configuration before polish */
     double e64_after;           /* FP64 energy after polish */
-    double e_gap;               /* |e32 - e64_before| */
+    double e_gap;               /* |e32 - e64_before|; 0 for a seed that was never improved on */
     uint64_t proposals;         /* attempted single-square moves */
     uint64_t accepted;
     uint64_t sweeps;            /* sweeps per replica completed */

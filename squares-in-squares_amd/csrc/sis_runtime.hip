@@ -733,7 +733,10 @@ void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis
     out->e32 = r.e32;
     out->e64_before = r.e64_before;
     out->e64_after = r.e64_after;
-    out->e_gap = std::fabs(r.e32 - r.e64_before);
+    // the caller's seed exported as it was (nothing improved on it) was never
+    // evaluated by a sweep: no FP32 energy, no gap
+    const bool seed_export = D.improve && D.seeded && !D.arch_valid;
+    out->e_gap = seed_export ? 0.0 : std::fabs(r.e32 - r.e64_before);
     out->proposals = D.proposals;
     out->accepted = D.accepted;
     out->sweeps = (uint64_t)D.sweeps_done;

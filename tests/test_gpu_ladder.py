@@ -278,5 +278,7 @@ def test_seed_kept_when_nothing_improves(sis, ref_certifier):
                       tmin=1.0, tmax=1.0, shrink=1e-3, polish_iters=20, rng_seed=3)
         assert r.feasible == 0 and r.target_L == 2 * f
         assert r.L_cert <= 2 * f + 1e-12
+        # no sweep evaluated the exported seed: no FP32 energy and no gap
+        assert r.e32 == 0 and r.e_gap == 0 and r.e64_before >= 0
         dil, L50 = certify_q(ref_certifier, r.q)
         assert dil is not None and float(L50) <= 2 * f + 1e-6
