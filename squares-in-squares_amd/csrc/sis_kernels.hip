@@ -1042,11 +1042,16 @@ __device__ void collective_step(const TileLds& t, int n, int G, int M, int m, fl
 // +-0.05 jitter and a uniform angle; then every square is jittered, squares
 // with energy above 1e-6 by +-0.08 / +-0.18 rad, the rest by +-0.008 /
 // +-0.015 rad, centres clamped to [0.48, L - 0.48].  t.eout holds the
-// canonical energy on entry and on exit.
+// canonical energy on entry and on exit.  With skip_certified, replicas that
+// the certified-side pass just certified (t.maxd marked) keep their state:
+// the exchange step may archive it as the best configuration.
+constexpr unsigned kCertifiedMark = 0xFFFFFFFFu;
 template <int NT>
 __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, int r0,
-                     uint32_t interval, bool relocate, uint32_t k0, uint32_t k1, unsigned& kicked) {
+                     uint32_t interval, bool relocate, uint32_t k0, uint32_t k1, unsigned& kicked,
+                     bool skip_certified) {
     const int tid = threadIdx.x, gw = grid_words(M);
+    auto live = [&](int g) { return t.eout[g] > 1e-12f && !(skip_certified && t.maxd[g] == kCertifiedMark); };
     float* sq = reinterpret_cast<float*>(t.rank);  // rank + items: 4 bytes per square
     for (int g = tid; g < G; g += NT) {
         t.gmask[g] = 0ull;
@@ -1057,7 +1062,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
     bin_tile<NT>(t, n, G, M, m, inv_w);  // cell lists of the current positions
     for (int q = tid; q < G * n; q += NT) {
         const int g = rep_of(t, q), i = q - g * n;
-        if (!(t.eout[g] > 1e-12f)) continue;
+        if (!live(g)) continue;
         const float4 p = t.pos[q];
         const int ck = t.cellk[q];
         const Stencil sc(t.A + g * gw, M, ck & 255, ck >> 8);
@@ -1078,9 +1083,9 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
         const unsigned long long k = t.gmask[g];
         w.worst = (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFull));
         w.eold = __uint_as_float((unsigned)(k >> 32));
-        w.accept = relocate && t.eout[g] > 1e-12f && w.eold > 1e-5f;  // relocate this replica
+        w.accept = relocate && live(g) && w.eold > 1e-5f;  // relocate this replica
         t.gmask[g] = 0ull;
-        if (t.eout[g] > 1e-12f) kicked += 1;
+        if (live(g)) kicked += 1;
     }
     tile_sync<NT>();
     if (relocate) {
@@ -1120,7 +1125,7 @@ __device__ void kick(const TileLds& t, int n, int G, int M, int m, float inv_w, 
     }
     for (int q = tid; q < G * n; q += NT) {
         const int g = rep_of(t, q), i = q - g * n;
-        if (!(t.eout[g] > 1e-12f)) continue;
+        if (!live(g)) continue;
         const bool hot = sq[q] > 1e-6f;
         const float axy = hot ? 0.08f : 0.008f, aa = hot ? 0.18f : 0.015f;
         const uint4 o = philox((uint32_t)i, (uint32_t)(r0 + g), interval, 8u, k0, k1);
@@ -1276,6 +1281,7 @@ __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs
         r->minx = ok ? dval(a[1]) : 0.0;
         r->miny = ok ? dval(a[3]) : 0.0;
         certs += ok ? 1u : 0u;
+        if (ok) t.maxd[g] = kCertifiedMark;  // the kick after this pass leaves it alone
     }
     tile_sync<NT>();
 }
@@ -1605,18 +1611,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int c = 0; c < ncoll; ++c)
         collective_step<NT>(t, n, G, M, m, inv_w, tl.r0, sweep0 + (uint32_t)nsw, (uint32_t)c, k0, k1, gstate, ctried,
                             caccd);
-    const bool kick_now = kick_every > 0 && (interval + 1u) % (uint32_t)kick_every == 0u;
-    if (kick_now)
-        kick<NT>(t, n, G, M, m, inv_w, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u, k0, k1,
-                 kicked);
+    // The certified sides come before the kick, as the reference checkpoints
+    // each chain before kicking it (cuda/packing.cu:689-745); a replica they
+    // certify is not kicked, so that its state is the configuration they
+    // certified.
     if (certify_now) {
-        // the cell lists and largest overlaps must be those of the final
-        // positions: collective moves and kicks change them after the first
-        // energy pass (the canonical energy it writes again is the same)
-        if (ncoll > 0 || kick_now) tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, true);
+        // the cell lists and largest overlaps must be those of the positions
+        // after the collective moves (the canonical energy written again is
+        // the same)
+        if (ncoll > 0) tile_energy<NT>(t, n, G, M, m, inv_w, t.eout, true);
         unsigned certs = 0;
         tile_certify<NT>(t, n, G, M, reps + rep_base, certs);
     }
+    const bool kick_now = kick_every > 0 && (interval + 1u) % (uint32_t)kick_every == 0u;
+    if (kick_now)
+        kick<NT>(t, n, G, M, m, inv_w, tl.r0, interval, (((interval + 1u) / (uint32_t)kick_every) & 1u) != 0u, k0, k1,
+                 kicked, certify_now);
     for (int q = tid; q < G * n; q += NT) gstate[q] = t.pos[q];
     for (int g = tid; g < G; g += NT) {
         RepState* rs = reps + rep_base + g;

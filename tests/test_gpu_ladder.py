@@ -175,6 +175,54 @@ def test_launch_certified_side_matches_oracle(sis, oracle, n, L, R, group):
     assert certified >= 1 and (R == 1 or dreps["Lc"][R - 1] == 1e9)
 
 
+@pytest.mark.parametrize("n,L,R,group", [(50, 8.3, 6, 3), (150, 13.5, 4, 1)])
+def test_launch_certifies_before_the_kick(sis, oracle, n, L, R, group):
+    """A launch with collective moves, a kick and certified sides, in the
+    reference's order (cuda/packing.cu:689-745: checkpoint, then kick): the
+    certified sides are those of the state after the collective moves, a
+    replica they certify is not kicked (its state stays the certified one),
+    the others are kicked; device == oracle model bit for bit."""
+    rng = np.random.default_rng(n + 1)
+    k = int(math.isqrt(n - 1)) + 1
+    pitch = (L - 0.1) / k
+    i = np.arange(n)
+    st = np.zeros((R, n, 4), np.float32)
+    jit = np.full((R, 1), 0.004)
+    st[:, :, 0] = (i % k + 0.5) * pitch + rng.uniform(-1, 1, (R, n)) * jit
+    st[:, :, 1] = (i // k + 0.5) * pitch + rng.uniform(-1, 1, (R, n)) * jit
+    st[R // 2:, :, :2] *= 0.93  # overlapping beyond a certifiable depth: kicked
+    a = rng.uniform(-1, 1, (R, n)) * jit
+    st[:, :, 2], st[:, :, 3] = np.cos(a), np.sin(a)
+    box = np.full(R, np.float32(L))
+    st[:, :, :2] = np.clip(st[:, :, :2], 0, L)
+    T = np.geomspace(1e-6, 1e-5, R)
+    ik = (LOG2E / T).astype(np.float32)
+    sxy = np.full(R, 0.004, np.float32)
+    sth = np.full(R, 0.004, np.float32)
+    say("certify before kick", n, R, group)
+    dst, dacc, den, dcnt, dreps = sis.dev_launch(st, L, box, 0.7075, ik, sxy, sth, 9, 2, 40, 6, group=group,
+                                                 collective=2, kick_every=1, interval=0, certify=True)
+    kicked = certified = 0
+    for r in range(R):
+        x, y, c, s = (np.ascontiguousarray(st[r, :, j]) for j in range(4))
+        for sw in range(40, 46):
+            oracle.sweep_model(x, y, c, s, L, 0.7075, sxy[r], sth[r], ik[r], 9, 2, r, sw, 4)
+        _, e = oracle.collective_model(x, y, c, s, L, 0.7075, sxy[r], sth[r], ik[r], 9, 2, r, 46, 2)
+        side, lam, mx, my = oracle.certify_xycs(np.stack([x, y, c, s], -1))
+        before = np.stack([x, y, c, s], -1).copy()
+        if side < 1e8:
+            certified += 1
+            assert dreps["Lc"][r] == pytest.approx(side, rel=1e-14)
+            assert np.array_equal(dst[r], before), r  # certified: not kicked
+        else:
+            assert dreps["Lc"][r] == 1e9
+            kk, e = oracle.kick_model(x, y, c, s, L, 0.7075, 9, 2, r, 0, True)
+            kicked += kk
+        assert np.array_equal(dst[r], np.stack([x, y, c, s], -1)), r
+        assert den[r] == e, r
+    assert dcnt[5] == kicked and certified >= 1 and kicked >= 1
+
+
 def _model_interval(oracle, s, w, spsh, certify_every):
     """One interval of the oracle: staging (scale, clamp to the slot box),
     sweeps, canonical energy, certified side (improve = 1, every
