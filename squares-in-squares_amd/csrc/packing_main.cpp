@@ -8,7 +8,7 @@
 // cuda/seeded.cu:832-969, SEED_* knobs, a fixed coldest temperature 1e-11,
 // switchable collective moves, polish and structured starts).  The search
 // itself runs through the C ABI of libsisamd (colour-checkerboard FP32
-// sweeps, collective moves and kicks, replica exchange, FP64 polish and
+// sweeps, collective moves, replica exchange, FP64 polish and
 // certified side on the GPU).
 //
 // Mapping of the reference arguments onto this engine:
@@ -24,7 +24,8 @@
 //           raises it)
 // The search follows the reference's improvement path (best certified side,
 // per-chain targets best - shrink * mults[b % 8], ladder heads restarted from
-// the best configuration) and its 200-epoch reheating cycle.
+// the best configuration) and its 200-epoch reheating cycle, with collective
+// moves and without the per-epoch kick (below).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -144,10 +145,16 @@ int main(int argc, char** argv) {
     p.sweeps_per_exchange = std::max(1, std::min(epoch_sweeps, 10));
     const int epoch_intervals = (epoch_sweeps + p.sweeps_per_exchange - 1) / p.sweeps_per_exchange;
     // the reference's epoch mixes collective moves into its steps
-    // (cuda/packing.cu:564-616) and, when it polishes, kicks once per epoch
-    // (:696-745)
+    // (cuda/packing.cu:564-616).  Its kick (:696-745) is the perturbation of
+    // a basin-hopping loop whose every epoch ends with 200 FP64 CG
+    // iterations per chain (:624-687) before the checkpoint; this engine
+    // polishes once, at the end, and kicking every epoch without that
+    // relaxation keeps the chains from settling (the demo seeds at the
+    // reference's argv: n = 11 reaches 3.955 without kicks and keeps 4.000
+    // with them; at 20 000 epochs n = 11 3.8771 vs 4.0000 and n = 50 7.6215
+    // vs 7.7071, the others equal), so the CLIs do not kick.
     p.collective_moves = collective ? 4 : 0;
-    p.kick_every = polish > 0 ? epoch_intervals : 0;
+    p.kick_every = 0;
     // improvement path (cuda/packing.cu:503-504, 831-851) and the reheating
     // cycle of 200 epochs (:516)
     p.improve = 1;

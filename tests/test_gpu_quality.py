@@ -6,10 +6,12 @@ Every output passes the reference's interval certifier (oracle/_ref).
 
 Margins: n = 5, 10, 17 and 50 reach or beat the recorded target_L (the
 engine starts the chains from Goebel-strip and tilted-grid topologies, the
-reference's portfolio, and polishes in FP64); n = 11 with the reference's
-argv runs ceil(32 / 11) = 3 sweeps in each of 80 epochs and keeps the seed,
-while `steps` counted as sweeps (PACK_SWEEP_STEPS=1) reaches below 3.93
-(recorded: 3.890)."""
+reference's portfolio, and polishes in FP64).  The recorded outputs come from
+an earlier build that ran all 20 000 epochs (chain_steps 163 840 000 =
+256 x 20 000 x 32); the reference's source caps them at 80
+(cuda/packing.cu:802-803), as this CLI does, so n = 11 gets ceil(32 / 11) =
+3 sweeps in each of 80 epochs and reaches below 3.96 (recorded: 3.890), and
+with `steps` counted as sweeps (PACK_SWEEP_STEPS=1) below 3.93."""
 from __future__ import annotations
 
 import json
@@ -55,7 +57,7 @@ def _run(tmp_path, seed, n, env_extra=None):
     return json.loads(out.read_text())
 
 
-@pytest.mark.parametrize("n,margin", [(5, 0.0), (10, 0.0), (17, 0.0), (50, 0.0), (11, None)])
+@pytest.mark.parametrize("n,margin", [(5, 0.0), (10, 0.0), (17, 0.0), (50, 0.0), (11, 3.96 - 3.890074968)])
 def test_packing_on_the_reference_demo_seeds(tmp_path, ref_certifier, n, margin):
     from refcert import certify_q
     seed, seed_L = _demo_seed(tmp_path, ref_certifier, n)
