@@ -216,7 +216,7 @@ def precision_leg(sis, sweeps: int = 10000):
                     "L_cert": r.L_cert, "certified_slack": r.L_cert - r.target_L,
                     "L_cert_vs_L0": r.L_cert / side - 1.0,
                     "note": "the box is the grid's own side: the unjittered grid fills it exactly, and a row of "
-                            "squares tilted by 0.01 rad is 1 % wider than the box, so zero overlap at L0 needs a "
+                            "squares tilted by 0.01 rad is 1 %% wider than the box, so zero overlap at L0 needs a "
                             "global rearrangement into the %d free cells; the certified side of the unjittered "
                             "grid is %g" % (side * side - n, grid_L),
                     "lambda": r.fields["lambda"],
