@@ -44,6 +44,28 @@ def c2_params(sis, sweeps_total):
                               sweeps_per_exchange=SPE, polish_iters=200, rng_seed=7, init_mode=1, improve=1)
 
 
+def full_length_leg(sis, sweeps: int):
+    """The plain workload run to its full length (1e6 sweeps: the timed
+    steps are a window of it) in the same plain form, with the polish and the
+    export: certified L, the wall time and the rate over the whole run."""
+    import numpy as np
+
+    sess = sis.BatchSession([(N, L0, None)], c2_params(sis, sweeps))
+    t0 = time.perf_counter()
+    sweep_ms, _, _ = sess.run(0, mode=0)
+    wall = time.perf_counter() - t0
+    r = sess.finalize()[0]
+    sess.close()
+    try:
+        dil, Ls = sis.export_certify(np.asarray(r.q))
+    except Exception as e:  # reported, not hidden
+        dil, Ls = None, f"not certified: {e}"
+    return {"sweeps": r.sweeps, "proposals": r.proposals, "seconds": wall, "proposals_per_s": r.proposals / wall,
+            "sweep_kernel_ms": sweep_ms, "feasible": r.feasible, "shrinks": r.shrinks, "target_L": r.target_L,
+            "L_cert": r.L_cert, "certified_L": Ls, "dilation": dil, "e32_vs_e64_gap": r.e_gap,
+            "feasible_checkpoints": r.feasible_checkpoints, "grid_L": float(math.ceil(math.sqrt(N)))}
+
+
 def host_cpu():
     """Host cores visible to this process and the CPU model."""
     model = "unknown"
@@ -232,7 +254,8 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
     each): all instances in one HIP-graph schedule vs the plain form (one
     instance per launch sequence, host sync per interval) on a sample of the
     same stream.  Returns proposals/s of both and the sweep-kernel roofline of
-    the batched form."""
+    the batched form; with the autotuner, also the rate and search quality of
+    the whole stream under the untuned settings."""
     import numpy as np
 
     rng = np.random.default_rng(3)
@@ -241,24 +264,44 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
     params = [sis.default_params(tmin=1e-5, tmax=5e-2, replicas=64, sweeps=sweeps, sweeps_per_exchange=SPE,
                                  rng_seed=1000 + k, init_mode=1, polish_iters=200, shrink=1e-4, improve=1)
               for k in range(n_inst)]
-    tune = None
+
+    def stream(ps):
+        spe = ps[0].sweeps_per_exchange
+        sess = sis.BatchSession(insts, ps)
+        warm = max(1, 50 // spe)
+        sess.run(warm, mode=1)
+        t0 = time.perf_counter()
+        _, _, dev_ms = sess.run(0, mode=1)
+        wall = time.perf_counter() - t0
+        res = sess.finalize()
+        sess.close()
+        return res, wall, dev_ms, sum(n * 64 * (sweeps - warm * spe) for n, _, _ in insts)
+
+    def reduction(res):
+        gain = [1.0 - r.L_cert / L for r, (_, L, _) in zip(res, insts) if r.L_cert < 1e8]
+        return float(np.mean(gain)) if gain else None
+
+    tune, untuned = None, None
     if autotune:
         # autotuner on a 64-instance slice of the stream, 20 probe sweeps per candidate
         tune = sis.autotune(insts[:64], params[:64], probe_sweeps=20)
+        # the untuned settings on the whole stream, for the tuner's speed and
+        # search quality at full length (the timed run itself when the tuner
+        # kept them)
+        own = (params[0].replicas_per_group, params[0].sweeps_per_exchange, params[0].cell)
+        if own == (tune["replicas_per_group"], tune["sweeps_per_exchange"], tune["cell"]):
+            untuned = {"same_settings_as_tuned": True}
+        else:
+            u_res, u_wall, _, u_timed = stream(params)
+            untuned = {"replicas_per_group": own[0], "sweeps_per_exchange": own[1], "cell": own[2],
+                       "proposals_per_s": u_timed / u_wall, "feasible_instances": sum(r.feasible for r in u_res),
+                       "mean_relative_side_reduction": reduction(u_res)}
         for p in params:
             p.replicas_per_group = tune["replicas_per_group"]
             p.sweeps_per_exchange = tune["sweeps_per_exchange"]
             p.cell = tune["cell"]
     spe = params[0].sweeps_per_exchange
-    sess = sis.BatchSession(insts, params)
-    warm = max(1, 50 // spe)
-    sess.run(warm, mode=1)
-    t0 = time.perf_counter()
-    _, _, dev_ms = sess.run(0, mode=1)
-    wall = time.perf_counter() - t0
-    res = sess.finalize()
-    sess.close()
-    timed = sum(n * 64 * (sweeps - warm * spe) for n, _, _ in insts)
+    res, wall, dev_ms, timed = stream(params)
     batched = timed / wall
     props = sum(r.proposals for r in res)
     c_per = sum(r.candidates for r in res) / props
@@ -279,16 +322,15 @@ def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool
         plain_L.append(pr.L_cert)
         same += bool(np.array_equal(pr.q, res[k].q) and pr.L_cert == res[k].L_cert)
     plain = plain_props / plain_wall if plain_wall > 0 else None
-    gain = [1.0 - r.L_cert / L for r, (_, L, _) in zip(res, insts) if r.L_cert < 1e8]
     return {
         "workload": f"{n_inst} instances, n ~ U{{20..200}} (seed 3), 64 replicas each, {sweeps} sweeps each, "
-                    "exchange every 10 sweeps (autotuned below), random starts in a box of side ceil(sqrt n) + 1, "
+                    f"exchange every {spe} sweeps (the autotuner's choice below; {SPE} untuned), random starts in a box of side ceil(sqrt n) + 1, "
                     "the reference's improvement path; batched = one HIP-graph schedule over all instances",
         "batched_proposals_per_s": batched, "plain_proposals_per_s": plain,
         "ratio": batched / plain if plain else None,
         "plain_sample_instances": plain_sample, "timed_proposals": timed, "wall_s": wall,
-        "device_ms": dev_ms, "feasible_instances": feas, "autotuned": tune,
-        "quality": {"instances_improved": feas, "mean_relative_side_reduction": float(np.mean(gain)) if gain else None,
+        "device_ms": dev_ms, "feasible_instances": feas, "autotuned": tune, "untuned": untuned,
+        "quality": {"instances_improved": feas, "mean_relative_side_reduction": reduction(res),
                     "mean_L_cert_sample_batched": float(np.mean([r.L_cert for r in res[:plain_sample]]))
                     if plain_sample else None,
                     "mean_L_cert_sample_plain": float(np.mean(plain_L)) if plain_L else None,
@@ -513,6 +555,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the n=500/1000 precision leg")
     ap.add_argument("--precision-sweeps", type=int, default=10000)
     ap.add_argument("--no-reference-seeds", action="store_true", help="skip the demo-seed quality leg")
+    ap.add_argument("--full-sweeps", type=int, default=1000000,
+                    help="sweeps of the untimed full-length run of the plain workload (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (requests shard across GPUs "
@@ -607,6 +651,8 @@ def main():
             "exchange_rate": res.exchanges_accepted / max(1, res.exchanges_tried),
         },
     }
+    if a.full_sweeps > 0:
+        line["full_length"] = full_length_leg(sis, a.full_sweeps)
     if not a.no_batched:
         line["batched"] = batched_leg(sis, a.batched_instances, a.batched_sweeps, a.plain_sample)
     packings = []

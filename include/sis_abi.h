@@ -218,18 +218,40 @@ int sis_batch_destroy(sis_batch* b);
 
 /* Autotuning (no reference counterpart: the reference picks chains from an
  * occupancy query, cuda/packing.cu:788-799, and its autotune.py runs a fixed
- * launch matrix, cuda/autotune.py:8-20).  Runs `probe_sweeps` sweeps of the
+ * launch matrix, cuda/autotune.py:8-20, ranked by polished L first and
+ * chain-steps/s second, :37; here speed ranks, guarded by a quality
+ * probe where the trajectory changes).  Runs `probe_sweeps` sweeps of the
  * given batch for every candidate of replicas per workgroup (0 = the automatic
  * size of each instance by its n and the load, 1, 2, 4, 8), sweeps per
- * exchange (5, 10, 20) and cell width (0.7075, 0.8) and reports the fastest by
- * HIP-event time.  Replicas per workgroup never change results; the other two
- * change the trajectory. */
+ * exchange (5, 10, 20) and cell width (0.7075, 0.8), by HIP-event time.
+ * Replicas per workgroup never change results; the other two change the
+ * trajectory, so a faster candidate that changes them is put through a
+ * quality probe first: the batch runs 50 x probe_sweeps sweeps (polish off)
+ * under the candidate and under the caller's own settings, to the same sweep
+ * of every instance, and the candidate is taken when the mean paired loss of
+ * relative side reduction 1 - min(L_cert, L) / L plus two standard errors is
+ * at most 0.001 (non-inferiority).  At most three such candidates are
+ * checked, fastest first; otherwise the fastest candidate with the caller's
+ * sweeps per exchange and cell wins (the caller's own values are added as
+ * candidates when they are not on the grid).  sweeps_per_exchange and cell
+ * report instance 0's values. */
 typedef struct sis_tuning {
     int replicas_per_group;
     int sweeps_per_exchange;
     double cell;
     double proposals_per_s;     /* of the winner during its probe */
     int candidates_tried;
+    int trajectory_kept;        /* 1: the winner keeps the caller's sweeps per exchange and cell */
+    int quality_checked;        /* candidates put through the quality probe */
+    int quality_sweeps;         /* sweeps of the quality probe (0: none ran) */
+    /* The last candidate put through the quality probe (the winner when
+     * trajectory_kept = 0; zeros when none was): its sweeps per exchange and
+     * cell, and its mean paired loss of relative side reduction with the
+     * standard error of that mean. */
+    int checked_sweeps_per_exchange;
+    double checked_cell;
+    double quality_delta;
+    double quality_se;
 } sis_tuning;
 int sis_autotune(const sis_instance* insts, int count, const sis_params* params, int probe_sweeps,
                  sis_tuning* out);

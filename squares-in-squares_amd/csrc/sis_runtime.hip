@@ -863,6 +863,86 @@ int sis_solve_batch(const sis_instance* insts, int count, const sis_params* para
     return SIS_OK;
 }
 
+namespace {
+
+// One autotuner candidate: replicas per workgroup, and the two settings that
+// change the trajectory (0 / 0.0: each instance keeps its own).
+struct TuneCand {
+    int G, spe;
+    double cell;
+    double rate = 0;
+};
+
+// The caller's own value of a setting for instance k, or the candidate's.
+inline int cand_spe(const TuneCand& c, const sis_params& p) { return c.spe ? c.spe : p.sweeps_per_exchange; }
+inline double cand_cell(const TuneCand& c, const sis_params& p) { return c.cell > 0 ? c.cell : p.cell; }
+
+// Speed probe: `probe_sweeps` sweeps (whole intervals) of every instance under
+// the candidate, rate by HIP-event time of the batched graph.  1: candidate
+// invalid for some instance.
+int tune_speed(const sis_instance* insts, int count, const sis_params* params, int probe_sweeps, TuneCand& c) {
+    std::vector<InstSpec> v;
+    long long props = 0;
+    for (int k = 0; k < count; ++k) {
+        InstSpec s = make_spec(insts + k, params + k);
+        s.p.replicas_per_group = c.G;
+        s.p.sweeps_per_exchange = cand_spe(c, params[k]);
+        s.p.cell = cand_cell(c, params[k]);
+        // whole intervals only, so every candidate does the same work
+        s.p.sweeps = (std::max(probe_sweeps, s.p.sweeps_per_exchange) / s.p.sweeps_per_exchange) *
+                     s.p.sweeps_per_exchange;
+        if (validate(insts + k, &s.p)) return 1;
+        props += (long long)s.n * s.p.replicas * s.p.sweeps;
+        v.push_back(std::move(s));
+    }
+    Batch b;
+    int rc;
+    if ((rc = b.build(std::move(v)))) return -rc;
+    if ((rc = b.run(0, 1))) return -rc;
+    c.rate = props / (b.device_ms * 1e-3);
+    return 0;
+}
+
+long long gcd_ll(long long a, long long b) { return b ? gcd_ll(b, a % b) : a; }
+
+// Largest loss of mean relative side reduction (upper two-standard-error
+// bound) a trajectory-changing candidate may show in the quality probe.
+constexpr double kQualityMargin = 1e-3;
+
+// Quality probe: every instance run for its share of `sweeps` (a multiple of
+// the caller's and the candidate's sweeps per exchange, so both runs end on
+// an interval) with the polish off; per instance the relative side reduction
+// 1 - min(L_cert, L0) / L0 of its certified export.
+int tune_quality(const sis_instance* insts, int count, const sis_params* params, const TuneCand& c, int sweeps,
+                 const TuneCand& ref, std::vector<double>& red) {
+    std::vector<InstSpec> v;
+    for (int k = 0; k < count; ++k) {
+        InstSpec s = make_spec(insts + k, params + k);
+        const long long a = cand_spe(c, params[k]), b = cand_spe(ref, params[k]);
+        const long long l = a / gcd_ll(a, b) * b;
+        s.p.replicas_per_group = 0;
+        s.p.sweeps_per_exchange = (int)a;
+        s.p.cell = cand_cell(c, params[k]);
+        s.p.sweeps = std::max<long long>(l, (sweeps / l) * l);
+        s.p.polish_iters = 0;
+        v.push_back(std::move(s));
+    }
+    Batch b;
+    int rc;
+    if ((rc = b.build(std::move(v)))) return rc;
+    if ((rc = b.run(0, 1))) return rc;
+    std::vector<FinalRecord> rec;
+    if ((rc = b.finalize(rec))) return rc;
+    red.resize(count);
+    for (int k = 0; k < count; ++k) {
+        const double L0 = insts[k].L;
+        red[k] = 1.0 - std::min(rec[k].L_cert, L0) / L0;
+    }
+    return SIS_OK;
+}
+
+}  // namespace
+
 int sis_autotune(const sis_instance* insts, int count, const sis_params* params, int probe_sweeps,
                  sis_tuning* out) {
     if (!insts || !params || !out || count < 1 || probe_sweeps < 1) return fail(SIS_EINVAL, "bad autotune arguments");
@@ -874,43 +954,84 @@ int sis_autotune(const sis_instance* insts, int count, const sis_params* params,
     const int groups[5] = {0, 1, 2, 4, 8};
     const int spes[3] = {5, 10, 20};
     const double cells[2] = {0.7075, 0.8};
-    double best = -1;
-    int tried = 0;
     *out = sis_tuning{};
+    std::vector<TuneCand> cands;
     for (double cell : cells)
         for (int spe : spes)
-            for (int G : groups) {
-                std::vector<InstSpec> v;
-                long long props = 0;
-                for (int k = 0; k < count; ++k) {
-                    InstSpec s = make_spec(insts + k, params + k);
-                    s.p.replicas_per_group = G;
-                    s.p.sweeps_per_exchange = spe;
-                    s.p.cell = cell;
-                    // whole intervals only, so every candidate does the same work
-                    s.p.sweeps = (std::max(probe_sweeps, spe) / spe) * spe;
-                    if (validate(insts + k, &s.p)) goto next;
-                    props += (long long)s.n * s.p.replicas * s.p.sweeps;
-                    v.push_back(std::move(s));
-                }
-                {
-                    Batch b;
-                    if ((rc = b.build(std::move(v)))) return rc;
-                    if ((rc = b.run(0, 1))) return rc;
-                    const double rate = props / (b.device_ms * 1e-3);
-                    ++tried;
-                    if (rate > best) {
-                        best = rate;
-                        out->replicas_per_group = G;
-                        out->sweeps_per_exchange = spe;
-                        out->cell = cell;
-                        out->proposals_per_s = rate;
-                    }
-                }
-            next:;
-            }
-    out->candidates_tried = tried;
-    return tried ? SIS_OK : fail(SIS_EINVAL, "no candidate configuration was valid");
+            for (int G : groups) cands.push_back({G, spe, cell});
+    // a candidate keeps the trajectory when its sweeps per exchange and cell
+    // are every instance's own (tile size never changes results)
+    auto keeps = [&](const TuneCand& c) {
+        for (int k = 0; k < count; ++k)
+            if (cand_spe(c, params[k]) != params[k].sweeps_per_exchange || cand_cell(c, params[k]) != params[k].cell)
+                return false;
+        return true;
+    };
+    bool own_on_grid = false;
+    for (const TuneCand& c : cands) own_on_grid |= keeps(c);
+    if (!own_on_grid)
+        for (int G : groups) cands.push_back({G, 0, 0.0});
+    std::vector<TuneCand> ran;
+    for (TuneCand& c : cands) {
+        rc = tune_speed(insts, count, params, probe_sweeps, c);
+        if (rc < 0) return -rc;
+        if (rc == 0) ran.push_back(c);
+    }
+    if (ran.empty()) return fail(SIS_EINVAL, "no candidate configuration was valid");
+    std::stable_sort(ran.begin(), ran.end(), [](const TuneCand& a, const TuneCand& b) { return a.rate > b.rate; });
+    // Faster settings that change the trajectory are taken only if they search
+    // as well: the quality probe runs the batch for 50 x probe_sweeps sweeps
+    // under the candidate and under the caller's settings, and the candidate
+    // passes when the mean paired loss of relative side reduction, plus two
+    // standard errors, is at most kQualityMargin (a non-inferiority test; for
+    // one instance the loss itself).  At most three are checked, fastest
+    // first; then the fastest candidate that keeps the trajectory wins.
+    const TuneCand own{0, 0, 0.0};
+    const int qsweeps = 50 * probe_sweeps;
+    std::vector<double> base, red;
+    const TuneCand* win = nullptr;
+    const TuneCand* last = nullptr;
+    double delta = 0, se = 0;
+    int checked = 0;
+    for (const TuneCand& c : ran) {
+        if (keeps(c)) {
+            win = &c;
+            break;
+        }
+        if (checked == 3) continue;
+        // both runs stop at the same sweep of every instance
+        if ((rc = tune_quality(insts, count, params, own, qsweeps, c, base))) return rc;
+        if ((rc = tune_quality(insts, count, params, c, qsweeps, own, red))) return rc;
+        ++checked;
+        double m = 0, v = 0;
+        for (int k = 0; k < count; ++k) m += base[k] - red[k];
+        m /= count;
+        for (int k = 0; k < count; ++k) v += (base[k] - red[k] - m) * (base[k] - red[k] - m);
+        const double e = count > 1 ? std::sqrt(v / (count - 1) / count) : 0.0;
+        last = &c;
+        delta = m;
+        se = e;
+        if (m + 2.0 * e <= kQualityMargin) {
+            win = &c;
+            break;
+        }
+    }
+    if (!win) return fail(SIS_EINVAL, "no candidate that keeps the trajectory was valid");
+    out->replicas_per_group = win->G;
+    out->sweeps_per_exchange = cand_spe(*win, params[0]);
+    out->cell = cand_cell(*win, params[0]);
+    out->proposals_per_s = win->rate;
+    out->candidates_tried = (int)ran.size();
+    out->trajectory_kept = keeps(*win) ? 1 : 0;
+    out->quality_checked = checked;
+    out->quality_sweeps = checked ? qsweeps : 0;
+    if (last) {
+        out->checked_sweeps_per_exchange = cand_spe(*last, params[0]);
+        out->checked_cell = cand_cell(*last, params[0]);
+        out->quality_delta = delta;
+        out->quality_se = se;
+    }
+    return SIS_OK;
 }
 
 struct sis_batch {

@@ -307,11 +307,16 @@ def solve_batch(instances, params) -> list[Result]:
 class SisTuning(ctypes.Structure):
     _fields_ = [("replicas_per_group", ctypes.c_int), ("sweeps_per_exchange", ctypes.c_int),
                 ("cell", ctypes.c_double), ("proposals_per_s", ctypes.c_double),
-                ("candidates_tried", ctypes.c_int)]
+                ("candidates_tried", ctypes.c_int), ("trajectory_kept", ctypes.c_int),
+                ("quality_checked", ctypes.c_int), ("quality_sweeps", ctypes.c_int),
+                ("checked_sweeps_per_exchange", ctypes.c_int), ("checked_cell", ctypes.c_double),
+                ("quality_delta", ctypes.c_double), ("quality_se", ctypes.c_double)]
 
 
 def autotune(instances, params, probe_sweeps: int = 20) -> dict:
-    """Fastest (replicas per workgroup, sweeps per exchange, cell width) for a batch."""
+    """Fastest (replicas per workgroup, sweeps per exchange, cell width) for a
+    batch whose search quality, where the trajectory changes, passes the
+    paired quality probe against the caller's own settings."""
     k = len(instances)
     if isinstance(params, SisParams):
         params = [params] * k

@@ -495,19 +495,53 @@ def test_near_pair_counting_changes_nothing_else(sis):
 
 def test_autotune_returns_a_candidate_that_runs(sis):
     """The autotuner tries every (tile size, sweeps per exchange, cell width)
-    candidate on the batch and returns the fastest; its choice is one of the
-    candidates and a batch run with it proposes the requested sweeps."""
+    candidate on the batch; its choice is one of the candidates and a batch
+    run with it proposes the requested sweeps.  A choice that changes the
+    trajectory passed the quality probe: its reported mean paired loss of
+    relative side reduction is what two runs of the batch (the candidate and
+    the caller's settings, polish off, to the same sweep) give, and with two
+    standard errors added is at most 0.001; a kept trajectory after a check
+    means the last candidate checked failed that bound."""
+    import math
+
+    import numpy as np
+
     say("autotune")
-    inst = [(30, 7.0, None), (120, 13.0, None), (60, 9.5, None)]
-    p = sis.default_params(replicas=16, sweeps=100, init_mode=1, tmin=1e-4, tmax=3e-2)
+    inst = [(30, 7.0, None), (120, 13.0, None), (60, 9.5, None), (45, 8.0, None), (80, 10.0, None)]
+    kw = dict(replicas=16, sweeps=100, init_mode=1, tmin=1e-4, tmax=3e-2, improve=1, shrink=1e-3)
+    p = sis.default_params(**kw)
     t = sis.autotune(inst, p, probe_sweeps=20)
+    say("autotune", t)
     assert t["candidates_tried"] == 30
     assert t["replicas_per_group"] in (0, 1, 2, 4, 8)
     assert t["sweeps_per_exchange"] in (5, 10, 20)
     assert t["cell"] in (0.7075, 0.8)
     assert t["proposals_per_s"] > 0
-    tuned = sis.default_params(replicas=16, sweeps=100, init_mode=1, tmin=1e-4, tmax=3e-2,
-                               replicas_per_group=t["replicas_per_group"],
+    assert 0 <= t["quality_checked"] <= 3
+    if t["trajectory_kept"]:
+        assert (t["sweeps_per_exchange"], t["cell"]) == (p.sweeps_per_exchange, p.cell)
+    else:
+        assert t["quality_checked"] >= 1
+        assert (t["checked_sweeps_per_exchange"], t["checked_cell"]) == (t["sweeps_per_exchange"], t["cell"])
+    if t["quality_checked"]:
+        # the last candidate checked, recomputed from two runs of the batch
+        assert t["quality_sweeps"] == 1000
+        spe, cell = t["checked_sweeps_per_exchange"], t["checked_cell"]
+        lcm = spe * p.sweeps_per_exchange // math.gcd(spe, p.sweeps_per_exchange)
+        sw = max(lcm, 1000 // lcm * lcm)
+        own = sis.solve_batch(inst, sis.default_params(**{**kw, "sweeps": sw, "polish_iters": 0}))
+        cand = sis.solve_batch(inst, sis.default_params(**{**kw, "sweeps": sw, "polish_iters": 0,
+                                                           "sweeps_per_exchange": spe, "cell": cell}))
+        d = np.array([(1 - min(a.L_cert, L) / L) - (1 - min(b.L_cert, L) / L)
+                      for a, b, (_, L, _) in zip(own, cand, inst)])
+        say("autotune probe", d)
+        assert d.mean() == pytest.approx(t["quality_delta"], abs=1e-12)
+        assert d.std(ddof=1) / math.sqrt(len(d)) == pytest.approx(t["quality_se"], rel=1e-9, abs=1e-15)
+        # taken exactly when the upper bound of the loss is within the margin
+        assert (t["quality_delta"] + 2 * t["quality_se"] <= 1e-3) == (not t["trajectory_kept"])
+    else:
+        assert t["trajectory_kept"] and t["quality_delta"] == 0.0
+    tuned = sis.default_params(**kw, replicas_per_group=t["replicas_per_group"],
                                sweeps_per_exchange=t["sweeps_per_exchange"], cell=t["cell"])
     res = sis.solve_batch(inst, tuned)
     for (n, _, _), r in zip(inst, res):
