@@ -147,7 +147,8 @@ bool finalize_source(const InstDesc& D, double& box, double& target_L);
 bool device_loaded(long long replicas);
 InstSpec make_spec(const sis_instance* in, const sis_params* p);
 void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);
-const void* sweep_kernel_ptr(int threads);
+// the sweep kernel of a workgroup size; near_pairs: the form that counts near pairs
+const void* sweep_kernel_ptr(int threads, bool near_pairs = false);
 // Launch order of a tile set: tiles of instances with kBigN squares or more
 // (four-wavefront workgroups) first, then the rest, each part heaviest first
 // so the dispatcher hands the long tiles out while compute units free up.

@@ -177,7 +177,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int n, int M, int G, bool wide
     if (sweep_records(n, (int)gn, wide))
         b += kMoverRecBytes * gn;               // mover records
     else if (sweep_prop_in_lds((int)gn, wide))
-        b += 16 * gn + 8 * ((gn + 1) & ~(size_t)1);  // staged proposals
+        b += 32 * gn;                           // staged proposals by level item
     if (sweep_spos_in_lds((int)gn)) b += 16 * gn;  // float4 positions in cell-list order
     b += 2 * (size_t)G * grid_words(M);         // cell starts (u16)
     b += 2 * gn * 5;                            // sorted, cell coords, rank/key, items, list slot

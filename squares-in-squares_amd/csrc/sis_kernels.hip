@@ -234,9 +234,10 @@ static_assert(kRecWords * 16 == kMoverRecBytes, "mover record size");
 
 struct TileLds {
     float4* pos;
-    float4* prop;     // per square: proposed x, y, cos, sin of the sweep (cos = kNoMove:
+    float4* prop;     // per level item: proposed x, y, cos, sin of the sweep (cos = kNoMove:
                       // rejected before the stencil); null when the tile proposes inline
-    float2* pw;       // per square: wall-energy change of the proposal, accept uniform
+    float4* pw;       // per level item: wall-energy change of the proposal, accept uniform,
+                      // square | list slot << 16, cell kx | ky << 8 (the last two as bits)
     float4* spos;     // positions in cell-list order: spos[k] is the square sorted[k];
                       // null when the tile reads positions through `sorted`
     uint4* rec;       // mover records in level order, kRecWords per item (replace prop
@@ -296,8 +297,8 @@ __device__ inline TileLds carve(unsigned char* smem, int n, int M, int G, bool w
         next += kRecWords * gn;
     }
     if (t.prop) {
-        t.pw = reinterpret_cast<float2*>(next);
-        next += (gn + 1) / 2;  // keeps the cell starts 16-byte aligned
+        t.pw = next;
+        next += gn;
     }
     t.A = reinterpret_cast<uint16_t*>(next);
     t.sorted = t.A + (size_t)G * grid_words(M);
@@ -386,15 +387,31 @@ __device__ void bin_tile(const TileLds& t, int n, int G, int M, int m, float inv
 // row ranges of the CSR cell list, visited row by row and, inside a row, in
 // ascending CSR position.  One flat loop per lane costs the wave the largest
 // per-lane total instead of the sum of per-row maxima.
-struct Stencil {
-    int a0, d1, d2, d3, d4, c1, c2, c3, c4, total;
-    __device__ __forceinline__ Stencil(const uint16_t* A, int M, int kx, int ky) {
-        int a[5], l[5];
+struct StencilRows {
+    int a[5], e[5];  // CSR start of each row range and its end
+    __device__ __forceinline__ void load(const uint16_t* A, int M, int kx, int ky) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
             const int row = (ky + r - 2) * M + kx;
             a[r] = A[row - 2];
-            l[r] = A[row + 3] - a[r];
+            e[r] = A[row + 3];
+        }
+    }
+};
+struct Stencil {
+    int a0, d1, d2, d3, d4, c1, c2, c3, c4, total;
+    __device__ __forceinline__ Stencil(const uint16_t* A, int M, int kx, int ky) {
+        StencilRows rw;
+        rw.load(A, M, kx, ky);
+        init(rw);
+    }
+    __device__ __forceinline__ explicit Stencil(const StencilRows& rw) { init(rw); }
+    __device__ __forceinline__ void init(const StencilRows& rw) {
+        int a[5], l[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            a[r] = rw.a[r];
+            l[r] = rw.e[r] - rw.a[r];
         }
         c1 = l[0];
         c2 = c1 + l[1];
@@ -636,6 +653,7 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
                                         int r0, uint32_t sweep, uint32_t k0, uint32_t k1,
                                         int lpi, int sub, unsigned long long& cC,
                                         unsigned long long& cP) {
+    // squares outside the levels (the serial group) draw their move here
     const int g = rep_of(t, q);
     const int gw = grid_words(M);
     const float4 p = t.pos[q];
@@ -644,25 +662,105 @@ __device__ __forceinline__ void propose(const TileLds& t, int q, int n, int M, i
     const int kx = ck & 255, ky = ck >> 8;
     // the stencil depends only on the current cell: its loads go out first
     const Stencil sc(t.A + g * gw, M, kx, ky);
-    float ik = t.ik[g];
+    const float ik = t.ik[g];
     Move mv;
-    float u;
-    if (t.prop) {
-        float4 a = t.prop[q];
-        float2 b = t.pw[q];
-        // the staged records and 1/T in one round trip: without this the
-        // compiler reads the flag first, the rest after the test and 1/T
-        // only once the energy change is known
-        asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(ik));
-        if (a.z == kNoMove) return;
-        mv = Move{a.x, a.y, a.z, a.w, b.x};
-        u = b.y;
-    } else {
-        const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
-        if (!make_move(t, g, p, r, kx, ky, m, inv_w, mv)) return;
-        u = r.w;
+    const float4 r = move_variates((uint32_t)(q - g * n), (uint32_t)(r0 + g), sweep, k0, k1);
+    if (!make_move(t, g, p, r, kx, ky, m, inv_w, mv)) return;
+    evaluate<kQuads, kNear>(t, q, g, self, p, sc, mv, r.w, ik, n, lpi, sub, cC, cP);
+}
+
+// The staged proposal of level item j (tiles with t.prop): the move, wall
+// change, uniform, square, list slot and cell in one LDS round trip; then
+// the square's position, 1/T and stencil.
+template <bool kQuads, bool kNear>
+__device__ __forceinline__ void propose_item(const TileLds& t, int j, int n, int M, int lpi, int sub,
+                                             unsigned long long& cC, unsigned long long& cP) {
+    const float4 a = t.prop[j];
+    const float4 b = t.pw[j];
+    if (a.z == kNoMove) return;
+    const unsigned qs = __float_as_uint(b.z), ck = __float_as_uint(b.w);
+    const int q = (int)(qs & 0xFFFFu), g = rep_of(t, q);
+    const Stencil sc(t.A + g * grid_words(M), M, (int)(ck & 255u), (int)(ck >> 8));
+    evaluate<kQuads, kNear>(t, q, g, (int)(qs >> 16), t.pos[q], sc, Move{a.x, a.y, a.z, a.w, b.x}, b.y, t.ik[g], n,
+                            lpi, sub, cC, cP);
+}
+
+// Level passes of a tile with staged proposals: each mover's data is read one
+// level ahead, its staged proposal and indices when the level before starts
+// (in flight while that level computes) and its stencil rows, position and
+// 1/T once that level's moves are written (in flight across the barrier), so
+// that a level waits on one LDS round trip, its candidates.  A square moves
+// only in its own level and no cell list changes during a sweep, so nothing
+// read ahead is written in between.  A level with more movers than one round
+// of lanes takes rounds of propose_item instead.  gs: lane l holds the start
+// of level l (lane kGroups: the end of the last).
+template <int NT, bool kNear>
+__device__ __forceinline__ void staged_levels(const TileLds& t, int n, int M, int gs, unsigned long long& cC,
+                                              unsigned long long& cP) {
+    const int tid = threadIdx.x;
+    const int gw = grid_words(M);
+    auto lsh_of = [](int cnt) {
+        return cnt > NT / 2 ? 0 : cnt > NT / 4 ? 1 : cnt > NT / 8 ? 2 : cnt > NT / 16 ? 3 : 4;
+    };
+    struct Ahead {
+        float4 mv, side, p;
+        float ik;
+        StencilRows rw;
+    };
+    auto stage1 = [&](int j, Ahead& x) {
+        x.mv = t.prop[j];
+        x.side = t.pw[j];
+    };
+    auto stage2 = [&](Ahead& x) {
+        const unsigned qs = __float_as_uint(x.side.z), ck = __float_as_uint(x.side.w);
+        const int q = (int)(qs & 0xFFFFu), g = rep_of(t, q);
+        x.p = t.pos[q];
+        x.ik = t.ik[g];
+        x.rw.load(t.A + g * gw, M, (int)(ck & 255u), (int)(ck >> 8));
+    };
+    int a = __builtin_amdgcn_readlane(gs, 0), b = __builtin_amdgcn_readlane(gs, 1);
+    int lsh = lsh_of(b - a);
+    Ahead cur;
+    bool cur_ok = false;
+    {
+        const int j = a + (tid >> lsh);
+        if (j < b && b - a <= (NT >> lsh)) {
+            stage1(j, cur);
+            stage2(cur);
+            cur_ok = true;
+        }
     }
-    evaluate<kQuads, kNear>(t, q, g, self, p, sc, mv, u, ik, n, lpi, sub, cC, cP);
+    for (int lv = 0; lv < kGroups && a < b; ++lv) {
+        const int cnt = b - a;
+        const int lpi = 1 << lsh, sub = tid & (lpi - 1);
+        const int a2 = b, b2 = lv + 1 < kGroups ? __builtin_amdgcn_readlane(gs, lv + 2) : b;
+        const int lsh2 = lsh_of(b2 - a2);
+        const int j2 = a2 + (tid >> lsh2);
+        const bool ok2 = j2 < b2 && b2 - a2 <= (NT >> lsh2);
+        Ahead nxt;
+        if (ok2) stage1(j2, nxt);
+        if (cnt <= (NT >> lsh)) {
+            if (cur_ok && cur.mv.z != kNoMove) {
+                const unsigned qs = __float_as_uint(cur.side.z);
+                const int q = (int)(qs & 0xFFFFu);
+                evaluate<(NT > 64), kNear>(t, q, rep_of(t, q), (int)(qs >> 16), cur.p, Stencil(cur.rw),
+                                           Move{cur.mv.x, cur.mv.y, cur.mv.z, cur.mv.w, cur.side.x}, cur.side.y,
+                                           cur.ik, n, lpi, sub, cC, cP);
+            }
+        } else {
+            const int per = NT >> lsh;
+            for (int base = a; base < b; base += per)
+                if (base + (tid >> lsh) < b)
+                    propose_item<(NT > 64), kNear>(t, base + (tid >> lsh), n, M, lpi, sub, cC, cP);
+        }
+        if (ok2) stage2(nxt);
+        tile_sync<NT>();
+        cur = nxt;
+        cur_ok = ok2;
+        a = a2;
+        b = b2;
+        lsh = lsh2;
+    }
 }
 
 // Mover record j of a level (one-wavefront tiles): square q (replica g,
@@ -1295,15 +1393,17 @@ __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs
 // shift): every square's staged move is drawn again from its current position,
 // either into prop / pw or into its mover record j (one-wavefront tiles with
 // records; square, CSR slot, replica, 1/T and stencil stay).
-__device__ __forceinline__ void stage_prop(const TileLds& t, int q, int n, int m, float inv_w, int r0,
+// The staged proposal of square q as level item j.
+__device__ __forceinline__ void stage_item(const TileLds& t, int j, int q, int n, int m, float inv_w, int r0,
                                            uint32_t sweep, uint32_t k0, uint32_t k1) {
     const int g = rep_of(t, q), i = q - g * n;
     const int ck = t.cellk[q];
     const float4 r = move_variates((uint32_t)i, (uint32_t)(r0 + g), sweep, k0, k1);
     Move mv;
     const bool ok = make_move(t, g, t.pos[q], r, ck & 255, ck >> 8, m, inv_w, mv);
-    t.prop[q] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
-    t.pw[q] = make_float2(ok ? mv.wdiff : 0.0f, r.w);
+    t.prop[j] = ok ? make_float4(mv.nx, mv.ny, mv.nc, mv.ns) : make_float4(0.0f, 0.0f, kNoMove, 0.0f);
+    t.pw[j] = make_float4(ok ? mv.wdiff : 0.0f, r.w, __uint_as_float((unsigned)q | (unsigned)t.slotq[q] << 16),
+                          __uint_as_float((unsigned)ck));
 }
 __device__ __forceinline__ void restage_record(const TileLds& t, int j, int n, int m, float inv_w, int r0, uint32_t sweep, uint32_t k0, uint32_t k1) {
     uint4* w = t.rec + (size_t)j * kRecWords;
@@ -1489,7 +1589,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int items = __builtin_amdgcn_readlane(gs, kGroups);
                 for (int j = tid; j < items; j += NT) restage_record(t, j, n, m, inv_w, tl.r0, sweep, k0, k1);
             } else if (t.prop) {
-                for (int q = tid; q < G * n; q += NT) stage_prop(t, q, n, m, inv_w, tl.r0, sweep, k0, k1);
+                const int items = __builtin_amdgcn_readlane(gs, kGroups);
+                for (int j = tid; j < items; j += NT)
+                    stage_item(t, j, t.items[j], n, m, inv_w, tl.r0, sweep, k0, k1);
             }
             tile_sync<NT>();
         } else if (NT == 64 && t.rec && G * n <= kSweepThreads) {
@@ -1502,10 +1604,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             tile_sync<NT>();
             bin_tile<NT>(t, n, G, M, m, inv_w);
             // group key: occupancy rank x colour; ranks >= kRankCap go serial.
-            // Small tiles stage every square's proposal here (one Philox block
-            // per square; a square's own state does not change before its level
-            // runs), while the LDS atomics of the key pass are in flight; tiles
-            // with mover records draw it when the records are written.
+            // Tiles with staged proposals or mover records draw each square's
+            // move (one Philox block per square; a square's own state does not
+            // change before its level runs) when its level item is written.
             for (int q = tid; q < G * n; q += NT) {
                 const int g = rep_of(t, q);
                 const int ck = t.cellk[q];
@@ -1514,7 +1615,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int key = r < kRankCap ? r * 9 + colour : kGroups;
                 t.rank[q] = (uint16_t)key;
                 atomicOr(&t.gmask[g], 1ull << key);
-                if (t.prop) stage_prop(t, q, n, m, inv_w, tl.r0, sweep, k0, k1);
             }
             if (next_in_launch)
                 for (int g = tid; g < G; g += NT) {
@@ -1551,10 +1651,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int lvl = t.rank[q];
                 if (lvl < kGroups) {
                     const int j = atomicAdd(&t.gcnt[lvl], 1);
-                    if (NT == 64 && t.rec)
+                    if (NT == 64 && t.rec) {
                         write_record(t, j, q, n, M, m, inv_w, tl.r0, sweep, k0, k1);
-                    else
+                    } else {
                         t.items[j] = (uint16_t)q;
+                        if (t.prop) stage_item(t, j, q, n, m, inv_w, tl.r0, sweep, k0, k1);
+                    }
                 }
             }
             tile_sync<NT>();
@@ -1562,7 +1664,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if constexpr (NT == 64) {
             if (t.rec) rec_levels<kNear>(t, n, gs, cC, cP);
         }
-        for (int lv = 0; lv < kGroups && !(NT == 64 && t.rec); ++lv) {
+        // four-wavefront tiles with staged proposals read each level ahead (a
+        // lone wavefront has no registers to spare for it)
+        if constexpr (NT > 64) {
+            if (t.prop) staged_levels<NT, kNear>(t, n, M, gs, cC, cP);
+        }
+        for (int lv = 0; lv < kGroups && !(NT == 64 && t.rec) && !(NT > 64 && t.prop); ++lv) {
             const int a = __builtin_amdgcn_readlane(gs, lv);
             const int b = __builtin_amdgcn_readlane(gs, lv + 1);
             if (a == b) break;  // levels are dense: the first empty one is the last
@@ -1575,14 +1682,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int slot = tid >> lsh, sub = tid & (lpi - 1);
             for (int base = a; base < b; base += per) {
                 if (base + slot >= b) continue;
-                if constexpr (NT == 64) {
-                    if (t.rec) {
-                        propose_rec<kNear>(t, base + slot, n, lpi, sub, cC, cP);
-                        continue;
-                    }
-                }
-                propose<(NT > 64), kNear>(t, t.items[base + slot], n, M, m, inv_w, tl.r0, sweep, k0, k1, lpi, sub,
-                                          cC, cP);
+                if (t.prop)
+                    propose_item<(NT > 64), kNear>(t, base + slot, n, M, lpi, sub, cC, cP);
+                else
+                    propose<(NT > 64), kNear>(t, t.items[base + slot], n, M, m, inv_w, tl.r0, sweep, k0, k1, lpi,
+                                              sub, cC, cP);
             }
             tile_sync<NT>();
         }
@@ -2553,8 +2657,10 @@ hipError_t launch_accept_exp2(hipStream_t st, int count, const float* x, float* 
     return hipGetLastError();
 }
 
-const void* sweep_kernel_ptr(int threads) {
-    // the production form (no near-pair counts); both forms use the same LDS
+const void* sweep_kernel_ptr(int threads, bool near_pairs) {
+    if (near_pairs)
+        return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, true>)
+                                           : reinterpret_cast<const void*>(k_sweep<kSweepThreads, true>);
     return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, false>)
                                        : reinterpret_cast<const void*>(k_sweep<kSweepThreads, false>);
 }

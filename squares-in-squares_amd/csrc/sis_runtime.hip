@@ -55,8 +55,9 @@ int ensure_device(int device) {
     HIPCHECK(hipGetDeviceProperties(&prop, id));
     // a sweep tile may use all the LDS a workgroup can hold (160 KB on gfx950)
     for (int threads : {kSweepThreads, kSweepThreadsBig})
-        HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)prop.sharedMemPerBlock));
+        for (bool near_pairs : {false, true})
+            HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads, near_pairs),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)prop.sharedMemPerBlock));
     // the allocation-free solve's device buffer: its address on this device
     if (int rc = noalloc_setup(id)) return rc;
     g_dev.cus = prop.multiProcessorCount;

@@ -185,8 +185,9 @@ def test_deadline_passes_in_the_queue(sis):
 def test_arena_follows_the_budget(sis):
     """The arena follows hbm_budget_bytes (here 6 GiB, past the default
     4 GiB): 70 requests of 1000 squares x 4096 replicas (4.6 GB of replica
-    state), each searching until its 6 s deadline, are resident at once, and
-    all return their results."""
+    state), each searching until its 15 s deadline (admitting all 70 takes
+    several seconds: each uploads 65 MB of replica state), are resident at
+    once, and all return their results."""
     say("server arena")
     budget = 6 * 2**30
     srv = sis.Server(max_instances=80, intervals_per_round=1, certifier_threads=1, certify=False,
@@ -196,7 +197,7 @@ def test_arena_follows_the_budget(sis):
     L0 = 32 * 1.00002
     q0 = np.array([((i % 32) + 0.5, (i // 32) + 0.5, 0.0) for i in range(1000)]) * [1.00002, 1.00002, 1]
     tickets = [srv.submit(1000, L0, q0, replicas=4096, sweeps=10**6, polish_iters=0, init_mode=0, rng_seed=k,
-                          timeout_s=6.0) for k in range(70)]
+                          timeout_s=15.0) for k in range(70)]
     out = _outcome(sis, srv, tickets, timeout=120)
     assert all(kind == "ok" and r.status == sis.SIS_ETIMEOUT and r.L_cert <= L0 for kind, r in out.values())
     st = srv.stats()
