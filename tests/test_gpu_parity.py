@@ -119,7 +119,7 @@ def _model_run(oracle, st, L, w, ik, sxy, sth, key0, key1, sweep0, sweeps, shift
                                                 (64, 10.0, 2, 3, 1), (60, 3.0, 2, 3, 1),
                                                 (100, 11.0, 4, 3, 4), (120, 12.5, 3, 3, 2),
                                                 (300, 17.5, 2, 3, 1), (300, 17.5, 4, 2, 2),
-                                                (700, 27.0, 1, 2, 1)])
+                                                (400, 21.0, 2, 2, 2), (700, 27.0, 1, 2, 1)])
 def test_sweeps_match_cpu_model(sis, oracle, n, L, R, sweeps, group):
     """k_sweep == the CPU model of the same colour-checkerboard sweep: same
     states, accept counts and canonical energies, bit for bit.  Below n = 96
