@@ -207,14 +207,24 @@ int sis_solve_batch(const sis_instance* insts, int count, const sis_params* para
  * (cuda/packing.cu:823-853) opened up so a caller can run, time and finish
  * it in pieces.  mode 0 = plain (host synchronisation after every interval),
  * mode 1 = HIP graph (no host synchronisation).  intervals <= 0 runs what is
- * left of the budget.  sweep_ms is the HIP-event time of the sweep kernel
- * launches (plain mode; 0 in graph mode), total_ms that of the whole run. */
+ * left of the budget; a plain run stops early once every instance has run its
+ * sweeps.  sweep_ms is the HIP-event time of the sweep launches that were
+ * timed and sweep_launches their number (plain mode: every eighth launch of
+ * the call, starting with the first; 0 in graph mode), total_ms the time of
+ * the whole run. */
 typedef struct sis_batch sis_batch;
 sis_batch* sis_batch_create(const sis_instance* insts, int count, const sis_params* params);
 int sis_batch_run(sis_batch* b, int64_t intervals, int mode, double* sweep_ms, int64_t* sweep_launches,
                   double* total_ms);
 int sis_batch_finalize(sis_batch* b, sis_result* out);
 int sis_batch_destroy(sis_batch* b);
+/* The status of instance k as the host last read it, without a device copy:
+ * in mode 0 each exchange step writes its instance's descriptor into mapped
+ * host memory, which the plain loop reads after every interval (the
+ * reference's per-epoch blocking read of the chains' best sides,
+ * cuda/packing.cu:831); a mode 1 run copies it once at its end. */
+struct sis_ladder_desc;  /* declared with the parity probes below */
+int sis_batch_status(sis_batch* b, int k, struct sis_ladder_desc* d);
 
 /* Autotuning (no reference counterpart: the reference picks chains from an
  * occupancy query, cuda/packing.cu:788-799, and its autotune.py runs a fixed

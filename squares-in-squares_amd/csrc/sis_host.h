@@ -46,7 +46,13 @@ public:
     int enqueue_interval();
     int enqueue_chain(int c);
     int run(long long intervals, int mode);
+    // The plain schedule times one sweep launch in this many with a HIP
+    // event pair: the two event markers around every launch cost about 5 us
+    // of a 122 us interval at n = 50 (1.04e9 against 1.08e9 proposals/s with
+    // every eighth launch timed, 1.09e9 with none).
+    static constexpr int kSweepTimingEvery = 8;
     int run_plain(long long intervals);
+    int wait_status(const std::vector<unsigned>& want);
     int run_graph(int per_graph, long long launches);
     int finalize(std::vector<FinalRecord>& rec);
     void release();
@@ -90,7 +96,10 @@ public:
     hipEvent_t ev_cfork = nullptr;
 
     InstDesc* d_inst = nullptr;
-    InstDesc* h_pin = nullptr;  // pinned copy of the descriptors
+    InstDesc* h_pin = nullptr;  // mapped host mirror of the descriptors
+    InstDesc* d_pin = nullptr;  // its device address
+    unsigned* h_seq = nullptr;  // mapped host words: interval count of each mirror entry
+    unsigned* d_seq = nullptr;  // their device address
     Tile* d_tiles = nullptr;
     RepState* d_reps = nullptr;
     SlotState* d_slots = nullptr;

@@ -207,7 +207,7 @@ __host__ __device__ inline size_t ladder_lds_bytes(int n, double L, int R) {
     b += 4 * (mc * mc + 1);                  // cell starts
     b += 4 * (size_t)kLadderThreads + 4;     // scan partials, flag
     b += 2 * (size_t)n;                      // square indices in cell order
-    const size_t ladder = 12 * (size_t)R;    // before the test: energy, T, replica per slot
+    const size_t ladder = 16 * (size_t)R;    // before the test: energy, 1/T, replica, acceptance per slot
     return ((b > ladder ? b : ladder) + 15) & ~(size_t)15;
 }
 

@@ -2047,43 +2047,53 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                                                            SlotState* __restrict__ slots,
                                                            int* __restrict__ rep_of_slot,
                                                            float4* __restrict__ state,
-                                                           float4* __restrict__ arch) {
+                                                           float4* __restrict__ arch,
+                                                           InstDesc* __restrict__ status,
+                                                           unsigned* __restrict__ seq) {
     extern __shared__ __align__(16) unsigned char lsm[];
     __shared__ int zero_slot;
     __shared__ int best_r;
     __shared__ unsigned long long best_lc;
-    __shared__ unsigned long long tot[7];  // accepted, certified, candidates, near pairs, collective tried / accepted, kicks
-    InstDesc* D = insts + (list ? list[blockIdx.x] : (int)blockIdx.x);
-    const long long left = D->sweeps_total - D->sweeps_done;
+    // accepted, certified, candidates, near pairs, collective tried / accepted,
+    // kicks, exchanges tried / accepted
+    __shared__ unsigned long long tot[9];
+    const int di = list ? list[blockIdx.x] : (int)blockIdx.x;
+    InstDesc* D = insts + di;
+    // Every descriptor field in one round of loads; the kernel's updates are
+    // made to this copy and stored by one thread at the end (no other kernel
+    // writes the descriptor meanwhile, and each instance has one workgroup).
+    InstDesc d = *D;
+    const long long left = d.sweeps_total - d.sweeps_done;
     if (left <= 0) return;
-    const int ran = (int)min((long long)D->spe, left);
+    const int ran = (int)min((long long)d.spe, left);
     const int tid = threadIdx.x;
-    const int R = D->R, n = D->n, lad = D->ladder, base = D->rep_off;
-    const int improve = D->improve;
+    const int R = d.R, n = d.n, lad = d.ladder, base = d.rep_off;
+    const int improve = d.improve;
 
     // The ladder of the instance in LDS, by temperature slot: energy and
-    // 1/T (log2 units) of the slot, replica holding it.  One round of global
-    // loads (replica records, then each replica's slot record).
+    // 1/T (log2 units) of the slot, replica holding it and that replica's
+    // accepted moves.  The replica and the slot records load together.
     float* eS = reinterpret_cast<float*>(lsm);
     float* kS = eS + R;
     int* rS = reinterpret_cast<int*>(kS + R);
-    const float target = D->target_acc;
+    unsigned* aS = reinterpret_cast<unsigned*>(rS + R);
     if (tid == 0) {
         zero_slot = R;
         best_r = R;
         best_lc = ~0ull;
     }
-    if (tid < 7) tot[tid] = 0ull;
+    if (tid < 9) tot[tid] = 0ull;
     __syncthreads();
 
-    // 1. step adaptation per temperature slot, from the replica that held it
-    //    (and the smallest certified side: per thread, per wavefront, then one
-    //    LDS atomic per wavefront); the interval's counters of the replicas and
-    //    of the sweep workgroups move into the instance's totals
+    // 1. the replicas' energies, acceptances and certified sides by the slot
+    //    they held (the smallest side: per thread, per wavefront, then one LDS
+    //    atomic per wavefront); the interval's counters of the replicas and of
+    //    the sweep workgroups move into the instance's totals
     unsigned long long my_lc = ~0ull;
     unsigned long long c_acc = 0, c_cert = 0, c_cand = 0, c_near = 0, c_ct = 0, c_ca = 0, c_kick = 0;
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
+        kS[r] = slots[base + r].inv_t_log2e;  // slot r
         c_acc += rs.acc;
         c_cert += (improve && rs.Lc < kCertNone) ? 1u : 0u;
         if (rs.cand | rs.near_pairs | rs.coll_tried | rs.coll_acc | rs.kicks) {
@@ -2099,16 +2109,9 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             w->coll_acc = 0u;
             w->kicks = 0u;
         }
-        const float rate = (float)rs.acc / (float)((long long)n * ran);
-        // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
-        const float f = accept_exp2((rate - target) * 0.72134752f);
-        SlotState* sl = slots + base + rs.slot;
-        const SlotState ss = *sl;
-        sl->base_xy = fminf(fmaxf(ss.base_xy * f, 1.7320508e-7f), 0.8660254f);
-        sl->base_th = fminf(fmaxf(ss.base_th * f, 1.7320508e-7f), 0.7794229f);
         eS[rs.slot] = rs.E;
-        kS[rs.slot] = ss.inv_t_log2e;
         rS[rs.slot] = r;
+        aS[rs.slot] = rs.acc;
         if (improve) my_lc = min(my_lc, (unsigned long long)__double_as_longlong(rs.Lc));
     }
     if (improve) {
@@ -2138,9 +2141,11 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     __syncthreads();
 
     // 2. improvement: the replica with the smallest certified side
-    const uint32_t interval = D->interval, key0 = D->key0, key1 = D->key1;
-    const double L0 = D->L;
+    const uint32_t interval = d.interval, key0 = d.key0, key1 = d.key1;
+    const double L0 = d.L;
+    const double shrink = (double)d.shrink, lmin = d.Lmin + 1e-6;
     bool improved = false;
+    double Lb = 0.0;
     if (improve) {
         const double lc = __longlong_as_double((long long)best_lc);
         improved = lc < L0 - 1e-11;
@@ -2150,40 +2155,35 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             __syncthreads();
             const int rb = best_r;
             const RepState rs = reps[base + rb];
-            const double Lb = rs.Lc, lam = rs.lam, mx = rs.minx, my = rs.miny;
-            const float E_b = rs.E;
+            const double lam = rs.lam, mx = rs.minx, my = rs.miny;
+            Lb = rs.Lc;
             const float box_b = slots[base + rs.slot].L;
-            const float4* src = state + D->state_off + (size_t)rb * n;
-            for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
+            const float4* src = state + d.state_off + (size_t)rb * n;
+            for (int i = tid; i < n; i += kLadderThreads) arch[d.arch_off + i] = src[i];
             __syncthreads();
-            // every slot's box below the new best side
-            const double shrink = (double)D->shrink, lmin = D->Lmin + 1e-6;
-            for (int s = tid; s < R; s += kLadderThreads)
-                slots[base + s].L = (float)fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
             // restarts: slot 0 of every ladder, and the whole first ladder on
-            // a large gain
+            // a large gain (every slot's box below the new best side is set
+            // with the slot records in step 4)
             const int heads = R / lad, extra = (L0 - Lb > 5e-4) ? lad - 1 : 0;
-            const float4* best = arch + D->arch_off;
+            const float4* best = arch + d.arch_off;
             for (int k = tid; k < (heads + extra) * n; k += kLadderThreads) {
                 const int h = k / n, i = k - h * n;
                 const int s = h < heads ? h * lad : h - heads + 1;
                 const double tL = fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
                 const double sc = tL / Lb;
                 const float4 v = best[i];
-                float4* dst = state + D->state_off + (size_t)rS[s] * n;
+                float4* dst = state + d.state_off + (size_t)rS[s] * n;
                 dst[i] = make_float4((float)(((double)v.x * lam - mx) * sc), (float)(((double)v.y * lam - my) * sc),
                                      v.z, v.w);
             }
-            if (tid == 0) {
-                D->L = Lb;
-                D->Lf = (float)Lb;
-                D->L_arch = Lb;
-                D->L_box_arch = (double)box_b;
-                D->E_arch = E_b;
-                D->arch_valid = 1;
-                D->feasible += 1;
-                D->shrinks += 1;
-            }
+            d.L = Lb;
+            d.Lf = (float)Lb;
+            d.L_arch = Lb;
+            d.L_box_arch = (double)box_b;
+            d.E_arch = rs.E;
+            d.arch_valid = 1;
+            d.feasible += 1;
+            d.shrinks += 1;
         }
     }
 
@@ -2212,25 +2212,51 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                 ++accd;
             }
         }
-        if (tried) atomicAdd(&D->ex_tried, tried);
-        if (accd) atomicAdd(&D->ex_acc, accd);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            tried += __shfl_xor(tried, off, 64);
+            accd += __shfl_xor(accd, off, 64);
+        }
+        if ((tid & 63) == 0 && tried) {
+            atomicAdd(tot + 7, tried);
+            atomicAdd(tot + 8, accd);
+        }
+    }
+
+    // 4. every slot record, written once: step adaptation from the acceptance
+    //    of the replica that held the slot, base x e^((rate - target) / 2)
+    //    clamped; the box (below the new best side after an improvement); the
+    //    next interval's temperature and step sizes (cooling cycle).  The
+    //    record was read in step 1 by this thread, so it comes from the cache.
+    {
+        const float target = d.target_acc;
+        double cool, root;
+        cool_factor(interval + 1u, d.cool_period, cool, root);
+        for (int s = tid; s < R; s += kLadderThreads) {
+            SlotState ss = slots[base + s];
+            const float rate = (float)aS[s] / (float)((long long)n * ran);
+            // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
+            const float f = accept_exp2((rate - target) * 0.72134752f);
+            ss.base_xy = fminf(fmaxf(ss.base_xy * f, 1.7320508e-7f), 0.8660254f);
+            ss.base_th = fminf(fmaxf(ss.base_th * f, 1.7320508e-7f), 0.7794229f);
+            if (improved) ss.L = (float)fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
+            slot_effective(ss, cool, root);
+            slots[base + s] = ss;
+        }
     }
     __syncthreads();
 
-    // 4. zero-overlap test: coldest replica with FP32 energy exactly zero.
-    //    Descriptor fields are read into registers before any thread writes.
+    // 5. zero-overlap test: coldest replica with FP32 energy exactly zero
     double scale = 1.0, Ln = L0;
-    bool archive = false;
+    bool archive = false, rejected = false;
     if (!improve) {
-        const int had_arch = D->arch_valid;
-        const double L_arch = D->L_arch;
         for (int s = tid; s < R; s += kLadderThreads)
             if (eS[s] == 0.0f) atomicMin(&zero_slot, s);
         __syncthreads();
         if (zero_slot < R) {
             const int r = rS[zero_slot];
             __syncthreads();  // the ladder's LDS is reused below
-            const float4* src = state + D->state_off + (size_t)r * n;
+            const float4* src = state + d.state_off + (size_t)r * n;
             const LadderLds t = carve_ladder(lsm, n, L0);
             for (int i = tid; i < n; i += kLadderThreads) {
                 const float4 v = src[i];
@@ -2240,56 +2266,56 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             }
             __syncthreads();
             const bool zero64 = overlap_free64<kLadderThreads>(t.x, t.y, t.c, t.s, n, L0, t.zg);
-            if (!zero64 && tid == 0) D->fp64_rejects += 1;
+            rejected = !zero64;
             if (zero64) {
-                archive = !had_arch || L0 < L_arch;
+                archive = !d.arch_valid || L0 < d.L_arch;
                 if (archive)
-                    for (int i = tid; i < n; i += kLadderThreads) arch[D->arch_off + i] = src[i];
-                Ln = fmax(L0 - (double)D->shrink, D->Lmin);
+                    for (int i = tid; i < n; i += kLadderThreads) arch[d.arch_off + i] = src[i];
+                Ln = fmax(L0 - shrink, d.Lmin);
                 if (Ln < L0) scale = Ln / L0;
             }
         }
-        __syncthreads();
         if (Ln < L0)
             for (int s = tid; s < R; s += kLadderThreads) slots[base + s].L = (float)Ln;
     }
+    __syncthreads();
 
-    // 5. the next interval's temperatures and step sizes (after every base
-    //    write of step 1: the barriers above order them)
-    {
-        double cool, root;
-        cool_factor(interval + 1u, D->cool_period, cool, root);
-        for (int s = tid; s < R; s += kLadderThreads) {
-            SlotState ss = slots[base + s];
-            slot_effective(ss, cool, root);
-            *reinterpret_cast<float4*>(slots + base + s) =
-                make_float4(ss.inv_t_log2e, ss.step_xy, ss.step_th, ss.L);
-        }
-    }
     if (tid == 0) {
+        if (rejected) d.fp64_rejects += 1;
         if (archive) {
-            D->arch_valid = 1;
-            D->L_arch = L0;
-            D->L_box_arch = L0;
-            D->E_arch = 0.0f;
-            D->feasible += 1;
+            d.arch_valid = 1;
+            d.L_arch = L0;
+            d.L_box_arch = L0;
+            d.E_arch = 0.0f;
+            d.feasible += 1;
         }
         if (Ln < L0) {
-            D->L = Ln;
-            D->Lf = (float)Ln;
-            D->shrinks += 1;
+            d.L = Ln;
+            d.Lf = (float)Ln;
+            d.shrinks += 1;
         }
-        D->scale = (float)scale;
-        D->sweeps_done += ran;
-        D->interval += 1u;
-        D->proposals += (unsigned long long)n * (unsigned long long)R * (unsigned long long)ran;
-        D->accepted += tot[0];
-        D->certs += tot[1];
-        D->cand += tot[2];
-        D->near += tot[3];
-        D->coll_tried += tot[4];
-        D->coll_acc += tot[5];
-        D->kicks += (int)tot[6];
+        d.scale = (float)scale;
+        d.sweeps_done += ran;
+        d.interval += 1u;
+        d.proposals += (unsigned long long)n * (unsigned long long)R * (unsigned long long)ran;
+        d.accepted += tot[0];
+        d.certs += tot[1];
+        d.cand += tot[2];
+        d.near += tot[3];
+        d.coll_tried += tot[4];
+        d.coll_acc += tot[5];
+        d.kicks += (int)tot[6];
+        d.ex_tried += tot[7];
+        d.ex_acc += tot[8];
+        *D = d;
+        if (status) {
+            // the host's mirror (mapped host memory), then the sequence word
+            // the host polls: the interval count, stored with system-scope
+            // release after the mirror, so a host that reads the new count
+            // also reads the whole descriptor
+            status[di] = d;
+            __hip_atomic_store(seq + di, d.interval, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -2611,9 +2637,9 @@ hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tile
 
 hipError_t launch_ladder(hipStream_t st, int ninst, size_t lds, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
-                         const int* list) {
+                         const int* list, InstDesc* status, unsigned* seq) {
     hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), lds, st, insts, list, reps, slots,
-                       rep_of_slot, state, arch);
+                       rep_of_slot, state, arch, status, seq);
     return hipGetLastError();
 }
 

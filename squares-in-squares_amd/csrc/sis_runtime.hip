@@ -359,7 +359,11 @@ void Batch::release() {
                     (void*)d_state, (void*)d_arch, (void*)d_jobs, (void*)d_fout, (void*)d_qout})
         if (p) (void)hipFree(p);
     if (h_pin) (void)hipHostFree(h_pin);
+    if (h_seq) (void)hipHostFree(h_seq);
     h_pin = nullptr;
+    d_pin = nullptr;
+    h_seq = nullptr;
+    d_seq = nullptr;
     for (Chain& c : chains) {
         if (c.done) (void)hipEventDestroy(c.done);
         if (c.st) (void)hipStreamDestroy(c.st);
@@ -439,9 +443,16 @@ int Batch::build(std::vector<InstSpec>&& in) {
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-    // pinned mirror of the descriptors: the plain schedule reads them back
-    // after every interval
-    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(InstDesc) * ni));
+    // mapped, coherent host mirror of the descriptors and their sequence
+    // words: in the plain schedule every exchange step writes its instance's
+    // descriptor and then its interval count, which the host polls after
+    // every interval
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(InstDesc) * ni,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_pin), h_pin, 0));
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_seq), sizeof(unsigned) * ni,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_seq), h_seq, 0));
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHECK(hipEventCreate(&ev0));
@@ -564,33 +575,105 @@ int Batch::enqueue_interval() {
     return SIS_OK;
 }
 
-// Plain schedule: one interval at a time, host waits on the instance status
-// after every interval (the reference's per-epoch blocking copy,
-// cuda/packing.cu:831).  The sweep kernel of every interval is timed with its
-// own event pair on the launch stream.
+// Plain schedule: one interval at a time, and the host learns every
+// instance's progress after every interval before it starts the next (the
+// reference's per-epoch blocking copy, cuda/packing.cu:831).  The exchange
+// step writes each instance's descriptor into the mapped host mirror and then
+// its interval count; the host polls the counts, so it sees the step's end
+// as soon as the step has written it and its next launch is queued while the
+// step finishes.  Progress is counted on the host from the counts (the mirror
+// is read back from the device once the run has ended).  The sweep kernel of
+// every kSweepTimingEvery-th interval is timed with its own event pair on the
+// launch stream.
 int Batch::run_plain(long long intervals) {
+    const int ni = (int)specs.size();
+    HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * ni, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    std::vector<unsigned> want(ni), iv_of(ni);
+    std::vector<long long> done(ni);
+    for (int k = 0; k < ni; ++k) {
+        iv_of[k] = h_pin[k].interval;
+        done[k] = h_pin[k].sweeps_done;
+        __atomic_store_n(h_seq + k, iv_of[k], __ATOMIC_RELEASE);
+    }
     HIPCHECK(hipEventRecord(ev0, stream));
     for (long long iv = 0; iv < intervals; ++iv) {
-        HIPCHECK(hipEventRecord(evs0, stream));
+        // stop once every instance has run its sweeps
+        bool live = false;
+        for (int k = 0; k < ni; ++k) {
+            const bool l = done[k] < h_pin[k].sweeps_total;
+            want[k] = l ? iv_of[k] + 1u : 0u;
+            live = live || l;
+        }
+        if (!live) break;
+        const bool timed = iv % kSweepTimingEvery == 0;
+        if (timed) HIPCHECK(hipEventRecord(evs0, stream));
         if (int rc = launch_tiles()) return rc;
-        HIPCHECK(hipEventRecord(evs1, stream));
-        HIPCHECK(launch_ladder(stream, (int)specs.size(), lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
-        HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * specs.size(), hipMemcpyDeviceToHost, stream));
-        HIPCHECK(hipStreamSynchronize(stream));
-        float sms = 0;
-        HIPCHECK(hipEventElapsedTime(&sms, evs0, evs1));
-        sweep_ms += sms;
-        sweep_launches += 1;
+        if (timed) HIPCHECK(hipEventRecord(evs1, stream));
+        HIPCHECK(launch_ladder(stream, ni, lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch, nullptr,
+                               d_pin, d_seq));
+        if (int rc = wait_status(want)) return rc;
+        for (int k = 0; k < ni; ++k)
+            if (want[k]) {
+                iv_of[k] = want[k];
+                done[k] += std::min<long long>(h_pin[k].spe, h_pin[k].sweeps_total - done[k]);
+            }
+        if (timed) {
+            HIPCHECK(hipEventSynchronize(evs1));
+            float sms = 0;
+            HIPCHECK(hipEventElapsedTime(&sms, evs0, evs1));
+            sweep_ms += sms;
+            sweep_launches += 1;
+        }
+        intervals_done += 1;
     }
     HIPCHECK(hipEventRecord(ev1, stream));
     HIPCHECK(hipEventSynchronize(ev1));
-    std::copy(h_pin, h_pin + specs.size(), h_inst.begin());
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
     device_ms += ms;
     last_ms = ms;
-    intervals_done += intervals;
+    HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * ni, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
     return SIS_OK;
+}
+
+// Wait until every instance with want[k] != 0 has published interval count
+// want[k].  The stream is queried every few hundred polls; an error ends the
+// wait with SIS_EDEVICE.  The stream's completion and the count written by
+// its last kernel reach the host by different paths, so after the stream has
+// drained the count is polled for a while longer, and the device's own
+// descriptors decide in the end.
+int Batch::wait_status(const std::vector<unsigned>& want) {
+    const int ni = (int)want.size();
+    auto published = [&]() {
+        for (int k = 0; k < ni; ++k)
+            if (want[k] != 0u && __atomic_load_n(h_seq + k, __ATOMIC_ACQUIRE) != want[k]) return false;
+        return true;
+    };
+    for (unsigned polls = 1;; ++polls) {
+        if (published()) return SIS_OK;
+        if ((polls & 255u) == 0u) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady)
+                return fail(SIS_EDEVICE, std::string("device error: ") + hipGetErrorString(q));
+        } else {
+            __builtin_ia32_pause();
+        }
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    while (std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(20))
+        if (published()) return SIS_OK;
+    std::vector<InstDesc> dev(ni);
+    HIPCHECK(hipMemcpyAsync(dev.data(), d_inst, sizeof(InstDesc) * ni, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    std::string msg;
+    for (int k = 0; k < ni; ++k)
+        if (want[k] != 0u && dev[k].interval != want[k])
+            msg += " instance " + std::to_string(k) + ": interval " + std::to_string(dev[k].interval) +
+                   " after the step, " + std::to_string(want[k]) + " expected;";
+    return msg.empty() ? SIS_OK : fail(SIS_EDEVICE, "exchange step did not advance:" + msg);
 }
 
 int Batch::run(long long intervals, int mode) {
@@ -617,6 +700,9 @@ int Batch::run(long long intervals, int mode) {
         last_ms = graph_ms + ms;
         intervals_done += rest;
     }
+    // the host mirror after a graph run (untimed): one copy per call
+    HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * specs.size(), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
     return SIS_OK;
 }
 
@@ -1087,6 +1173,22 @@ int sis_batch_destroy(sis_batch* h) {
     return SIS_OK;
 }
 
+static void ladder_desc_of(const InstDesc& D, sis_ladder_desc* d) {
+    d->n = D.n; d->R = D.R; d->ladder = D.ladder; d->improve = D.improve; d->cool_period = D.cool_period;
+    d->spe = D.spe; d->sweeps_total = D.sweeps_total; d->sweeps_done = D.sweeps_done;
+    d->key0 = D.key0; d->key1 = D.key1; d->interval = D.interval;
+    d->shrink = D.shrink; d->target_acc = D.target_acc; d->scale = D.scale; d->Lf = D.Lf;
+    d->L = D.L; d->L_arch = D.L_arch; d->L_box_arch = D.L_box_arch;
+    d->arch_valid = D.arch_valid; d->shrinks = D.shrinks; d->feasible = D.feasible;
+    d->fp64_rejects = D.fp64_rejects; d->ex_tried = D.ex_tried; d->ex_acc = D.ex_acc; d->certs = D.certs;
+}
+
+int sis_batch_status(sis_batch* h, int k, sis_ladder_desc* d) {
+    if (!h || !d || k < 0 || k >= (int)h->b.specs.size()) return fail(SIS_EINVAL, "bad batch handle or instance");
+    ladder_desc_of(h->b.h_pin[k], d);
+    return SIS_OK;
+}
+
 int sis_dev_batch_state(sis_batch* h, int k, sis_ladder_desc* d, sis_rep_state* reps, sis_slot_state* slots,
                         int* rep_of_slot, float* xycs, float* arch) {
     if (!h || k < 0 || k >= (int)h->b.specs.size()) return fail(SIS_EINVAL, "bad batch handle or instance");
@@ -1100,15 +1202,7 @@ int sis_dev_batch_state(sis_batch* h, int k, sis_ladder_desc* d, sis_rep_state* 
     if (rep_of_slot) SIS_HIPCHECK(hipMemcpy(rep_of_slot, b.d_ros + D.rep_off, sizeof(int) * R, hipMemcpyDeviceToHost));
     if (xycs) SIS_HIPCHECK(hipMemcpy(xycs, b.d_state + D.state_off, sizeof(float4) * R * n, hipMemcpyDeviceToHost));
     if (arch) SIS_HIPCHECK(hipMemcpy(arch, b.d_arch + D.arch_off, sizeof(float4) * n, hipMemcpyDeviceToHost));
-    if (d) {
-        d->n = D.n; d->R = D.R; d->ladder = D.ladder; d->improve = D.improve; d->cool_period = D.cool_period;
-        d->spe = D.spe; d->sweeps_total = D.sweeps_total; d->sweeps_done = D.sweeps_done;
-        d->key0 = D.key0; d->key1 = D.key1; d->interval = D.interval;
-        d->shrink = D.shrink; d->target_acc = D.target_acc; d->scale = D.scale; d->Lf = D.Lf;
-        d->L = D.L; d->L_arch = D.L_arch; d->L_box_arch = D.L_box_arch;
-        d->arch_valid = D.arch_valid; d->shrinks = D.shrinks; d->feasible = D.feasible;
-        d->fp64_rejects = D.fp64_rejects; d->ex_tried = D.ex_tried; d->ex_acc = D.ex_acc; d->certs = D.certs;
-    }
+    if (d) ladder_desc_of(D, d);
     return SIS_OK;
 }
 

@@ -171,6 +171,7 @@ def lib() -> ctypes.CDLL:
     L.sis_dev_ladder.argtypes = [ctypes.POINTER(SisLadderDesc), vp, vp, ip, fp, fp]
     L.sis_dev_batch_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SisLadderDesc), vp, vp, ip,
                                       fp, fp]
+    L.sis_batch_status.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(SisLadderDesc)]
     L.sis_dev_launch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, fp, ctypes.c_float, ctypes.c_int,
                                  fp, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, fp, ip,
@@ -353,7 +354,9 @@ class BatchSession:
             _err(lib().sis_last_error_code())
 
     def run(self, intervals: int, mode: int = 0):
-        """Returns (sweep_kernel_ms, sweep_launches, total_ms)."""
+        """Returns (sweep_kernel_ms, sweep_launches, total_ms): in plain mode
+        the HIP-event time and number of the sweep launches timed (every
+        eighth launch of the call)."""
         sm, tm = ctypes.c_double(), ctypes.c_double()
         nl = ctypes.c_int64()
         rc = lib().sis_batch_run(self.h, intervals, mode, ctypes.byref(sm), ctypes.byref(nl), ctypes.byref(tm))
@@ -372,6 +375,15 @@ class BatchSession:
         if rc:
             _err(rc)
         return [_result(res[j], outs[j]) for j in range(self.k)]
+
+    def status(self, k: int = 0) -> dict:
+        """Descriptor fields of instance k as the host last read them (no
+        device copy): after a plain run, what the last exchange step wrote."""
+        d = SisLadderDesc()
+        rc = lib().sis_batch_status(self.h, k, ctypes.byref(d))
+        if rc:
+            _err(rc)
+        return desc_dict(d)
 
     def state(self, k: int = 0) -> dict:
         """Device state of instance k: descriptor fields, replica and slot records,

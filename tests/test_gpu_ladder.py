@@ -283,6 +283,8 @@ def test_batch_trajectory_matches_model(sis, oracle, case):
         say("trajectory", case, iv)
         b.run(1, 0)
         dev = b.state(0)
+        # the host's status read back after the interval is the device's descriptor
+        assert b.status(0) == dev["desc"], iv
         st, reps = _model_interval(oracle, s, w, spsh, p.certify_every)
         # energies and accept counts come from the sweeps; the states are
         # compared after the exchange step, which may restart replicas
