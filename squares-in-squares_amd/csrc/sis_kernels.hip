@@ -1965,13 +1965,21 @@ __device__ bool overlap_free64(const double* x, const double* y, const double* c
         // few squares: every pair i < j directly, with the arguments total64
         // uses, spread evenly over the threads (flat index over the n x n
         // grid) and without the grid's passes and barriers
+        // (the verdict through the LDS flag, as below: __syncthreads_or
+        // would reserve static LDS in every kernel that can reach here)
+        if (tid == 0) *zg.flag = 0;
+        __syncthreads();
         bool bad = false;
         for (int i = tid; i < n; i += NT) bad |= wall64(x[i], y[i], c[i], s[i], L) > 0.0;
         for (int p = tid; p < n * n; p += NT) {
             const int i = p / n, j = p - i * n;
             if (j > i) bad |= pair64(x[j] - x[i], y[j] - y[i], c[i], s[i], c[j], s[j]) > 0.0;
         }
-        return __syncthreads_or(bad) == 0;
+        if (bad) *zg.flag = 1;
+        __syncthreads();
+        const bool ok = *zg.flag == 0;
+        __syncthreads();
+        return ok;
     }
     const int mc = zero_grid_side(L);
     const int ncell = mc * mc;
