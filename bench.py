@@ -207,6 +207,32 @@ def cpu_c1(sis):
     }
 
 
+def c1_gpu_leg(sis):
+    """The CPU configuration (n=11, L0=5, centres U[0.5, 4.5]^2, angles
+    U[0, pi/2), 8 replicas on 1e-4..1e-1, 2e5 sweeps, exchange every 10, seed
+    42) on the GPU, in the plain form, under its own rule (L -= 1e-4 on a
+    zero-overlap configuration) and under the reference's improvement path
+    with the 200-interval reheating cycle (the `packing` CLI's search); each
+    result certified exactly."""
+    import numpy as np
+
+    out = {}
+    for name, improve, cool in (("shrink_on_zero_overlap", 0, 0), ("improvement_path", 1, 200)):
+        t0 = time.perf_counter()
+        r = sis.solve(11, 5.0, None, replicas=8, sweeps=200000, sweeps_per_exchange=10, tmin=1e-4, tmax=1e-1,
+                      shrink=1e-4, rng_seed=42, init_mode=1, improve=improve, cool_period=cool, polish_iters=200)
+        wall = time.perf_counter() - t0
+        try:
+            dil, L50 = sis.export_certify(np.asarray(r.q))
+        except Exception as e:  # reported, not hidden
+            dil, L50 = None, f"not certified: {e}"
+        out[name] = {"certified_L": L50, "dilation": dil, "L_cert": r.L_cert, "shrinks": r.shrinks,
+                     "seconds": wall, "proposals_per_s": r.proposals / wall}
+    out["grid_L"] = 4.0
+    out["reference_recorded_target_L"] = 3.890074968  # cuda/results/raw11.json
+    return out
+
+
 def precision_leg(sis, sweeps: int = 10000):
     """n = 500 and 1000 started from the grid of side ceil(sqrt(n)) with
     jitter U(+-0.02) on the centres and U(+-0.01 rad) on the angles, 128
@@ -672,6 +698,7 @@ def main():
         line["precision"] = precision_leg(sis, a.precision_sweeps)
     if not a.no_reference_seeds:
         line["reference_seeds"] = reference_seed_leg(sis)
+        line["cpu_config_on_gpu"] = c1_gpu_leg(sis)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         line["cpu_baseline"]["c1_exact"] = cpu_c1(sis)
