@@ -842,16 +842,39 @@ __device__ __forceinline__ MoverRec load_rec(const TileLds& t, int j) {
     return r;
 }
 
+// 2^x of the Metropolis test without a branch: accept_exp2's value for
+// x >= -100 (the same operations on the same argument) and 0 below; x is
+// clamped to [-127, 0] first, which changes nothing where the value is used
+// (the test is made only when the energy rises, so x < 0) and keeps the
+// integer conversion in range elsewhere.
+__device__ __forceinline__ float accept_exp2_sel(float x) {
+    const float xc = fminf(fmaxf(x, -127.0f), 0.0f);
+    const float k = rintf(xc);
+    const float f = xc - k;
+    float p = 0x1.444p-13f;
+    p = __builtin_fmaf(p, f, 0x1.5f48cp-10f);
+    p = __builtin_fmaf(p, f, 0x1.3b2a1cp-7f);
+    p = __builtin_fmaf(p, f, 0x1.c6aeccp-5f);
+    p = __builtin_fmaf(p, f, 0x1.ebfbep-3f);
+    p = __builtin_fmaf(p, f, 0x1.62e43p-1f);
+    p = __builtin_fmaf(p, f, 1.0f);
+    const float v = ldexpf(p, (int)k);
+    return x >= -100.0f ? v : 0.0f;
+}
+
 // One level of a one-wavefront tile with mover records when all its movers
-// fit the wavefront at once: lane l serves record a + (l >> lsh) as its
-// (l & (lpi - 1))-th lane.  The arithmetic, the order of the four partial
-// sums and the Metropolis test are those of evaluate(); a mover rejected
-// before the stencil (or a lane past the level's end) runs the same
-// instructions with its contributions masked, so the level has no divergent
-// exit.  Counts are 32-bit per lane.
-template <bool kNear>
-__device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, bool act, int lpi, int sub,
-                                          unsigned& cC, unsigned& cP) {
+// fit the wavefront at once, for 2^LSH lanes per mover: lane l serves record
+// a + (l >> LSH) as its (l & (lpi - 1))-th lane.  The arithmetic, the order
+// of the four partial sums and the Metropolis test are those of evaluate();
+// a mover rejected before the stencil (or a lane past the level's end) runs
+// the same instructions with its contributions masked, so the level has no
+// divergent exit.  The lane count is a template argument so that the loop
+// and the partial-sum order carry no run-time branches, and the Metropolis
+// test is evaluated without one.  Counts are 32-bit per lane.
+template <int LSH, bool kNear>
+__device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, bool act, int sub, unsigned& cC,
+                                          unsigned& cP) {
+    constexpr int lpi = 1 << LSH;
     const bool ok = act && __uint_as_float(r.w1.z) != kNoMove;
     const float px = __uint_as_float(r.w0.x), py = __uint_as_float(r.w0.y);
     const float pc = __uint_as_float(r.w0.z), ps = __uint_as_float(r.w0.w);
@@ -874,7 +897,7 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
         return other ? en - eo : 0.0f;
     };
     float sum;
-    if (lpi >= 8) {
+    if constexpr (lpi >= 8) {
         // rounds of lpi candidates, f = round + sub; partial k (f = k mod 4)
         // folds lanes k, k + 4 (k + 8, k + 12) of each round into lane k
         float sa = 0.0f;
@@ -883,7 +906,7 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
             const int k = sc.at(min(f, last));
             const float d = delta(t.spos[k], k, f <= last);
             const float d4 = row_up<4>(d);
-            if (lpi == 8) {
+            if constexpr (lpi == 8) {
                 sa = (sa + d) + d4;
             } else {
                 const float d8 = row_up<8>(d), d12 = row_up<12>(d);
@@ -892,38 +915,54 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
         }
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
-    } else {
-        // f = sub + lpi j; partial f mod 4 of step j: all four by j (lpi 1),
-        // sub or sub + 2 by the parity of j (lpi 2), always sub (lpi 4)
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        int j = 0;
-        for (int f = sub; f <= last; f += lpi, ++j) {
+    } else if constexpr (lpi == 4) {
+        // f = sub + 4 j: always partial sub
+        float s0 = 0.0f;
+        for (int f = sub; f <= last; f += 4) {
             const int k = sc.at(f);
-            const float d = delta(t.spos[k], k, true);
-            const int slot = lpi == 1 ? (j & 3) : lpi == 2 ? 2 * (j & 1) : 0;
-            if (slot == 0)
-                s0 += d;
-            else if (slot == 1)
-                s1 += d;
-            else if (slot == 2)
-                s2 += d;
-            else
-                s3 += d;
+            s0 += delta(t.spos[k], k, true);
         }
-        if (lpi == 1) {
-            sum = (s0 + s1) + (s2 + s3);
-        } else if (lpi == 2) {
-            sum = (s0 + quad_xor1(s0)) + (s2 + quad_xor1(s2));
-        } else {
-            const float h = s0 + quad_xor1(s0);
-            sum = h + quad_xor2(h);
+        const float h = s0 + quad_xor1(s0);
+        sum = h + quad_xor2(h);
+    } else if constexpr (lpi == 2) {
+        // f = sub + 2 j: partial sub for even j, sub + 2 for odd j
+        float s0 = 0.0f, s2 = 0.0f;
+        for (int f = sub; f <= last; f += 4) {
+            const int k = sc.at(f);
+            s0 += delta(t.spos[k], k, true);
+            if (f + 2 <= last) {
+                const int k2 = sc.at(f + 2);
+                s2 += delta(t.spos[k2], k2, true);
+            }
         }
+        sum = (s0 + quad_xor1(s0)) + (s2 + quad_xor1(s2));
+    } else {
+        // one lane: partial f mod 4 of candidate f
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        for (int f = 0; f <= last; f += 4) {
+            const int k0 = sc.at(f);
+            s0 += delta(t.spos[k0], k0, true);
+            if (f + 1 <= last) {
+                const int k1 = sc.at(f + 1);
+                s1 += delta(t.spos[k1], k1, true);
+            }
+            if (f + 2 <= last) {
+                const int k2 = sc.at(f + 2);
+                s2 += delta(t.spos[k2], k2, true);
+            }
+            if (f + 3 <= last) {
+                const int k3 = sc.at(f + 3);
+                s3 += delta(t.spos[k3], k3, true);
+            }
+        }
+        sum = (s0 + s1) + (s2 + s3);
     }
     const bool lead = ok && sub == 0;
     cC += lead ? (unsigned)last : 0u;
     cP += near;
     const float dE = __uint_as_float(r.w2.x) + sum;
-    if (lead && (dE <= 0.0f || __uint_as_float(r.w2.y) < accept_exp2(-dE * __uint_as_float(r.w2.z)))) {
+    const float pacc = accept_exp2_sel(-dE * __uint_as_float(r.w2.z));
+    if (lead && (dE <= 0.0f || __uint_as_float(r.w2.y) < pacc)) {
         const float4 np = make_float4(nx, ny, nc, ns);
         t.pos[r.w2.w & 0xFFFFu] = np;
         t.spos[self] = np;
@@ -931,13 +970,30 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
     }
 }
 
+// One level of mover records with the lane count of its mover count.
+template <bool kNear>
+__device__ __forceinline__ void rec_level_any(const TileLds& t, const MoverRec& r, int a, int b, int lsh,
+                                              unsigned& cC, unsigned& cP) {
+    const int lane = threadIdx.x;
+    const bool act = a + (lane >> lsh) < b;
+    const int sub = lane & ((1 << lsh) - 1);
+    switch (lsh) {
+        case 4: rec_level<4, kNear>(t, r, act, sub, cC, cP); break;
+        case 3: rec_level<3, kNear>(t, r, act, sub, cC, cP); break;
+        case 2: rec_level<2, kNear>(t, r, act, sub, cC, cP); break;
+        case 1: rec_level<1, kNear>(t, r, act, sub, cC, cP); break;
+        default: rec_level<0, kNear>(t, r, act, sub, cC, cP); break;
+    }
+}
+
 // The levels of one sweep of a one-wavefront tile with mover records (gs:
 // lane l holds the start of level l, lane kGroups the end of the last).
 // Records do not change during a sweep (a square moves once, and a move never
 // leaves its cell), so the next level's records are read while this level
-// computes; only the candidates' positions wait for the level before.
-// Levels of more than 64 movers (tiles of more than 64 squares) take rounds
-// of the general pass.
+// computes; only the candidates' positions wait for the level before.  Two
+// levels per iteration, alternating between two record registers, so the
+// read-ahead needs no copy.  Levels of more than 64 movers (tiles of more
+// than 64 squares) take rounds of the general pass.
 template <bool kNear>
 __device__ __forceinline__ void rec_levels(const TileLds& t, int n, int gs, unsigned long long& cC,
                                            unsigned long long& cP) {
@@ -945,30 +1001,33 @@ __device__ __forceinline__ void rec_levels(const TileLds& t, int n, int gs, unsi
     unsigned c32 = 0, p32 = 0;
     int a = __builtin_amdgcn_readlane(gs, 0), b = __builtin_amdgcn_readlane(gs, 1);
     int lsh = level_lsh(b - a);
-    MoverRec cur;
-    if (a < b && b - a <= kSweepThreads) cur = load_rec(t, min(a + (lane >> lsh), b - 1));
-    for (int lv = 0; lv < kGroups && a < b; ++lv) {
-        const int cnt = b - a;
-        const int lpi = 1 << lsh, sub = lane & (lpi - 1);
-        const bool act = a + (lane >> lsh) < b;
-        // the next level's bounds and records
+    int lv = 0;
+    MoverRec A, B;
+    if (a < b && b - a <= kSweepThreads) A = load_rec(t, min(a + (lane >> lsh), b - 1));
+    // level lv from `cur`, with the next level's records read into `nxt`
+    auto level = [&](const MoverRec& cur, MoverRec& nxt) {
         const int a2 = b, b2 = lv + 1 < kGroups ? __builtin_amdgcn_readlane(gs, lv + 2) : b;
         const int lsh2 = level_lsh(b2 - a2);
-        if (cnt <= kSweepThreads) {
-            MoverRec nxt = cur;
-            if (a2 < b2 && b2 - a2 <= kSweepThreads) nxt = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
-            rec_level<kNear>(t, cur, act, lpi, sub, c32, p32);
-            cur = nxt;
+        const bool ahead = a2 < b2 && b2 - a2 <= kSweepThreads;
+        if (b - a <= kSweepThreads) {
+            if (ahead) nxt = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
+            rec_level_any<kNear>(t, cur, a, b, lsh, c32, p32);
         } else {
-            const int per = kSweepThreads >> lsh;
+            const int lpi = 1 << lsh, sub = lane & (lpi - 1), per = kSweepThreads >> lsh;
             for (int base = a; base < b; base += per)
                 if (base + (lane >> lsh) < b) propose_rec<kNear>(t, base + (lane >> lsh), n, lpi, sub, cC, cP);
-            if (a2 < b2 && b2 - a2 <= kSweepThreads) cur = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
+            if (ahead) nxt = load_rec(t, min(a2 + (lane >> lsh2), b2 - 1));
         }
         wave_sync();
         a = a2;
         b = b2;
         lsh = lsh2;
+        ++lv;
+    };
+    while (lv < kGroups && a < b) {
+        level(A, B);
+        if (!(lv < kGroups && a < b)) break;
+        level(B, A);
     }
     cC += c32;
     cP += p32;
