@@ -68,6 +68,7 @@ public:
     double device_ms = 0, last_ms = 0, sweep_ms = 0;
     long long sweep_launches = 0;
     bool near_pairs = false;  // some instance asks for near-pair counts (SIS_FLAG_NEAR_PAIRS)
+    bool alone = false;       // every tile planned for an idle device: the one-wavefront-per-SIMD kernels
 
     hipStream_t stream = nullptr, stream2 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
@@ -157,7 +158,7 @@ bool device_loaded(long long replicas);
 InstSpec make_spec(const sis_instance* in, const sis_params* p);
 void fill_result(const FinalRecord& r, const Batch& b, int k, double wall_s, sis_result* out);
 // the sweep kernel of a workgroup size; near_pairs: the form that counts near pairs
-const void* sweep_kernel_ptr(int threads, bool near_pairs = false);
+const void* sweep_kernel_ptr(int threads, bool near_pairs = false, bool alone = false);
 // Launch order of a tile set: tiles of instances with kBigN squares or more
 // (four-wavefront workgroups) first, then the rest, each part heaviest first
 // so the dispatcher hands the long tiles out while compute units free up.

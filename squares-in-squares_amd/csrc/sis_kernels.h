@@ -29,9 +29,11 @@ struct FinalizeOut {
 // threads: kSweepThreads for instances below kBigN squares, kSweepThreadsBig
 // for the others (sweep_threads(n)); every tile of one launch uses the same.
 // near_pairs: also count every proposal's near pairs (SIS_FLAG_NEAR_PAIRS).
+// alone: the tiles are planned for an idle device (Plan::wide); the launch
+// takes the kernel compiled for one wavefront per SIMD.
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
                         InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
-                        int threads, bool near_pairs);
+                        int threads, bool near_pairs, bool alone = false);
 // Probe: the exchange kernel's zero-overlap test and the all-pairs FP64
 // energy of one configuration.
 hipError_t launch_zero64(int n, double L, const float4* q, int* zero, double* e64);

@@ -1573,10 +1573,12 @@ __device__ int setup_lane_squares(const TileLds& t, int n, int G, int M, int m, 
     return gs;
 }
 
-// Four waves per SIMD: at most 128 VGPRs (the tiles under load need the
-// residency more than the kernel needs registers).
-template <int NT, bool kNear>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep(const Tile* __restrict__ tiles,
+// W = 4 waves per SIMD: at most 128 VGPRs (the tiles under load need the
+// residency more than the kernel needs registers).  W = 1 for launches whose
+// tiles are planned for an idle device (one workgroup per compute unit): the
+// registers of a whole SIMD, so nothing spills (the same arithmetic).
+template <int NT, bool kNear, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) void k_sweep(const Tile* __restrict__ tiles,
                                               const InstDesc* __restrict__ insts,
                                               RepState* __restrict__ reps,
                                               const SlotState* __restrict__ slots,
@@ -2679,26 +2681,25 @@ __global__ void k_cellcoord(int count, const float* __restrict__ x, float shift,
 // ---------------------------------------------------------------------------
 // Launch wrappers.
 // ---------------------------------------------------------------------------
+template <int NT, bool kNear, int W>
+static void launch_sweep_t(hipStream_t st, int ntiles, size_t lds, const Tile* tiles, InstDesc* insts,
+                           RepState* reps, const SlotState* slots, float4* state) {
+    hipLaunchKernelGGL((k_sweep<NT, kNear, W>), dim3(ntiles), dim3(NT), lds, st, tiles, insts, reps, slots, state);
+}
+
 hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tiles,
                         InstDesc* insts, RepState* reps, const SlotState* slots, float4* state,
-                        int threads, bool near_pairs) {
-    if (threads == kSweepThreadsBig) {
-        if (near_pairs)
-            hipLaunchKernelGGL((k_sweep<kSweepThreadsBig, true>), dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
-                               tiles, insts, reps, slots, state);
-        else
-            hipLaunchKernelGGL((k_sweep<kSweepThreadsBig, false>), dim3(ntiles), dim3(kSweepThreadsBig), lds, st,
-                               tiles, insts, reps, slots, state);
-    } else if (threads == kSweepThreads) {
-        if (near_pairs)
-            hipLaunchKernelGGL((k_sweep<kSweepThreads, true>), dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
-                               insts, reps, slots, state);
-        else
-            hipLaunchKernelGGL((k_sweep<kSweepThreads, false>), dim3(ntiles), dim3(kSweepThreads), lds, st, tiles,
-                               insts, reps, slots, state);
-    } else {
-        return hipErrorInvalidValue;
-    }
+                        int threads, bool near_pairs, bool alone) {
+    if (threads != kSweepThreadsBig && threads != kSweepThreads) return hipErrorInvalidValue;
+    const bool big = threads == kSweepThreadsBig;
+    if (big && near_pairs && alone) launch_sweep_t<kSweepThreadsBig, true, 1>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (big && near_pairs) launch_sweep_t<kSweepThreadsBig, true, 4>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (big && alone) launch_sweep_t<kSweepThreadsBig, false, 1>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (big) launch_sweep_t<kSweepThreadsBig, false, 4>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (near_pairs && alone) launch_sweep_t<kSweepThreads, true, 1>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (near_pairs) launch_sweep_t<kSweepThreads, true, 4>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else if (alone) launch_sweep_t<kSweepThreads, false, 1>(st, ntiles, lds, tiles, insts, reps, slots, state);
+    else launch_sweep_t<kSweepThreads, false, 4>(st, ntiles, lds, tiles, insts, reps, slots, state);
     return hipGetLastError();
 }
 
@@ -2750,12 +2751,20 @@ hipError_t launch_accept_exp2(hipStream_t st, int count, const float* x, float* 
     return hipGetLastError();
 }
 
-const void* sweep_kernel_ptr(int threads, bool near_pairs) {
+const void* sweep_kernel_ptr(int threads, bool near_pairs, bool alone) {
+    const bool big = threads == kSweepThreadsBig;
+    if (alone) {
+        if (near_pairs)
+            return big ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, true, 1>)
+                       : reinterpret_cast<const void*>(k_sweep<kSweepThreads, true, 1>);
+        return big ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, false, 1>)
+                   : reinterpret_cast<const void*>(k_sweep<kSweepThreads, false, 1>);
+    }
     if (near_pairs)
-        return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, true>)
-                                           : reinterpret_cast<const void*>(k_sweep<kSweepThreads, true>);
-    return threads == kSweepThreadsBig ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, false>)
-                                       : reinterpret_cast<const void*>(k_sweep<kSweepThreads, false>);
+        return big ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, true, 4>)
+                   : reinterpret_cast<const void*>(k_sweep<kSweepThreads, true, 4>);
+    return big ? reinterpret_cast<const void*>(k_sweep<kSweepThreadsBig, false, 4>)
+               : reinterpret_cast<const void*>(k_sweep<kSweepThreads, false, 4>);
 }
 
 }  // namespace sis

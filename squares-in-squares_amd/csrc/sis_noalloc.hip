@@ -136,7 +136,7 @@ int solve_noalloc(const sis_instance* inst, const sis_params* params, sis_result
     const auto t1 = std::chrono::steady_clock::now();
     for (long long k = 0; k < intervals; ++k) {
         NA_CHECK(launch_sweep(nullptr, (int)tiles.size(), pl.lds, d_tiles, d_inst, d_reps, d_slots, d_state, threads,
-                              (spec.p.flags & SIS_FLAG_NEAR_PAIRS) != 0));
+                              (spec.p.flags & SIS_FLAG_NEAR_PAIRS) != 0, pl.wide));
         NA_CHECK(launch_ladder(nullptr, 1, lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch));
     }
     NA_CHECK(hipStreamSynchronize(nullptr));

@@ -56,8 +56,10 @@ int ensure_device(int device) {
     // a sweep tile may use all the LDS a workgroup can hold (160 KB on gfx950)
     for (int threads : {kSweepThreads, kSweepThreadsBig})
         for (bool near_pairs : {false, true})
-            HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads, near_pairs),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)prop.sharedMemPerBlock));
+            for (bool alone : {false, true})
+                HIPCHECK(hipFuncSetAttribute(sweep_kernel_ptr(threads, near_pairs, alone),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)prop.sharedMemPerBlock));
     // the allocation-free solve's device buffer: its address on this device
     if (int rc = noalloc_setup(id)) return rc;
     g_dev.cus = prop.multiProcessorCount;
@@ -419,6 +421,8 @@ int Batch::build(std::vector<InstSpec>&& in) {
     }
     if (nsq > (size_t)INT32_MAX) return fail(SIS_EINVAL, "batch too large");
     n_big = order_tiles(h_tiles, [&](int k) { return specs[k].n; });
+    alone = true;
+    for (const Plan& pl : plans) alone = alone && pl.wide;
     n_sq = nsq; n_rep = nrep; n_arch = narch;
 
     // host initial state, slots, ladders
@@ -535,10 +539,10 @@ int Batch::enqueue_chain(int ci) {
     const Tile* tiles = d_ctiles + c.tile_off;
     if (c.n_big > 0)
         HIPCHECK(launch_sweep(c.st, c.n_big, c.lds_big, tiles, d_inst, d_reps, d_slots, d_state, kSweepThreadsBig,
-                              near_pairs));
+                              near_pairs, alone));
     if (c.ntiles > c.n_big)
         HIPCHECK(launch_sweep(c.st, c.ntiles - c.n_big, c.lds_small, tiles + c.n_big, d_inst, d_reps, d_slots,
-                              d_state, kSweepThreads, near_pairs));
+                              d_state, kSweepThreads, near_pairs, alone));
     HIPCHECK(launch_ladder(c.st, c.ninst, c.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
                            d_clist + c.list_off));
     return SIS_OK;
@@ -558,10 +562,10 @@ int Batch::launch_tiles() {
     }
     if (n_big > 0)
         HIPCHECK(launch_sweep(stream, n_big, lds_big, d_tiles, d_inst, d_reps, d_slots, d_state,
-                              kSweepThreadsBig, near_pairs));
+                              kSweepThreadsBig, near_pairs, alone));
     if (nt > n_big)
         HIPCHECK(launch_sweep(both ? stream2 : stream, nt - n_big, lds_small, d_tiles + n_big, d_inst, d_reps,
-                              d_slots, d_state, kSweepThreads, near_pairs));
+                              d_slots, d_state, kSweepThreads, near_pairs, alone));
     if (both) {
         HIPCHECK(hipEventRecord(ev_join, stream2));
         HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
@@ -904,7 +908,8 @@ int sis_occupancy(const sis_instance* inst, const sis_params* params, int* block
     if ((rc = plan_instance(s, s.p.replicas, pl))) return rc;
     const int threads = sweep_threads(s.n);
     int nb = 0;
-    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_kernel_ptr(threads), threads, pl.lds));
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_kernel_ptr(threads, false, pl.wide), threads,
+                                                          pl.lds));
     if (blocks_per_cu) *blocks_per_cu = nb;
     if (replicas_per_block) *replicas_per_block = pl.G;
     return SIS_OK;

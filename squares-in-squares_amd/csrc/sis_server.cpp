@@ -130,6 +130,7 @@ struct Lane {
     bool dirty = true;
     bool planned_loaded = false;      // device load the tiles were planned for
     bool near_pairs = false;          // some active request counts near pairs (SIS_FLAG_NEAR_PAIRS)
+    bool alone = false;               // every tile planned for an idle device (one-wavefront-per-SIMD kernels)
     std::vector<Req*> active;         // resident, in list order
     std::vector<Tile> tiles;          // four-wavefront tiles first (n_big of them)
     std::vector<int> list;            // descriptor slots of `active`
@@ -341,6 +342,7 @@ int Server::launch_round(Lane& ln) {
         ln.list.clear();
         ln.lds_big = ln.lds_small = ln.lds_ladder = 0;
         ln.near_pairs = false;
+        ln.alone = true;
         for (Req* r : ln.active) {
             ln.near_pairs = ln.near_pairs || (r->spec.p.flags & SIS_FLAG_NEAR_PAIRS) != 0;
             Plan pl;
@@ -349,6 +351,7 @@ int Server::launch_round(Lane& ln) {
                 return SIS_EDEVICE;
             }
             r->plan.G = pl.G;  // grouping never changes a replica's arithmetic
+            ln.alone = ln.alone && pl.wide;
             size_t& lds = r->spec.n >= kBigN ? ln.lds_big : ln.lds_small;
             lds = std::max(lds, pl.lds);
             ln.lds_ladder = std::max(ln.lds_ladder, ladder_lds_bytes(r->spec.n, r->spec.L, r->spec.p.replicas));
@@ -378,10 +381,10 @@ int Server::launch_round(Lane& ln) {
     for (int k = 0; k < ipr; ++k) {
         if (ln.n_big > 0)
             SRV_CHECK(launch_sweep(ln.stream, ln.n_big, ln.lds_big, ln.d_tiles, d_inst, d_reps, d_slots, d_state,
-                                   kSweepThreadsBig, ln.near_pairs));
+                                   kSweepThreadsBig, ln.near_pairs, ln.alone));
         if (nt > ln.n_big)
             SRV_CHECK(launch_sweep(ln.stream, nt - ln.n_big, ln.lds_small, ln.d_tiles + ln.n_big, d_inst, d_reps,
-                                   d_slots, d_state, kSweepThreads, ln.near_pairs));
+                                   d_slots, d_state, kSweepThreads, ln.near_pairs, ln.alone));
         SRV_CHECK(launch_ladder(ln.stream, (int)ln.list.size(), ln.lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch,
                                 ln.d_list));
     }
