@@ -667,7 +667,7 @@ def main():
             "device": dev, "compute_units": cus,
         },
         "roofline": {
-            "bound": "valu", "kernel": "k_sweep<64>",
+            "bound": "valu", "kernel": "k_sweep<64, false, 1> (one-wavefront tiles; compiled for one wave per SIMD)",
             "roof": "FP32 vector ALU peak (157.3 TFLOP/s); no dense contraction on this path, so no matrix-core "
                     "work; HBM traffic is far below its roof; at this load (one wavefront per compute unit) the "
                     "launch is latency-bound",
