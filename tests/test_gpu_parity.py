@@ -466,6 +466,36 @@ def test_batch_equals_alone(sis):
             assert a.fields[k] == m.fields[k], k
 
 
+def test_plain_runs_stop_at_the_budget_and_read_status(sis):
+    """The plain schedule reads every instance's status after every interval:
+    a run longer than the budget stops at it, a run after the end launches
+    nothing, the status the host read equals the device's descriptor, every
+    eighth launch of a call is timed, and stopping early changes no result
+    (same packing as the graph schedule run to the end)."""
+    insts = [(13, 5.0, None), (37, 8.0, None)]
+    ps = [sis.default_params(replicas=8, sweeps=s, rng_seed=seed, init_mode=1, tmin=1e-4, tmax=3e-2, improve=1)
+          for s, seed in ((120, 11), (45, 12))]  # 12 and 5 intervals of 10 sweeps
+    say("plain early stop")
+    b = sis.BatchSession(insts, ps)
+    sm, launches, _ = b.run(3, mode=0)
+    assert launches == 1 and sm > 0  # intervals 0 of the call timed (every eighth)
+    st = [b.status(k) for k in range(2)]
+    assert [d["sweeps_done"] for d in st] == [30, 30] and [d["interval"] for d in st] == [3, 3]
+    assert st == [b.state(k)["desc"] for k in range(2)]
+    _, launches, _ = b.run(100, mode=0)  # 9 more intervals end both budgets
+    assert launches == 2  # calls' intervals 0 and 8 timed
+    st = [b.status(k) for k in range(2)]
+    assert [d["sweeps_done"] for d in st] == [120, 45] and [d["interval"] for d in st] == [12, 5]
+    assert st == [b.state(k)["desc"] for k in range(2)]
+    _, launches, _ = b.run(5, mode=0)
+    assert launches == 0
+    res = b.finalize()
+    b.close()
+    ref = sis.solve_batch(insts, ps)
+    for r, m in zip(res, ref):
+        assert np.array_equal(r.q, m.q) and r.L_cert == m.L_cert and r.proposals == m.proposals
+
+
 def test_near_pair_counting_changes_nothing_else(sis):
     """SIS_FLAG_NEAR_PAIRS only adds counting: the same packing, counters and
     certified side with and without it (plain, batched with a four-wavefront
