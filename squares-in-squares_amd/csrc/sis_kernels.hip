@@ -1060,15 +1060,31 @@ __device__ void tile_energy(const TileLds& t, int n, int G, int M, int m, float 
         const float4* pg = t.pos + g * n;
         float ei = wall32(p.x, p.y, p.z, p.w, t.Lr[g]);
         float md = 0.0f;
-        for (int f = 0; f < sc.total; ++f) {
-            const int k = sc.at(f);
-            const int j = t.sorted[k];
-            // with the cell-ordered copy the position does not wait for the index
-            const float4 v = t.spos ? t.spos[k] : pg[j];
-            if (j <= i) continue;
-            const float d2 = depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
-            ei += d2;
-            if (kMaxDepth) md = fmaxf(md, d2);
+        // four candidates' index and position reads in flight at once, then
+        // their terms added in candidate order
+        for (int f0 = 0; f0 < sc.total; f0 += 4) {
+            int kk[4], jj[4];
+            float4 vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kk[u] = sc.at(min(f0 + u, sc.total - 1));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                jj[u] = t.sorted[kk[u]];
+                // with the cell-ordered copy the position does not wait for the index
+                if (t.spos) vv[u] = t.spos[kk[u]];
+            }
+            if (!t.spos) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) vv[u] = pg[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (f0 + u >= sc.total || jj[u] <= i) continue;
+                const float4 v = vv[u];
+                const float d2 = depth2(v.x - p.x, v.y - p.y, p.z, p.w, v.z, v.w);
+                ei += d2;
+                if (kMaxDepth) md = fmaxf(md, d2);
+            }
         }
         e[q] = ei;
         if (kMaxDepth && md > 0.0f) atomicMax(&t.maxd[g], __float_as_uint(md));
@@ -1383,23 +1399,39 @@ __device__ void tile_certify(const TileLds& t, int n, int G, int M, RepState* rs
         const float4* pg = t.pos + (size_t)g * n;
         double ci = 0.0, si = 0.0, lam = 1.0;
         bool have = false;
-        for (int f = 0; f < sc.total; ++f) {
-            const int k = sc.at(f);
-            const int j = t.sorted[k];
-            if (j <= i) continue;
-            const float4 v = t.spos ? t.spos[k] : pg[j];
-            const float fx = v.x - p.x, fy = v.y - p.y;
-            if (!(__builtin_fmaf(fx, fx, fy * fy) < 2.06f)) continue;
-            if (!(sat_gap(fx, fy, p.z, p.w, v.z, v.w) > kCertPairGap)) continue;
-            const double dx = (double)p.x - (double)v.x, dy = (double)p.y - (double)v.y;
-            if (dx * dx + dy * dy >= kCertNear2) continue;
-            if (!have) {
-                unit64(p.z, p.w, ci, si);
-                have = true;
+        // four candidates' reads in flight at once (the maximum does not
+        // depend on the order)
+        for (int f0 = 0; f0 < sc.total; f0 += 4) {
+            int kk[4], jj[4];
+            float4 vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kk[u] = sc.at(min(f0 + u, sc.total - 1));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                jj[u] = t.sorted[kk[u]];
+                if (t.spos) vv[u] = t.spos[kk[u]];
             }
-            double cj, sj;
-            unit64(v.z, v.w, cj, sj);
-            lam = fmax(lam, cert_need(dx, dy, ci, si, cj, sj));
+            if (!t.spos) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) vv[u] = pg[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (f0 + u >= sc.total || jj[u] <= i) continue;
+                const float4 v = vv[u];
+                const float fx = v.x - p.x, fy = v.y - p.y;
+                if (!(__builtin_fmaf(fx, fx, fy * fy) < 2.06f)) continue;
+                if (!(sat_gap(fx, fy, p.z, p.w, v.z, v.w) > kCertPairGap)) continue;
+                const double dx = (double)p.x - (double)v.x, dy = (double)p.y - (double)v.y;
+                if (dx * dx + dy * dy >= kCertNear2) continue;
+                if (!have) {
+                    unit64(p.z, p.w, ci, si);
+                    have = true;
+                }
+                double cj, sj;
+                unit64(v.z, v.w, cj, sj);
+                lam = fmax(lam, cert_need(dx, dy, ci, si, cj, sj));
+            }
         }
         if (lam > 1.0) atomicMax(acc + kStride * g, dkey(lam));
     }
