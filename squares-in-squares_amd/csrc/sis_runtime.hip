@@ -457,6 +457,7 @@ int Batch::build(std::vector<InstSpec>&& in) {
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_seq), sizeof(unsigned) * ni,
                            hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_seq), h_seq, 0));
+    std::copy(h_inst.begin(), h_inst.end(), h_pin);  // the status before any run
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHECK(hipEventCreate(&ev0));

@@ -477,6 +477,7 @@ def test_plain_runs_stop_at_the_budget_and_read_status(sis):
           for s, seed in ((120, 11), (45, 12))]  # 12 and 5 intervals of 10 sweeps
     say("plain early stop")
     b = sis.BatchSession(insts, ps)
+    assert [b.status(k) for k in range(2)] == [b.state(k)["desc"] for k in range(2)]  # before any run
     sm, launches, _ = b.run(3, mode=0)
     assert launches == 1 and sm > 0  # intervals 0 of the call timed (every eighth)
     st = [b.status(k) for k in range(2)]
