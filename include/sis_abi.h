@@ -205,8 +205,9 @@ int sis_solve_batch(const sis_instance* insts, int count, const sis_params* para
 
 /* Step-wise control of resident instances: the reference's epoch loop
  * (cuda/packing.cu:823-853) opened up so a caller can run, time and finish
- * it in pieces.  mode 0 = plain (host synchronisation after every interval),
- * mode 1 = HIP graph (no host synchronisation).  intervals <= 0 runs what is
+ * it in pieces.  mode 0 = plain (the host reads every interval's status,
+ * with the next interval already queued), mode 1 = HIP graph (no host
+ * synchronisation).  intervals <= 0 runs what is
  * left of the budget; a plain run stops early once every instance has run its
  * sweeps.  sweep_ms is the HIP-event time of the sweep launches that were
  * timed and sweep_launches their number (plain mode: every eighth launch of
@@ -220,7 +221,7 @@ int sis_batch_finalize(sis_batch* b, sis_result* out);
 int sis_batch_destroy(sis_batch* b);
 /* The status of instance k as the host last read it, without a device copy:
  * in mode 0 each exchange step writes its instance's descriptor into mapped
- * host memory, which the plain loop reads after every interval (the
+ * host memory, which the plain loop reads for every interval (the
  * reference's per-epoch blocking read of the chains' best sides,
  * cuda/packing.cu:831); a mode 1 run copies it once at its end. */
 struct sis_ladder_desc;  /* declared with the parity probes below */

@@ -580,58 +580,76 @@ int Batch::enqueue_interval() {
     return SIS_OK;
 }
 
-// Plain schedule: one interval at a time, and the host learns every
-// instance's progress after every interval before it starts the next (the
-// reference's per-epoch blocking copy, cuda/packing.cu:831).  The exchange
-// step writes each instance's descriptor into the mapped host mirror and then
-// its interval count; the host polls the counts, so it sees the step's end
-// as soon as the step has written it and its next launch is queued while the
-// step finishes.  Progress is counted on the host from the counts (the mirror
-// is read back from the device once the run has ended).  The sweep kernel of
-// every kSweepTimingEvery-th interval is timed with its own event pair on the
+// Plain schedule: one interval per launch pair, and the host reads every
+// instance's status after every interval (the reference's per-epoch blocking
+// read, cuda/packing.cu:831).  The exchange step writes each instance's
+// descriptor into the mapped host mirror and then its interval count; the
+// host polls the counts.  Every decision of the epoch loop is made on the
+// device and the number of sweeps each interval runs is known in advance, so
+// the host keeps one interval queued ahead of the one whose status it waits
+// for: the device never idles on the host's round trip, and the host still
+// sees every interval's end, one interval later.  The sweep kernel of every
+// kSweepTimingEvery-th interval is timed with its own event pair on the
 // launch stream.
 int Batch::run_plain(long long intervals) {
     const int ni = (int)specs.size();
     HIPCHECK(hipMemcpyAsync(h_pin, d_inst, sizeof(InstDesc) * ni, hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    std::vector<unsigned> want(ni), iv_of(ni);
-    std::vector<long long> done(ni);
+    std::vector<unsigned> want(ni), iv_of(ni), pend(ni);
+    std::vector<long long> done(ni), total(ni), spe(ni);
     for (int k = 0; k < ni; ++k) {
         iv_of[k] = h_pin[k].interval;
         done[k] = h_pin[k].sweeps_done;
+        total[k] = h_pin[k].sweeps_total;
+        spe[k] = h_pin[k].spe;
         __atomic_store_n(h_seq + k, iv_of[k], __ATOMIC_RELEASE);
     }
-    HIPCHECK(hipEventRecord(ev0, stream));
-    for (long long iv = 0; iv < intervals; ++iv) {
-        // stop once every instance has run its sweeps
-        bool live = false;
-        for (int k = 0; k < ni; ++k) {
-            const bool l = done[k] < h_pin[k].sweeps_total;
-            want[k] = l ? iv_of[k] + 1u : 0u;
-            live = live || l;
-        }
-        if (!live) break;
-        const bool timed = iv % kSweepTimingEvery == 0;
-        if (timed) HIPCHECK(hipEventRecord(evs0, stream));
-        if (int rc = launch_tiles()) return rc;
-        if (timed) HIPCHECK(hipEventRecord(evs1, stream));
-        HIPCHECK(launch_ladder(stream, ni, lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch, nullptr,
-                               d_pin, d_seq));
-        if (int rc = wait_status(want)) return rc;
-        for (int k = 0; k < ni; ++k)
-            if (want[k]) {
-                iv_of[k] = want[k];
-                done[k] += std::min<long long>(h_pin[k].spe, h_pin[k].sweeps_total - done[k]);
-            }
-        if (timed) {
+    bool have_pend = false, pend_timed = false;
+    // the status of the interval launched before the last one, and its timing
+    auto settle = [&]() -> int {
+        if (int rc = wait_status(pend)) return rc;
+        if (pend_timed) {
             HIPCHECK(hipEventSynchronize(evs1));
             float sms = 0;
             HIPCHECK(hipEventElapsedTime(&sms, evs0, evs1));
             sweep_ms += sms;
             sweep_launches += 1;
         }
+        have_pend = false;
+        return SIS_OK;
+    };
+    HIPCHECK(hipEventRecord(ev0, stream));
+    for (long long iv = 0; iv < intervals; ++iv) {
+        // stop once every instance has run its sweeps
+        bool live = false;
+        for (int k = 0; k < ni; ++k) {
+            const bool l = done[k] < total[k];
+            want[k] = l ? iv_of[k] + 1u : 0u;
+            live = live || l;
+        }
+        if (!live) break;
+        // one event pair: the previous timed interval has been read before
+        // the pair is recorded again (kSweepTimingEvery >= 2)
+        const bool timed = iv % kSweepTimingEvery == 0;
+        if (timed) HIPCHECK(hipEventRecord(evs0, stream));
+        if (int rc = launch_tiles()) return rc;
+        if (timed) HIPCHECK(hipEventRecord(evs1, stream));
+        HIPCHECK(launch_ladder(stream, ni, lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch, nullptr,
+                               d_pin, d_seq));
+        for (int k = 0; k < ni; ++k)
+            if (want[k]) {
+                iv_of[k] = want[k];
+                done[k] += std::min(spe[k], total[k] - done[k]);
+            }
         intervals_done += 1;
+        if (have_pend)
+            if (int rc = settle()) return rc;
+        pend.swap(want);
+        pend_timed = timed;
+        have_pend = true;
     }
+    if (have_pend)
+        if (int rc = settle()) return rc;
     HIPCHECK(hipEventRecord(ev1, stream));
     HIPCHECK(hipEventSynchronize(ev1));
     float ms = 0;
@@ -644,16 +662,18 @@ int Batch::run_plain(long long intervals) {
 }
 
 // Wait until every instance with want[k] != 0 has published interval count
-// want[k].  The stream is queried every few hundred polls; an error ends the
+// want[k] or a later one (the interval queued behind it may have ended
+// too).  The stream is queried every few hundred polls; an error ends the
 // wait with SIS_EDEVICE.  The stream's completion and the count written by
 // its last kernel reach the host by different paths, so after the stream has
 // drained the count is polled for a while longer, and the device's own
 // descriptors decide in the end.
 int Batch::wait_status(const std::vector<unsigned>& want) {
     const int ni = (int)want.size();
+    auto reached = [](unsigned have, unsigned w) { return (int)(have - w) >= 0; };
     auto published = [&]() {
         for (int k = 0; k < ni; ++k)
-            if (want[k] != 0u && __atomic_load_n(h_seq + k, __ATOMIC_ACQUIRE) != want[k]) return false;
+            if (want[k] != 0u && !reached(__atomic_load_n(h_seq + k, __ATOMIC_ACQUIRE), want[k])) return false;
         return true;
     };
     for (unsigned polls = 1;; ++polls) {
@@ -675,7 +695,7 @@ int Batch::wait_status(const std::vector<unsigned>& want) {
     HIPCHECK(hipStreamSynchronize(stream));
     std::string msg;
     for (int k = 0; k < ni; ++k)
-        if (want[k] != 0u && dev[k].interval != want[k])
+        if (want[k] != 0u && !reached(dev[k].interval, want[k]))
             msg += " instance " + std::to_string(k) + ": interval " + std::to_string(dev[k].interval) +
                    " after the step, " + std::to_string(want[k]) + " expected;";
     return msg.empty() ? SIS_OK : fail(SIS_EDEVICE, "exchange step did not advance:" + msg);

@@ -257,7 +257,8 @@ def portfolio(n: int, L: float, q=None, params: SisParams | None = None, jitter:
 
 
 def solve(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Result:
-    """Plain form: one instance, host sync per exchange interval."""
+    """Plain form: one instance, the host reads its status after every exchange
+    interval (the next interval already queued)."""
     p = params if params is not None else default_params(**kw)
     inst, keep = _instance(n, L, q)
     out = np.zeros((n, 3), dtype=np.float64)
