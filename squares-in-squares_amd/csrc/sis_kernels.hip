@@ -2192,9 +2192,14 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     //    the sweep workgroups move into the instance's totals
     unsigned long long my_lc = ~0ull;
     unsigned long long c_acc = 0, c_cert = 0, c_cand = 0, c_near = 0, c_ct = 0, c_ca = 0, c_kick = 0;
+    // slot record `tid`, kept in registers for step 4 (no kernel step before
+    // step 4 writes slot records)
+    SlotState my_slot{};
     for (int r = tid; r < R; r += kLadderThreads) {
         const RepState rs = reps[base + r];
-        kS[r] = slots[base + r].inv_t_log2e;  // slot r
+        const SlotState sr = slots[base + r];
+        kS[r] = sr.inv_t_log2e;  // slot r
+        if (r == tid) my_slot = sr;
         c_acc += rs.acc;
         c_cert += (improve && rs.Lc < kCertNone) ? 1u : 0u;
         if (rs.cand | rs.near_pairs | rs.coll_tried | rs.coll_acc | rs.kicks) {
@@ -2328,13 +2333,13 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     //    of the replica that held the slot, base x e^((rate - target) / 2)
     //    clamped; the box (below the new best side after an improvement); the
     //    next interval's temperature and step sizes (cooling cycle).  The
-    //    record was read in step 1 by this thread, so it comes from the cache.
+    //    thread's first record is the one it read in step 1.
     {
         const float target = d.target_acc;
         double cool, root;
         cool_factor(interval + 1u, d.cool_period, cool, root);
         for (int s = tid; s < R; s += kLadderThreads) {
-            SlotState ss = slots[base + s];
+            SlotState ss = s == tid ? my_slot : slots[base + s];
             const float rate = (float)aS[s] / (float)((long long)n * ran);
             // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
             const float f = accept_exp2((rate - target) * 0.72134752f);

@@ -663,12 +663,16 @@ int Batch::run_plain(long long intervals) {
 
 // Wait until every instance with want[k] != 0 has published interval count
 // want[k] or a later one (the interval queued behind it may have ended
-// too).  The stream is queried every few hundred polls; an error ends the
+// too).  The counts alone are polled for the first kQueryAfterMs: a stream
+// query enqueues a marker whose system-scope release sits between the
+// kernels already queued and stalls the next launch by microseconds.  After
+// that the stream is queried every few hundred polls, and an error ends the
 // wait with SIS_EDEVICE.  The stream's completion and the count written by
 // its last kernel reach the host by different paths, so after the stream has
 // drained the count is polled for a while longer, and the device's own
 // descriptors decide in the end.
 int Batch::wait_status(const std::vector<unsigned>& want) {
+    constexpr int kQueryAfterMs = 50;
     const int ni = (int)want.size();
     auto reached = [](unsigned have, unsigned w) { return (int)(have - w) >= 0; };
     auto published = [&]() {
@@ -676,15 +680,19 @@ int Batch::wait_status(const std::vector<unsigned>& want) {
             if (want[k] != 0u && !reached(__atomic_load_n(h_seq + k, __ATOMIC_ACQUIRE), want[k])) return false;
         return true;
     };
+    const auto w0 = std::chrono::steady_clock::now();
+    bool query = false;
     for (unsigned polls = 1;; ++polls) {
         if (published()) return SIS_OK;
-        if ((polls & 255u) == 0u) {
+        if ((polls & 255u) != 0u) {
+            __builtin_ia32_pause();
+        } else if (!query) {
+            query = std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(kQueryAfterMs);
+        } else {
             const hipError_t q = hipStreamQuery(stream);
             if (q == hipSuccess) break;
             if (q != hipErrorNotReady)
                 return fail(SIS_EDEVICE, std::string("device error: ") + hipGetErrorString(q));
-        } else {
-            __builtin_ia32_pause();
         }
     }
     const auto t0 = std::chrono::steady_clock::now();
