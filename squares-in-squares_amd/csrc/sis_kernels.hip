@@ -175,6 +175,36 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+// Sum and minimum of a value over the 64 lanes of a wavefront by the same
+// DPP scan (lanes a shift or a row mask leaves out take the identity), lane
+// 63's result read back to every lane.  Every lane of the wavefront must be
+// active.  64-bit values move as two 32-bit halves.
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+    return (unsigned)__builtin_amdgcn_readlane(wave_incl_scan((int)v), 63);
+}
+template <int kCtrl, int kRows, bool kMin>
+__device__ __forceinline__ unsigned long long dpp_step_u64(unsigned long long v) {
+    constexpr unsigned long long id = kMin ? ~0ull : 0ull;
+    const unsigned lo =
+        (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)id, (int)(unsigned)v, kCtrl, kRows, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(id >> 32), (int)(unsigned)(v >> 32),
+                                                              kCtrl, kRows, 0xF, false);
+    const unsigned long long o = (unsigned long long)hi << 32 | lo;
+    return kMin ? (o < v ? o : v) : v + o;
+}
+template <bool kMin>
+__device__ __forceinline__ unsigned long long wave_reduce_u64(unsigned long long v) {
+    v = dpp_step_u64<0x111, 0xF, kMin>(v);  // row_shr:1
+    v = dpp_step_u64<0x112, 0xF, kMin>(v);  // row_shr:2
+    v = dpp_step_u64<0x114, 0xF, kMin>(v);  // row_shr:4
+    v = dpp_step_u64<0x118, 0xF, kMin>(v);  // row_shr:8
+    v = dpp_step_u64<0x142, 0xA, kMin>(v);  // row_bcast:15 into rows 1, 3
+    v = dpp_step_u64<0x143, 0xC, kMin>(v);  // row_bcast:31 into rows 2, 3
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return (unsigned long long)hi << 32 | lo;
+}
+
 // Value of the lane N places up inside a row of 16 lanes (DPP row shift).
 template <int N>
 __device__ __forceinline__ float row_up(float v) {
@@ -1835,14 +1865,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) void k_
     // added: the exchange step after every launch moves them into the
     // instance's totals) by one thread, so that the launch makes no atomics
     // on the shared descriptor and waits on no global load at its end.
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        cC += __shfl_xor(cC, off, 64);
-        cP += __shfl_xor(cP, off, 64);
-        ctried += __shfl_xor(ctried, off, 64);
-        caccd += __shfl_xor(caccd, off, 64);
-        kicked += __shfl_xor(kicked, off, 64);
-    }
+    cC = wave_reduce_u64<false>(cC);
+    cP = wave_reduce_u64<false>(cP);
+    ctried = wave_sum_u32(ctried);
+    caccd = wave_sum_u32(caccd);
+    kicked = wave_sum_u32(kicked);
     if constexpr (NT > 64) {
         // wavefront totals through the collective-move scratch (free now)
         unsigned long long* red = reinterpret_cast<unsigned long long*>(t.cw);
@@ -2221,20 +2248,16 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         if (improve) my_lc = min(my_lc, (unsigned long long)__double_as_longlong(rs.Lc));
     }
     if (improve) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) my_lc = min(my_lc, (unsigned long long)__shfl_xor(my_lc, off, 64));
+        my_lc = wave_reduce_u64<true>(my_lc);
         if ((tid & 63) == 0) atomicMin(&best_lc, my_lc);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        c_acc += __shfl_xor(c_acc, off, 64);
-        c_cert += __shfl_xor(c_cert, off, 64);
-        c_cand += __shfl_xor(c_cand, off, 64);
-        c_near += __shfl_xor(c_near, off, 64);
-        c_ct += __shfl_xor(c_ct, off, 64);
-        c_ca += __shfl_xor(c_ca, off, 64);
-        c_kick += __shfl_xor(c_kick, off, 64);
-    }
+    c_acc = wave_reduce_u64<false>(c_acc);
+    c_cert = wave_reduce_u64<false>(c_cert);
+    c_cand = wave_reduce_u64<false>(c_cand);
+    c_near = wave_reduce_u64<false>(c_near);
+    c_ct = wave_reduce_u64<false>(c_ct);
+    c_ca = wave_reduce_u64<false>(c_ca);
+    c_kick = wave_reduce_u64<false>(c_kick);
     if ((tid & 63) == 0 && (c_acc | c_cert | c_cand | c_near | c_ct | c_ca | c_kick)) {
         atomicAdd(tot + 0, c_acc);
         atomicAdd(tot + 1, c_cert);
@@ -2297,7 +2320,7 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     //    pairs are disjoint, so each thread owns the slots it swaps)
     if (!improved) {
         const int parity = (int)(interval & 1u);
-        unsigned long long tried = 0, accd = 0;
+        unsigned tried = 0, accd = 0;
         for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
             const int s = 2 * p + parity;
             if ((s % lad) + 1 >= lad) continue;
@@ -2318,14 +2341,11 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                 ++accd;
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            tried += __shfl_xor(tried, off, 64);
-            accd += __shfl_xor(accd, off, 64);
-        }
+        tried = wave_sum_u32(tried);
+        accd = wave_sum_u32(accd);
         if ((tid & 63) == 0 && tried) {
-            atomicAdd(tot + 7, tried);
-            atomicAdd(tot + 8, accd);
+            atomicAdd(tot + 7, (unsigned long long)tried);
+            atomicAdd(tot + 8, (unsigned long long)accd);
         }
     }
 
