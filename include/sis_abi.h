@@ -220,10 +220,11 @@ int sis_batch_run(sis_batch* b, int64_t intervals, int mode, double* sweep_ms, i
 int sis_batch_finalize(sis_batch* b, sis_result* out);
 int sis_batch_destroy(sis_batch* b);
 /* The status of instance k as the host last read it, without a device copy:
- * in mode 0 each exchange step writes its instance's descriptor into mapped
- * host memory, which the plain loop reads for every interval (the
+ * every run copies the descriptors to a host mirror once at its end (the
  * reference's per-epoch blocking read of the chains' best sides,
- * cuda/packing.cu:831); a mode 1 run copies it once at its end. */
+ * cuda/packing.cu:831, reduced to what the host needs between intervals:
+ * in mode 0 each exchange step writes its instance's interval count into
+ * mapped host memory, which the plain loop waits on). */
 struct sis_ladder_desc;  /* declared with the parity probes below */
 int sis_batch_status(sis_batch* b, int k, struct sis_ladder_desc* d);
 

@@ -97,9 +97,8 @@ public:
     hipEvent_t ev_cfork = nullptr;
 
     InstDesc* d_inst = nullptr;
-    InstDesc* h_pin = nullptr;  // mapped host mirror of the descriptors
-    InstDesc* d_pin = nullptr;  // its device address
-    unsigned* h_seq = nullptr;  // mapped host words: interval count of each mirror entry
+    InstDesc* h_pin = nullptr;  // pinned host mirror of the descriptors
+    unsigned* h_seq = nullptr;  // mapped host words: interval count of each instance
     unsigned* d_seq = nullptr;  // their device address
     Tile* d_tiles = nullptr;
     RepState* d_reps = nullptr;

@@ -39,15 +39,13 @@ hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tile
 hipError_t launch_zero64(int n, double L, const float4* q, int* zero, double* e64);
 // list: ninst instance indices into insts, or null for insts[0..ninst).
 // lds: the largest ladder_lds_bytes (n, initial L) over those instances.
-// status, seq: null, or a mirror of insts and one word per instance in
-// mapped host memory: each workgroup that runs a step writes its instance's
-// updated descriptor to status[k] (for insts[k]) and then, with a
-// system-scope release, the descriptor's new interval count to seq[k], so a
-// host that polls seq[k] can read status[k] before the launch has ended.  An
-// instance that has finished its sweeps writes neither.
+// seq: null, or one word per instance in mapped host memory: each workgroup
+// that runs a step writes its instance's new interval count to seq[k] (for
+// insts[k]), so a host that polls seq[k] sees the step's end before the
+// launch has ended.  An instance that has finished its sweeps writes nothing.
 hipError_t launch_ladder(hipStream_t st, int ninst, size_t lds, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
-                         const int* list = nullptr, InstDesc* status = nullptr, unsigned* seq = nullptr);
+                         const int* list = nullptr, unsigned* seq = nullptr);
 hipError_t launch_finalize(hipStream_t st, int njobs, const FinalizeJob* jobs, FinalizeOut* outs,
                            double* q_out);
 // out[k] = insts[list[k]] for k < count (a kernel, so it sees every earlier

@@ -2176,7 +2176,6 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
                                                            int* __restrict__ rep_of_slot,
                                                            float4* __restrict__ state,
                                                            float4* __restrict__ arch,
-                                                           InstDesc* __restrict__ status,
                                                            unsigned* __restrict__ seq) {
     extern __shared__ __align__(16) unsigned char lsm[];
     __shared__ int zero_slot;
@@ -2205,6 +2204,14 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     float* kS = eS + R;
     int* rS = reinterpret_cast<int*>(kS + R);
     unsigned* aS = reinterpret_cast<unsigned*>(rS + R);
+    // The thread's first replica and slot records (a clamped index keeps the
+    // loads in bounds) go out before anything waits, and the uniform of the
+    // first exchange pair the thread owns is drawn while they are in flight.
+    const RepState rs_first = reps[base + min(tid, R - 1)];
+    const SlotState ss_first = slots[base + min(tid, R - 1)];
+    const uint32_t interval = d.interval, key0 = d.key0, key1 = d.key1;
+    const int parity = (int)(interval & 1u);
+    const uint32_t ex_bits = philox((uint32_t)(2 * tid + parity), 0u, interval, 2u, key0, key1).x;
     if (tid == 0) {
         zero_slot = R;
         best_r = R;
@@ -2219,14 +2226,16 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     //    the sweep workgroups move into the instance's totals
     unsigned long long my_lc = ~0ull;
     unsigned long long c_acc = 0, c_cert = 0, c_cand = 0, c_near = 0, c_ct = 0, c_ca = 0, c_kick = 0;
-    // slot record `tid`, kept in registers for step 4 (no kernel step before
+    // slot record `tid` stays in registers for step 4 (no kernel step before
     // step 4 writes slot records)
-    SlotState my_slot{};
     for (int r = tid; r < R; r += kLadderThreads) {
-        const RepState rs = reps[base + r];
-        const SlotState sr = slots[base + r];
+        RepState rs = rs_first;
+        SlotState sr = ss_first;
+        if (r != tid) {
+            rs = reps[base + r];
+            sr = slots[base + r];
+        }
         kS[r] = sr.inv_t_log2e;  // slot r
-        if (r == tid) my_slot = sr;
         c_acc += rs.acc;
         c_cert += (improve && rs.Lc < kCertNone) ? 1u : 0u;
         if (rs.cand | rs.near_pairs | rs.coll_tried | rs.coll_acc | rs.kicks) {
@@ -2270,7 +2279,6 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     __syncthreads();
 
     // 2. improvement: the replica with the smallest certified side
-    const uint32_t interval = d.interval, key0 = d.key0, key1 = d.key1;
     const double L0 = d.L;
     const double shrink = (double)d.shrink, lmin = d.Lmin + 1e-6;
     bool improved = false;
@@ -2319,7 +2327,6 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
     // 3. even/odd exchange of adjacent temperatures inside each ladder (the
     //    pairs are disjoint, so each thread owns the slots it swaps)
     if (!improved) {
-        const int parity = (int)(interval & 1u);
         unsigned tried = 0, accd = 0;
         for (int p = tid; 2 * p + parity + 1 < R; p += kLadderThreads) {
             const int s = 2 * p + parity;
@@ -2327,9 +2334,10 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             const int ra = rS[s], rb = rS[s + 1];
             const float Ea = eS[s], Eb = eS[s + 1];
             const float x = (kS[s] - kS[s + 1]) * (Ea - Eb);
-            const uint4 o = philox((uint32_t)s, 0u, interval, 2u, key0, key1);
+            uint32_t bits = ex_bits;
+            if (p != tid) bits = philox((uint32_t)s, 0u, interval, 2u, key0, key1).x;
             ++tried;
-            if (x >= 0.0f || u01(o.x) < accept_exp2(x)) {
+            if (x >= 0.0f || u01(bits) < accept_exp2(x)) {
                 rS[s] = rb;
                 rS[s + 1] = ra;
                 eS[s] = Eb;
@@ -2359,7 +2367,8 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         double cool, root;
         cool_factor(interval + 1u, d.cool_period, cool, root);
         for (int s = tid; s < R; s += kLadderThreads) {
-            SlotState ss = s == tid ? my_slot : slots[base + s];
+            SlotState ss = ss_first;
+            if (s != tid) ss = slots[base + s];
             const float rate = (float)aS[s] / (float)((long long)n * ran);
             // e^((rate - target) / 2) as 2^((rate - target) log2(e) / 2)
             const float f = accept_exp2((rate - target) * 0.72134752f);
@@ -2434,14 +2443,10 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
         d.ex_tried += tot[7];
         d.ex_acc += tot[8];
         *D = d;
-        if (status) {
-            // the host's mirror (mapped host memory), then the sequence word
-            // the host polls: the interval count, stored with system-scope
-            // release after the mirror, so a host that reads the new count
-            // also reads the whole descriptor
-            status[di] = d;
-            __hip_atomic_store(seq + di, d.interval, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-        }
+        // the interval count the host polls (mapped host memory); the host
+        // reads nothing else of the instance while a run is in flight, so
+        // the store needs no ordering with the descriptor's
+        if (seq) __hip_atomic_store(seq + di, d.interval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -2762,9 +2767,9 @@ hipError_t launch_sweep(hipStream_t st, int ntiles, size_t lds, const Tile* tile
 
 hipError_t launch_ladder(hipStream_t st, int ninst, size_t lds, InstDesc* insts, RepState* reps,
                          SlotState* slots, int* rep_of_slot, float4* state, float4* arch,
-                         const int* list, InstDesc* status, unsigned* seq) {
+                         const int* list, unsigned* seq) {
     hipLaunchKernelGGL(k_ladder, dim3(ninst), dim3(kLadderThreads), lds, st, insts, list, reps, slots,
-                       rep_of_slot, state, arch, status, seq);
+                       rep_of_slot, state, arch, seq);
     return hipGetLastError();
 }
 

@@ -363,7 +363,6 @@ void Batch::release() {
     if (h_pin) (void)hipHostFree(h_pin);
     if (h_seq) (void)hipHostFree(h_seq);
     h_pin = nullptr;
-    d_pin = nullptr;
     h_seq = nullptr;
     d_seq = nullptr;
     for (Chain& c : chains) {
@@ -447,13 +446,11 @@ int Batch::build(std::vector<InstSpec>&& in) {
 
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-    // mapped, coherent host mirror of the descriptors and their sequence
-    // words: in the plain schedule every exchange step writes its instance's
-    // descriptor and then its interval count, which the host polls after
-    // every interval
-    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(InstDesc) * ni,
-                           hipHostMallocMapped | hipHostMallocCoherent));
-    HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_pin), h_pin, 0));
+    // pinned host mirror of the descriptors (copied at the start and the end
+    // of every run), and mapped, coherent sequence words: in the plain
+    // schedule every exchange step writes its instance's interval count,
+    // which the host polls for every interval
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(InstDesc) * ni, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_seq), sizeof(unsigned) * ni,
                            hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_seq), h_seq, 0));
@@ -580,15 +577,16 @@ int Batch::enqueue_interval() {
     return SIS_OK;
 }
 
-// Plain schedule: one interval per launch pair, and the host reads every
-// instance's status after every interval (the reference's per-epoch blocking
-// read, cuda/packing.cu:831).  The exchange step writes each instance's
-// descriptor into the mapped host mirror and then its interval count; the
-// host polls the counts.  Every decision of the epoch loop is made on the
-// device and the number of sweeps each interval runs is known in advance, so
-// the host keeps one interval queued ahead of the one whose status it waits
-// for: the device never idles on the host's round trip, and the host still
-// sees every interval's end, one interval later.  The sweep kernel of every
+// Plain schedule: one interval per launch pair, and the host waits for every
+// interval's end (the reference's per-epoch blocking read,
+// cuda/packing.cu:831).  The exchange step writes each instance's interval
+// count into mapped host memory, which the host polls; the descriptors reach
+// the host mirror by a copy at the start and the end of the run.  Every
+// decision of the epoch loop is made on the device and the number of sweeps
+// each interval runs is known in advance, so the host keeps one interval
+// queued ahead of the one whose end it waits for: the device never idles on
+// the host's round trip, and the host still sees every interval's end, one
+// interval later.  The sweep kernel of every
 // kSweepTimingEvery-th interval is timed with its own event pair on the
 // launch stream.
 int Batch::run_plain(long long intervals) {
@@ -635,7 +633,7 @@ int Batch::run_plain(long long intervals) {
         if (int rc = launch_tiles()) return rc;
         if (timed) HIPCHECK(hipEventRecord(evs1, stream));
         HIPCHECK(launch_ladder(stream, ni, lds_ladder, d_inst, d_reps, d_slots, d_ros, d_state, d_arch, nullptr,
-                               d_pin, d_seq));
+                               d_seq));
         for (int k = 0; k < ni; ++k)
             if (want[k]) {
                 iv_of[k] = want[k];
