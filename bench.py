@@ -427,6 +427,7 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
         L0, q0 = grid_seed(n)
         tickets.append(srv.submit(n, L0, q0, params=params(n, sweeps, R, polish, s)))
     results = {}
+    t_note = time.perf_counter()
     while len(results) < len(tickets):
         for tk in tickets:
             if tk not in results:
@@ -434,6 +435,10 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
                 if r is not None:
                     results[tk] = r
         time.sleep(0.002)
+        if time.perf_counter() - t_note > 20.0:
+            t_note = time.perf_counter()
+            print(f"served: {len(results)}/{len(tickets)} requests done after {t_note - t0:.0f} s",
+                  file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     lat, cert_ok, done_at = [], 0, []
     for tk, (ta, *_rest) in zip(tickets, reqs):
