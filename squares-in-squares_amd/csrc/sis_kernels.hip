@@ -2085,7 +2085,7 @@ __device__ bool overlap_free64(const double* x, const double* y, const double* c
         // few squares: every pair i < j directly, with the arguments total64
         // uses, spread evenly over the threads (flat index over the n x n
         // grid) and without the grid's passes and barriers
-        // (the verdict through the LDS flag, as below: __syncthreads_or
+        // (the result through the LDS flag, as below: __syncthreads_or
         // would reserve static LDS in every kernel that can reach here)
         if (tid == 0) *zg.flag = 0;
         __syncthreads();

@@ -7,8 +7,8 @@ properties hold for the device too.
   blocks of k sweeps under a random grid shift (k = 1 and 4).  Its time
   averages match the Boltzmann distribution exp(-E_wall/T) computed by
   quadrature (the reference's Metropolis step, cuda/packing.cu:80, 123).
-- Per-move energy change (cuda/tests/globaltest.cu:20-38, north-star bar of
-  1e-5 relative): every proposal's FP32 stencil energy change against the
+- Per-move energy change (cuda/tests/globaltest.cu:20-38, within 1e-5
+  relative): every proposal's FP32 stencil energy change against the
   FP64 change over every pair and the wall, from the same FP32 inputs."""
 from __future__ import annotations
 
