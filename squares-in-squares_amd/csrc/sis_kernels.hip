@@ -926,14 +926,19 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
         }
         return other ? en - eo : 0.0f;
     };
+    // Every branch runs its first round on every lane, unguarded: a position
+    // past `last` reads an in-bounds candidate (the clamp) and adds an exact
+    // +0 (never -0), which leaves a partial sum as a skipped round would; the
+    // rounds after the first loop as before.  Most levels need one round.
+    const int lastc = max(last, 0);
     float sum;
     if constexpr (lpi >= 8) {
         // rounds of lpi candidates, f = round + sub; partial k (f = k mod 4)
         // folds lanes k, k + 4 (k + 8, k + 12) of each round into lane k
         float sa = 0.0f;
-        for (int base = 0; base <= last; base += lpi) {
+        auto round = [&](int base) {
             const int f = base + sub;
-            const int k = sc.at(min(f, last));
+            const int k = sc.at(min(f, lastc));
             const float d = delta(t.spos[k], k, f <= last);
             const float d4 = row_up<4>(d);
             if constexpr (lpi == 8) {
@@ -942,13 +947,19 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
                 const float d8 = row_up<8>(d), d12 = row_up<12>(d);
                 sa = (((sa + d) + d4) + d8) + d12;
             }
-        }
+        };
+        round(0);
+        for (int base = lpi; base <= last; base += lpi) round(base);
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
     } else if constexpr (lpi == 4) {
         // f = sub + 4 j: always partial sub
         float s0 = 0.0f;
-        for (int f = sub; f <= last; f += 4) {
+        {
+            const int k = sc.at(min(sub, lastc));
+            s0 += delta(t.spos[k], k, sub <= last);
+        }
+        for (int f = sub + 4; f <= last; f += 4) {
             const int k = sc.at(f);
             s0 += delta(t.spos[k], k, true);
         }
@@ -957,7 +968,13 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
     } else if constexpr (lpi == 2) {
         // f = sub + 2 j: partial sub for even j, sub + 2 for odd j
         float s0 = 0.0f, s2 = 0.0f;
-        for (int f = sub; f <= last; f += 4) {
+        {
+            const int k = sc.at(min(sub, lastc));
+            const int k2 = sc.at(min(sub + 2, lastc));
+            s0 += delta(t.spos[k], k, sub <= last);
+            s2 += delta(t.spos[k2], k2, sub + 2 <= last);
+        }
+        for (int f = sub + 4; f <= last; f += 4) {
             const int k = sc.at(f);
             s0 += delta(t.spos[k], k, true);
             if (f + 2 <= last) {
@@ -969,7 +986,15 @@ __device__ __forceinline__ void rec_level(const TileLds& t, const MoverRec& r, b
     } else {
         // one lane: partial f mod 4 of candidate f
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        for (int f = 0; f <= last; f += 4) {
+        {
+            const int k0 = sc.at(0), k1 = sc.at(min(1, lastc)), k2 = sc.at(min(2, lastc)),
+                      k3 = sc.at(min(3, lastc));
+            s0 += delta(t.spos[k0], k0, 0 <= last);
+            s1 += delta(t.spos[k1], k1, 1 <= last);
+            s2 += delta(t.spos[k2], k2, 2 <= last);
+            s3 += delta(t.spos[k3], k3, 3 <= last);
+        }
+        for (int f = 4; f <= last; f += 4) {
             const int k0 = sc.at(f);
             s0 += delta(t.spos[k0], k0, true);
             if (f + 1 <= last) {

@@ -172,6 +172,39 @@ def test_sweeps_match_cpu_model_for_any_shift_period(sis, oracle, n, L, R, group
     assert np.array_equal(dcnt, mcnt)
 
 
+@pytest.mark.parametrize("cols,rows,R,group", [(8, 8, 2, 1), (8, 5, 2, 1), (6, 4, 3, 1), (4, 3, 4, 1),
+                                              (4, 4, 4, 2), (8, 4, 2, 2)])
+def test_sweeps_match_cpu_model_with_crowded_levels(sis, oracle, cols, rows, R, group):
+    """Squares on a lattice of pitch three cells all share one colour class
+    under any grid shift, so one level holds (nearly) every square of a tile:
+    12 to 64 movers in one level of a one-wavefront tile with mover records,
+    which runs the record pass at 4, 2 and 1 lanes per mover (random packings
+    of these sizes never have more than about 15).  Device == model bit for
+    bit."""
+    w = 0.7075
+    pitch = 3.0 * w
+    n = cols * rows
+    L = 2 * 1.2 + pitch * (max(cols, rows) - 1)
+    rng = np.random.default_rng(900 + n + group)
+    gx, gy = np.meshgrid(1.2 + pitch * np.arange(cols), 1.2 + pitch * np.arange(rows))
+    x = np.broadcast_to(gx.ravel(), (R, n)) + rng.uniform(-0.01, 0.01, (R, n))
+    y = np.broadcast_to(gy.ravel(), (R, n)) + rng.uniform(-0.01, 0.01, (R, n))
+    a = rng.uniform(0, math.pi / 2, (R, n))
+    st = np.ascontiguousarray(np.stack([x, y, np.cos(a), np.sin(a)], axis=-1).astype(np.float32))
+    T = np.geomspace(1e-3, 1e-1, R)
+    ik = (LOG2E / T).astype(np.float32)
+    # short steps: the squares stay near the lattice, so the levels stay crowded
+    sxy = np.full(R, 0.02, np.float32)
+    sth = np.full(R, 0.1, np.float32)
+    say("sweeps crowded levels", cols, rows, R, group)
+    dst, dacc, den, dcnt = sis.dev_sweeps(st, L, w, ik, sxy, sth, 7, 0, 3, 6, group=group)
+    mst, macc, men, mcnt = _model_run(oracle, st, L, w, ik, sxy, sth, 7, 0, 3, 6)
+    assert np.array_equal(dacc, macc)
+    assert np.array_equal(dst, mst)
+    assert np.array_equal(den, men)
+    assert np.array_equal(dcnt, mcnt)
+
+
 @pytest.mark.parametrize("n,L,R,sweeps,group,w", [(1, 1.5, 4, 5, 1, 0.7075), (2, 2.5, 4, 5, 2, 0.7075),
                                                   (7, 5.0, 3, 4, 1, 2.0), (30, 8.0, 2, 3, 1, 4.0),
                                                   (1000, 64.0, 1, 1, 1, 0.7075)])
