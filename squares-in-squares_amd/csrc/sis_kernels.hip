@@ -606,36 +606,42 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
             sum = h + quad_xor2(h);
         }
     } else if (lpi == 1) {
+        // (every four-candidate branch runs its first round unguarded: quad()
+        // clamps the positions past `last` and masks them to an exact +0,
+        // which leaves a partial as a skipped round would)
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        for (int f = 0; f <= last; f += 4) {
+        for (int f = 0;; f += 4) {
             quad(f, f + 1, f + 2, f + 3, d0, d1, d2, d3);
             s0 += d0;
             s1 += d1;
             s2 += d2;
             s3 += d3;
+            if (f + 4 > last) break;
         }
         sum = (s0 + s1) + (s2 + s3);
     } else if (lpi == 2) {
         // sub 0 owns partials 0, 1; sub 1 owns 2, 3
         float sa = 0.0f, sb = 0.0f;
-        for (int f = 2 * sub; f <= last; f += 8) {
+        for (int f = 2 * sub;; f += 8) {
             quad(f, f + 1, f + 4, f + 5, d0, d1, d2, d3);
             sa += d0;
             sb += d1;
             sa += d2;
             sb += d3;
+            if (f + 8 > last) break;
         }
         const float h = sa + sb;
         sum = h + quad_xor1(h);
     } else if (lpi == 4) {
         // sub k owns partial k
         float sa = 0.0f;
-        for (int f = sub; f <= last; f += 16) {
+        for (int f = sub;; f += 16) {
             quad(f, f + 4, f + 8, f + 12, d0, d1, d2, d3);
             sa += d0;
             sa += d1;
             sa += d2;
             sa += d3;
+            if (f + 16 > last) break;
         }
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
@@ -646,7 +652,7 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
         // order into lane k; the lanes of a mover are aligned inside one row
         // of 16 lanes.
         float sa = 0.0f;
-        for (int base = 0; base <= last; base += lpi) {
+        for (int base = 0;; base += lpi) {
             const int f = base + sub;
             const int k = sc.at(min(f, last));
             const float4 v = t.spos ? t.spos[k] : pg[t.sorted[k]];
@@ -658,6 +664,7 @@ __device__ __forceinline__ void evaluate(const TileLds& t, int q, int g, int sel
                 const float d8 = row_up<8>(d), d12 = row_up<12>(d);
                 sa = (((sa + d) + d4) + d8) + d12;
             }
+            if (base + lpi > last) break;  // the first round runs unguarded (last >= 0)
         }
         const float h = sa + quad_xor1(sa);
         sum = h + quad_xor2(h);
