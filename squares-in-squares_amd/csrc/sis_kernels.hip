@@ -2328,22 +2328,50 @@ __global__ __launch_bounds__(kLadderThreads) void k_ladder(InstDesc* __restrict_
             Lb = rs.Lc;
             const float box_b = slots[base + rs.slot].L;
             const float4* src = state + d.state_off + (size_t)rb * n;
-            for (int i = tid; i < n; i += kLadderThreads) arch[d.arch_off + i] = src[i];
-            __syncthreads();
             // restarts: slot 0 of every ladder, and the whole first ladder on
             // a large gain (every slot's box below the new best side is set
-            // with the slot records in step 4)
+            // with the slot records in step 4).  The scale of each restarted
+            // slot goes to LDS (the energies and 1/T by slot are not read
+            // again on an improving interval); every square of the best
+            // configuration is read once, archived, and written to the
+            // restarted replicas, which the threads holding copies of that
+            // square share out.
             const int heads = R / lad, extra = (L0 - Lb > 5e-4) ? lad - 1 : 0;
-            const float4* best = arch + d.arch_off;
-            for (int k = tid; k < (heads + extra) * n; k += kLadderThreads) {
-                const int h = k / n, i = k - h * n;
-                const int s = h < heads ? h * lad : h - heads + 1;
+            const int H = heads + extra;
+            double* scS = reinterpret_cast<double*>(eS);  // eS and kS: 8 R bytes, H <= R
+            auto slot_of = [&](int h) { return h < heads ? h * lad : h - heads + 1; };
+            for (int h = tid; h < H; h += kLadderThreads) {
+                const int s = slot_of(h);
                 const double tL = fmax(lmin, Lb - shrink * kTargetMults[((s % lad) * 8) / lad]);
-                const double sc = tL / Lb;
-                const float4 v = best[i];
-                float4* dst = state + d.state_off + (size_t)rS[s] * n;
-                dst[i] = make_float4((float)(((double)v.x * lam - mx) * sc), (float)(((double)v.y * lam - my) * sc),
-                                     v.z, v.w);
+                scS[h] = tL / Lb;
+            }
+            auto restart = [&](int i, const float4& v, int h0, int step) {
+                for (int h = h0; h < H; h += step) {
+                    const double sc = scS[h];
+                    float4* dst = state + d.state_off + (size_t)rS[slot_of(h)] * n;
+                    dst[i] = make_float4((float)(((double)v.x * lam - mx) * sc),
+                                         (float)(((double)v.y * lam - my) * sc), v.z, v.w);
+                }
+            };
+            if (n <= kLadderThreads) {
+                // kLadderThreads / n copies of every square; all reads of the
+                // best replica precede the barrier, since it may be restarted
+                const int copies = kLadderThreads / n, c = tid / n, i = tid - c * n;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (c < copies) v = src[i];
+                __syncthreads();
+                if (c < copies) {
+                    if (c == 0) arch[d.arch_off + i] = v;
+                    restart(i, v, c, copies);
+                }
+            } else {
+                // each square read and written by one thread only
+                __syncthreads();
+                for (int i = tid; i < n; i += kLadderThreads) {
+                    const float4 v = src[i];
+                    arch[d.arch_off + i] = v;
+                    restart(i, v, 0, 1);
+                }
             }
             d.L = Lb;
             d.Lf = (float)Lb;

@@ -95,6 +95,8 @@ def _randomise(rng, desc, reps, st, step):
     (20, 32, 8, 1, 7, 5.2),      # improvement path with cooling
     (120, 24, 24, 0, 200, 11.2),  # one ladder, grid FP64 test (n >= 96)
     (50, 16, 16, 1, 0, 8.2),     # one ladder, improvement path, gains above 5e-4
+    (130, 32, 32, 1, 0, 12.2),   # restarts with one thread per square
+    (300, 16, 16, 1, 0, 18.2),   # restarts with squares looped over the threads (n > 256)
 ])
 def test_ladder_kernel_matches_model(sis, oracle, n, R, lad, improve, cool, L):
     rng, desc, reps, slots, ros, st, arch = _ladder_case(n, R, lad, improve, cool, L, 100 + n + improve)
