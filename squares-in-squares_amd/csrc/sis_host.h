@@ -51,6 +51,9 @@ public:
     // of a 122 us interval at n = 50 (1.04e9 against 1.08e9 proposals/s with
     // every eighth launch timed, 1.09e9 with none).
     static constexpr int kSweepTimingEvery = 8;
+    // the plain loop reads a timed interval's event pair one interval later,
+    // before the pair is recorded again
+    static_assert(kSweepTimingEvery >= 2, "one event pair per timed interval");
     int run_plain(long long intervals);
     int wait_status(const std::vector<unsigned>& want);
     int run_graph(int per_graph, long long launches);
