@@ -97,6 +97,8 @@ def _randomise(rng, desc, reps, st, step):
     (50, 16, 16, 1, 0, 8.2),     # one ladder, improvement path, gains above 5e-4
     (130, 32, 32, 1, 0, 12.2),   # restarts with one thread per square
     (300, 16, 16, 1, 0, 18.2),   # restarts with squares looped over the threads (n > 256)
+    (12, 600, 8, 1, 0, 4.2),     # R > 2 x 256: several replicas, slots and pairs per thread
+    (12, 600, 24, 0, 5, 4.2),    # the same with the zero-overlap shrink
 ])
 def test_ladder_kernel_matches_model(sis, oracle, n, R, lad, improve, cool, L):
     rng, desc, reps, slots, ros, st, arch = _ladder_case(n, R, lad, improve, cool, L, 100 + n + improve)
@@ -115,7 +117,8 @@ def test_ladder_kernel_matches_model(sis, oracle, n, R, lad, improve, cool, L):
         zero_tests += (od["fp64_rejects"] + od["shrinks"]) > (desc["fp64_rejects"] + desc["shrinks"])
         desc, reps, slots, ros, st, arch = od, orr, osl, oros, ost, oar
         desc["Lf"] = np.float32(desc["Lf"])
-    assert desc["interval"] == 60 and desc["ex_tried"] > 0
+    # an interval that improves does no exchange (with 600 replicas every one does)
+    assert desc["interval"] == 60 and (desc["ex_tried"] > 0 or improved == 60)
     if improve:
         assert improved >= 3
     else:
