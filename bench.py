@@ -10,9 +10,9 @@ side, per-temperature targets best - 1e-4 * mults, ladder heads restarted
 from the best configuration), final FP64 polish and certified-side export.
 
 One step = 1000 sweeps of every replica (100 exchange intervals of 10 sweeps,
-run in the plain form: one launch pair per interval, the host reading every
-interval's status with the next interval already queued).  One sweep = n proposals per replica, so one step is
-256 * 50 * 1000 = 1.28e7 proposals.  --steps K times exactly K steps after
+run in the plain form: one launch pair per interval, the host waiting for every
+interval's end with the next interval already queued).  One sweep = n proposals
+per replica, so one step is 256 * 50 * 1000 = 1.28e7 proposals.  --steps K times exactly K steps after
 --warmup W untimed steps.
 
 Prints one JSON line: proposals/s, the sweep kernel's roofline entry (FP32
@@ -278,7 +278,7 @@ def precision_leg(sis, sweeps: int = 10000):
 def batched_leg(sis, n_inst: int, sweeps: int, plain_sample: int, autotune: bool = True):
     """Mixed-n stream (512 instances, n ~ U{20..200} with seed 3, 64 replicas
     each): all instances in one HIP-graph schedule vs the plain form (one
-    instance per launch sequence, host status read per interval) on a sample of the
+    instance per launch sequence, the host waiting for every interval) on a sample of the
     same stream.  Returns proposals/s of both and the sweep-kernel roofline of
     the batched form; with the autotuner, also the rate and search quality of
     the whole stream under the untuned settings."""
@@ -393,7 +393,7 @@ def served_leg(sis, rate: float, duration: float, sweep_scale: float, plain_seco
     half without; every request starts from its certified grid seed, runs the
     reference's improvement path and is certified exactly.  Reports p50/p99
     time to a certified L and aggregate proposals/s, and the plain form
-    (requests one at a time, host status read per interval) on the same requests,
+    (requests one at a time, the host waiting for every interval) on the same requests,
     about `plain_seconds` split over three size classes (n < 64, 64-255,
     >= 256), each class in arrival order.  Returns the line and the packings."""
     import numpy as np
@@ -668,7 +668,7 @@ def main():
                         "interaction cells of side sqrt(2) realised as 5x5 stencils of colour cells of side "
                         "0.7075",
             "n": N, "replicas": REPLICAS, "L0": L0, "sweeps_per_step": SWEEPS_PER_STEP,
-            "sweeps_timed": a.steps * SWEEPS_PER_STEP, "schedule": "plain (host reads every interval's status, one interval queued ahead)",
+            "sweeps_timed": a.steps * SWEEPS_PER_STEP, "schedule": "plain (host waits for every interval's end, one interval queued ahead)",
             "device": dev, "compute_units": cus,
         },
         "roofline": {

@@ -257,7 +257,7 @@ def portfolio(n: int, L: float, q=None, params: SisParams | None = None, jitter:
 
 
 def solve(n: int, L: float, q=None, params: SisParams | None = None, **kw) -> Result:
-    """Plain form: one instance, the host reads its status after every exchange
+    """Plain form: one instance, the host waits for the end of every exchange
     interval (the next interval already queued)."""
     p = params if params is not None else default_params(**kw)
     inst, keep = _instance(n, L, q)

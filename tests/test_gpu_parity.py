@@ -500,8 +500,8 @@ def test_batch_equals_alone(sis):
 
 
 def test_plain_runs_stop_at_the_budget_and_read_status(sis):
-    """The plain schedule reads every instance's status for every interval
-    (one interval queued ahead):
+    """The plain schedule waits for every interval's end (one interval queued
+    ahead) and reads the instances' status at the end of a run:
     a run longer than the budget stops at it, a run after the end launches
     nothing, the status the host read equals the device's descriptor, every
     eighth launch of a call is timed, and stopping early changes no result
